@@ -1,0 +1,168 @@
+/*
+ * primus_hip.h -- C ABI of libprimus_hip.so: Primus's negacyclic NTT tables
+ * and the transform pipelines over them, running on an AMD MI355X (gfx950).
+ *
+ * The entry points mirror the reference's Rust traits (serendipity-crypto/primus):
+ *   trait NttTable   primus_ntt/src/ntt/mod.rs:20-117, impl UintNttTable
+ *                    primus_ntt/src/ntt/primitive.rs:110-398
+ *   trait DcrtTable  primus_ntt/src/dcrt/mod.rs:17-129, impl UintCrtNttTable
+ *                    primus_ntt/src/dcrt/primitive.rs:18-142
+ * with the reduction semantics of primus_reduce / primus_modulus (Barrett) and
+ * primus_factor (Shoup).  Each declaration cites the reference item it replaces.
+ *
+ * Conventions
+ *   - Every fallible entry point returns an int status (PH_OK = 0).  Codes 1-4
+ *     mirror NttError (primus_ntt/src/error.rs:7-35); the reference panics on a
+ *     bad modulus (primus_modulus/src/barrett/mod.rs:55-60) and debug-asserts
+ *     slice lengths, both reported here as PH_ERR_INVALID.  Nothing aborts.
+ *   - Tables are immutable after creation and may be shared between threads
+ *     (the Rust traits are Send + Sync).  Their twiddles live in device memory.
+ *   - Host-slice entry points (`*_slice`, `*_monomial`) take host memory, run on
+ *     the GPU and return when the result is back in the slice.
+ *   - Device entry points (`*_device`) take device pointers and a HIP stream
+ *     (hipStream_t passed as void*, NULL = the default stream); they only
+ *     enqueue work.
+ *   - Coefficients are uint64_t; multi-limb data is limb-major [L][N]
+ *     (primus_poly/src/dcrt/mod.rs:70-76).  In the batched device forms
+ *     limb-polynomial i of a batch uses modulus i % L.
+ *   - Forward transforms take normal order to bit-reversed order, inverse
+ *     transforms the reverse; slot k of a forward output is the evaluation at
+ *     psi^(2*rev(k)+1), psi the smallest primitive 2N-th root of unity.
+ *   - Supported ring degrees: N = 2^1 .. 2^17; modulus 1 < q < 2^62.
+ */
+#ifndef PRIMUS_HIP_H
+#define PRIMUS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PH_OK 0
+#define PH_ERR_NO_PRIMITIVE_ROOT 1 /* NttError::NoPrimitiveRoot     primus_ntt/src/error.rs:9 */
+#define PH_ERR_DEGREE_CONVERSION 2 /* NttError::DegreeConversionErr primus_ntt/src/error.rs:17 */
+#define PH_ERR_DEGREE_TOO_LARGE 3  /* NttError::DegreeTooLarge      primus_ntt/src/error.rs:25 */
+#define PH_ERR_NTT_TABLE 4         /* NttError::NttTableErr         primus_ntt/src/error.rs:33 */
+#define PH_ERR_INVALID 5           /* bad argument: modulus, length, range, null pointer */
+#define PH_ERR_DEVICE 6            /* a HIP call failed */
+#define PH_ERR_UNSUPPORTED 7       /* ring degree outside 2^1 .. 2^17 */
+#define PH_ERR_NO_MEMORY 8         /* host or device allocation failed */
+
+typedef struct ph_ntt_table ph_ntt_table;   /* UintNttTable<u64> */
+typedef struct ph_dcrt_table ph_dcrt_table; /* UintCrtNttTable<u64> */
+typedef struct ph_operand ph_operand;       /* device-resident Shoup-prepared NTT-domain operand */
+
+/* ---- diagnostics ---- */
+const char *ph_status_string(int status);
+/* Text of the last failure on the calling thread ("" if none). */
+const char *ph_last_error(void);
+
+/* ---- NttTable ---- */
+
+/* NttTable::new -> UintNttTable::new (primus_ntt/src/ntt/primitive.rs:114-185):
+ * psi = minimal primitive 2N-th root (primus_ntt/src/root.rs:89-111), Shoup
+ * twiddle tables, N^-1.  `modulus` plays BarrettModulus::new (q in (1, 2^62)). */
+int ph_ntt_table_new(uint32_t log_n, uint64_t modulus, ph_ntt_table **out);
+void ph_ntt_table_free(ph_ntt_table *table);
+
+/* NttTable::poly_length (primitive.rs:188-190) */
+size_t ph_ntt_table_poly_length(const ph_ntt_table *table);
+/* UintNttTable::{log_n, modulus, root, inv_root} (primitive.rs:51-71) */
+uint32_t ph_ntt_table_log_n(const ph_ntt_table *table);
+uint64_t ph_ntt_table_modulus(const ph_ntt_table *table);
+uint64_t ph_ntt_table_root(const ph_ntt_table *table);
+uint64_t ph_ntt_table_inv_root(const ph_ntt_table *table);
+/* UintNttTable::inv_n as ShoupFactor {value, quotient} (primitive.rs:81-83) */
+int ph_ntt_table_inv_n(const ph_ntt_table *table, uint64_t *value, uint64_t *quotient);
+/* UintNttTable::{root_powers (which=0, N), inv_root_powers (1, N),
+ * ordinal_root_powers (2, 2N)} (primitive.rs:87-101); either output may be NULL. */
+int ph_ntt_table_powers(const ph_ntt_table *table, int which, uint64_t *values, uint64_t *quotients);
+
+/* NttTable::lazy_transform_slice: [0,4q) -> [0,4q) (primitive.rs:211-232) */
+int ph_lazy_transform_slice(const ph_ntt_table *table, uint64_t *poly, size_t len);
+/* NttTable::transform_slice: [0,4q) -> [0,q) (primitive.rs:234-244) */
+int ph_transform_slice(const ph_ntt_table *table, uint64_t *poly, size_t len);
+/* NttTable::lazy_inverse_transform_slice: [0,2q) -> [0,q) for this table (primitive.rs:246-286) */
+int ph_lazy_inverse_transform_slice(const ph_ntt_table *table, uint64_t *values, size_t len);
+/* NttTable::inverse_transform_slice: [0,2q) -> [0,q) (primitive.rs:288-296) */
+int ph_inverse_transform_slice(const ph_ntt_table *table, uint64_t *values, size_t len);
+/* NttTable::transform_monomial (primitive.rs:298-347) */
+int ph_transform_monomial(const ph_ntt_table *table, uint64_t coeff, size_t degree, uint64_t *values,
+                          size_t len);
+/* NttTable::transform_coeff_one_monomial (primitive.rs:349-372) */
+int ph_transform_coeff_one_monomial(const ph_ntt_table *table, size_t degree, uint64_t *values,
+                                    size_t len);
+/* NttTable::transform_coeff_minus_one_monomial (primitive.rs:374-397) */
+int ph_transform_coeff_minus_one_monomial(const ph_ntt_table *table, size_t degree, uint64_t *values,
+                                          size_t len);
+
+/* ---- DcrtTable ---- */
+
+/* DcrtTable::new -> UintCrtNttTable::new (primus_ntt/src/dcrt/primitive.rs:24-43) */
+int ph_dcrt_table_new(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_count, ph_dcrt_table **out);
+void ph_dcrt_table_free(ph_dcrt_table *table);
+/* DcrtTable::{poly_length, moduli_count, crt_poly_length} (dcrt/primitive.rs:57-70) */
+size_t ph_dcrt_poly_length(const ph_dcrt_table *table);
+size_t ph_dcrt_moduli_count(const ph_dcrt_table *table);
+size_t ph_dcrt_crt_poly_length(const ph_dcrt_table *table);
+/* DcrtTable::ntt_tables()[i] (dcrt/primitive.rs:47-49); owned by the DCRT table */
+const ph_ntt_table *ph_dcrt_ntt_table(const ph_dcrt_table *table, uint32_t i);
+/* DcrtTable::{lazy_,}transform_slice, {lazy_,}inverse_transform_slice over
+ * limb-major [L][N] host slices (dcrt/primitive.rs:107-141) */
+int ph_dcrt_lazy_transform_slice(const ph_dcrt_table *table, uint64_t *poly, size_t len);
+int ph_dcrt_transform_slice(const ph_dcrt_table *table, uint64_t *poly, size_t len);
+int ph_dcrt_lazy_inverse_transform_slice(const ph_dcrt_table *table, uint64_t *poly, size_t len);
+int ph_dcrt_inverse_transform_slice(const ph_dcrt_table *table, uint64_t *poly, size_t len);
+
+/* ---- batched device forms ---- */
+
+/* `count` limb-polynomials at d_polys; limb-polynomial i uses modulus i % L.
+ * The slice semantics of transform_slice (lazy = 0) / lazy_transform_slice (lazy = 1)
+ * and inverse_transform_slice, applied to every limb-polynomial in one launch. */
+int ph_dcrt_transform_device(const ph_dcrt_table *table, uint64_t *d_polys, size_t count, int lazy,
+                             void *stream);
+int ph_dcrt_inverse_transform_device(const ph_dcrt_table *table, uint64_t *d_polys, size_t count,
+                                     void *stream);
+/* Single-modulus conveniences (L = 1). */
+int ph_ntt_transform_device(const ph_ntt_table *table, uint64_t *d_polys, size_t count, int lazy,
+                            void *stream);
+int ph_ntt_inverse_transform_device(const ph_ntt_table *table, uint64_t *d_polys, size_t count,
+                                    void *stream);
+
+/* Device-resident NTT-domain operand, one canonical [0,q_i) row of N per limb
+ * ([L][N] host array), held with its Shoup quotients as ShoupFactor values
+ * (primus_factor/src/shoup_factor/mod.rs:44-55). */
+int ph_operand_new(const ph_dcrt_table *table, const uint64_t *host_values, ph_operand **out);
+void ph_operand_free(ph_operand *op);
+
+/* Fused forward transform -> Shoup multiply by `op` -> inverse transform of each
+ * limb-polynomial, in place, inputs in [0,4q), outputs in [0,q).  Equals
+ * transform_slice, NttPolynomial::mul_assign (primus_poly/src/ntt/mul.rs:87-95),
+ * inverse_transform_slice on every limb. */
+int ph_dcrt_fwd_mul_inv_device(const ph_dcrt_table *table, uint64_t *d_polys, size_t count,
+                               const ph_operand *op, void *stream);
+/* Negacyclic product d_a <- d_a * d_b mod (X^N + 1, q_i) per limb-polynomial,
+ * inputs in [0,q); the NTT route to Polynomial::naive_mul_inplace
+ * (primus_poly/src/poly/mul.rs:111-142). */
+int ph_dcrt_negacyclic_mul_device(const ph_dcrt_table *table, uint64_t *d_a, const uint64_t *d_b,
+                                  size_t count, void *stream);
+/* NttPolynomial::mul_inplace (primus_poly/src/ntt/mul.rs:105-119): d_out = d_a * d_b. */
+int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *table, const uint64_t *d_a, const uint64_t *d_b,
+                                 uint64_t *d_out, size_t count, void *stream);
+/* NttPolynomial::add_mul_assign (primus_poly/src/ntt/mod.rs:97-108): d_acc += d_a * d_b. */
+int ph_dcrt_pointwise_mul_add_device(const ph_dcrt_table *table, uint64_t *d_acc, const uint64_t *d_a,
+                                     const uint64_t *d_b, size_t count, void *stream);
+
+/* ---- device memory helpers (for callers without their own HIP allocator) ---- */
+int ph_device_alloc(size_t bytes, void **d_ptr);
+int ph_device_free(void *d_ptr);
+int ph_memcpy_htod(void *d_dst, const void *h_src, size_t bytes);
+int ph_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes);
+int ph_stream_synchronize(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRIMUS_HIP_H */

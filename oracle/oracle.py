@@ -1,0 +1,223 @@
+"""ctypes view of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU
+baseline, never as the product path.  The C sources restate the reference's
+UintNttTable<u64> (primus_ntt/src/ntt/primitive.rs) and the Shoup/Barrett
+arithmetic under it; see oracle/primus_oracle.c for the citations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_u64 = ctypes.c_uint64
+_p64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+def build():
+    """Compile oracle/liboracle.so with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        for name, res, args in [
+            ("po_shoup_quotient", _u64, [_u64, _u64]),
+            ("po_shoup_lazy_mul", _u64, [_u64, _u64, _u64, _u64]),
+            ("po_shoup_mul", _u64, [_u64, _u64, _u64, _u64]),
+            ("po_multiply_factor", _u64, [_u64, ctypes.c_uint32, _u64]),
+            ("po_multiply_mod_lazy", _u64, [_u64, _u64, _u64, ctypes.c_uint32, _u64]),
+            ("po_barrett_ratio", ctypes.c_int, [_u64, _p64]),
+            ("po_barrett_lazy_reduce", _u64, [_u64, _u64, _p64]),
+            ("po_barrett_lazy_reduce_2limb", _u64, [_u64, _u64, _u64, _p64]),
+            ("po_barrett_reduce_mul", _u64, [_u64, _u64, _u64, _p64]),
+            ("po_barrett_reduce_mul_add", _u64, [_u64, _u64, _u64, _u64, _p64]),
+            ("po_pow_mod", _u64, [_u64, _u64, _u64]),
+            ("po_inv_mod", _u64, [_u64, _u64]),
+            ("po_is_primitive_root", ctypes.c_int, [_u64, ctypes.c_uint32, _u64]),
+            ("po_minimal_primitive_root", ctypes.c_int, [ctypes.c_uint32, _u64, _p64]),
+            ("po_reverse_lsbs", _u64, [_u64, ctypes.c_uint32]),
+            ("po_table_new", ctypes.c_int, [ctypes.c_uint32, _u64, ctypes.POINTER(ctypes.c_void_p)]),
+            ("po_table_free", None, [ctypes.c_void_p]),
+            ("po_table_n", ctypes.c_size_t, [ctypes.c_void_p]),
+            ("po_table_modulus", _u64, [ctypes.c_void_p]),
+            ("po_table_root", _u64, [ctypes.c_void_p]),
+            ("po_table_inv_root", _u64, [ctypes.c_void_p]),
+            ("po_table_inv_n", None, [ctypes.c_void_p, _p64, _p64]),
+            ("po_table_powers", None, [ctypes.c_void_p, ctypes.c_int, _p64, _p64]),
+            ("po_lazy_transform", None, [ctypes.c_void_p, _p64]),
+            ("po_transform", None, [ctypes.c_void_p, _p64]),
+            ("po_lazy_inverse_transform", None, [ctypes.c_void_p, _p64]),
+            ("po_inverse_transform", None, [ctypes.c_void_p, _p64]),
+            ("po_transform_monomial", None, [ctypes.c_void_p, _u64, ctypes.c_size_t, _p64]),
+            ("po_transform_coeff_one_monomial", None, [ctypes.c_void_p, ctypes.c_size_t, _p64]),
+            ("po_transform_coeff_minus_one_monomial", None, [ctypes.c_void_p, ctypes.c_size_t, _p64]),
+            ("po_pointwise_mul", None, [_p64, _p64, _p64, ctypes.c_size_t, _u64]),
+            ("po_pointwise_mul_add", None, [_p64, _p64, _p64, ctypes.c_size_t, _u64]),
+            ("po_shoup_pointwise", None, [_p64, _p64, _p64, ctypes.c_size_t, _u64]),
+            ("po_schoolbook_negacyclic", None, [_p64, _p64, _p64, ctypes.c_size_t, _u64]),
+            ("po_fwd_mul_inv_batch", None,
+             [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32, _p64, ctypes.c_size_t, _p64, _p64, ctypes.c_int]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_p64)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code):
+        super().__init__("oracle status %d" % code)
+        self.code = code
+
+
+def barrett_ratio(q):
+    r = np.zeros(2, dtype=np.uint64)
+    st = lib().po_barrett_ratio(q, ptr(r))
+    if st:
+        raise OracleError(st)
+    return r
+
+
+def minimal_primitive_root(log_degree, q):
+    out = _u64(0)
+    st = lib().po_minimal_primitive_root(log_degree, q, ctypes.byref(out))
+    if st:
+        raise OracleError(st)
+    return out.value
+
+
+class Table:
+    """UintNttTable<u64> restated (primus_ntt/src/ntt/primitive.rs:34-398)."""
+
+    def __init__(self, log_n, q):
+        h = ctypes.c_void_p()
+        st = lib().po_table_new(log_n, q, ctypes.byref(h))
+        if st:
+            raise OracleError(st)
+        self._h = h
+        self.log_n = log_n
+        self.n = 1 << log_n
+        self.q = q
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.po_table_free(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def root(self):
+        return lib().po_table_root(self._h)
+
+    @property
+    def inv_root(self):
+        return lib().po_table_inv_root(self._h)
+
+    @property
+    def inv_n(self):
+        v, w = _u64(0), _u64(0)
+        lib().po_table_inv_n(self._h, ctypes.byref(v), ctypes.byref(w))
+        return v.value, w.value
+
+    def powers(self, which):
+        cnt = 2 * self.n if which == 2 else self.n
+        v = np.zeros(cnt, dtype=np.uint64)
+        w = np.zeros(cnt, dtype=np.uint64)
+        lib().po_table_powers(self._h, which, ptr(v), ptr(w))
+        return v, w
+
+    def _run(self, fn, x):
+        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        assert y.shape[-1] == self.n
+        flat = y.reshape(-1, self.n)
+        for row in flat:
+            fn(self._h, ptr(row))
+        return y
+
+    def transform(self, x):
+        return self._run(lib().po_transform, x)
+
+    def lazy_transform(self, x):
+        return self._run(lib().po_lazy_transform, x)
+
+    def inverse_transform(self, x):
+        return self._run(lib().po_inverse_transform, x)
+
+    def lazy_inverse_transform(self, x):
+        return self._run(lib().po_lazy_inverse_transform, x)
+
+    def transform_monomial(self, coeff, degree):
+        out = np.zeros(self.n, dtype=np.uint64)
+        lib().po_transform_monomial(self._h, coeff, degree, ptr(out))
+        return out
+
+    def transform_coeff_one_monomial(self, degree):
+        out = np.zeros(self.n, dtype=np.uint64)
+        lib().po_transform_coeff_one_monomial(self._h, degree, ptr(out))
+        return out
+
+    def transform_coeff_minus_one_monomial(self, degree):
+        out = np.zeros(self.n, dtype=np.uint64)
+        lib().po_transform_coeff_minus_one_monomial(self._h, degree, ptr(out))
+        return out
+
+
+def pointwise_mul(a, b, q):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.zeros_like(a)
+    lib().po_pointwise_mul(ptr(a), ptr(b), ptr(out), a.size, q)
+    return out
+
+
+def pointwise_mul_add(acc, a, b, q):
+    acc = np.ascontiguousarray(acc, dtype=np.uint64).copy()
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    lib().po_pointwise_mul_add(ptr(acc), ptr(a), ptr(b), a.size, q)
+    return acc
+
+
+def shoup_quotients(w, q):
+    L = lib()
+    return np.array([L.po_shoup_quotient(int(x), q) for x in np.asarray(w).ravel()],
+                    dtype=np.uint64).reshape(np.shape(w))
+
+
+def schoolbook(a, b, q):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    c = np.zeros_like(a)
+    lib().po_schoolbook_negacyclic(ptr(a), ptr(b), ptr(c), a.size, q)
+    return c
+
+
+def fwd_mul_inv_batch(tables, polys, operand, operand_q, threads=1):
+    """Forward NTT, Shoup pointwise multiply, inverse NTT for each limb-poly in place."""
+    L = len(tables)
+    arr = (ctypes.c_void_p * L)(*[t.handle for t in tables])
+    n = tables[0].n
+    assert polys.dtype == np.uint64 and polys.flags["C_CONTIGUOUS"]
+    count = polys.size // n
+    lib().po_fwd_mul_inv_batch(arr, L, ptr(polys), count, ptr(operand), ptr(operand_q), threads)
+    return polys

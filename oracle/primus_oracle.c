@@ -1,0 +1,483 @@
+/*
+ * primus_oracle.c -- CPU restatement of Primus's UintNttTable<u64>, ShoupFactor,
+ * MultiplyFactor, BarrettModulus<u64> and the schoolbook negacyclic product.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libprimus_hip.so and the timed
+ * CPU baseline.  Nothing in the product links or calls it.
+ *
+ * The restatement keeps the reference's arithmetic step for step (the same
+ * Harvey lazy butterflies, the same Shoup quotients, the same twiddle order),
+ * so lazy outputs as well as canonical outputs are reproduced.  Citations are
+ * relative to the reference workspace root.
+ */
+#include "primus_oracle.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+/* ------------------------------------------------------------------------- */
+/* Shoup factor                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* ShoupFactor::new: quotient = floor(w * 2^64 / q), requires w < q.
+ * primus_factor/src/shoup_factor/mod.rs:44-55 */
+uint64_t po_shoup_quotient(uint64_t w, uint64_t q) {
+    return (uint64_t)(((u128)w << 64) / q);
+}
+
+/* LazyFactorMul::lazy_factor_mul_modulo: w*b - floor(w'*b / 2^64)*q (wrapping),
+ * result in [0, 2q) for any b < 2^64.  primus_factor/src/shoup_factor/mod.rs:108-113 */
+uint64_t po_shoup_lazy_mul(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
+    uint64_t hw = (uint64_t)(((u128)wq * b) >> 64);
+    return w * b - q * hw;
+}
+
+/* FactorMul::factor_mul_modulo: lazy then one conditional subtraction.
+ * primus_factor/src/shoup_factor/mod.rs:121-124 */
+uint64_t po_shoup_mul(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
+    uint64_t t = po_shoup_lazy_mul(w, wq, b, q);
+    return t >= q ? t - q : t;
+}
+
+/* ------------------------------------------------------------------------- */
+/* MultiplyFactor (HEXL Barrett factor)                                       */
+/* ------------------------------------------------------------------------- */
+
+/* MultiplyFactor::new: floor((op << bit_shift) / q), bit_shift in {32,52,64}.
+ * primus_factor/src/multiply.rs:18-41, 71-75 */
+uint64_t po_multiply_factor(uint64_t op, uint32_t bit_shift, uint64_t q) {
+    u128 n = (u128)op << bit_shift;
+    return (uint64_t)(n / q);
+}
+
+/* hexl::number_theory::multiply_mod_lazy: op*x - ((x*factor) >> shift)*q.
+ * primus_ntt/src/ntt/hexl/number_theory.rs:66-94 */
+uint64_t po_multiply_mod_lazy(uint64_t x, uint64_t y, uint64_t y_barrett, uint32_t bit_shift,
+                              uint64_t q) {
+    uint64_t hw = (uint64_t)(((u128)x * y_barrett) >> bit_shift);
+    return y * x - hw * q;
+}
+
+/* ------------------------------------------------------------------------- */
+/* BarrettModulus<u64>                                                       */
+/* ------------------------------------------------------------------------- */
+
+/* BarrettModulus::new / new_unchecked: ratio = floor(2^128 / q) as [lo, hi];
+ * panics (here: PO_ERR_INVALID) for q <= 1 or bit-length(q) >= 63.
+ * primus_modulus/src/barrett/mod.rs:55-75 */
+int po_barrett_ratio(uint64_t q, uint64_t ratio[2]) {
+    if (q <= 1) return PO_ERR_INVALID;
+    if ((64 - __builtin_clzll(q)) >= 63) return PO_ERR_INVALID;
+    /* floor(2^128/q) = floor((2^128-1)/q) unless q | 2^128, impossible for q>1 odd
+     * and handled exactly below for even q. */
+    u128 all = ~(u128)0;
+    u128 r = all / q;
+    if (all % q == q - 1) r += 1;
+    ratio[0] = (uint64_t)r;
+    ratio[1] = (uint64_t)(r >> 64);
+    return PO_OK;
+}
+
+/* LazyReduce<T>::lazy_reduce: value mod 2q via the 1-limb Barrett quotient.
+ * primus_modulus/src/barrett/ops.rs:8-31 */
+uint64_t po_barrett_lazy_reduce(uint64_t value, uint64_t q, const uint64_t ratio[2]) {
+    uint64_t tmp = (uint64_t)(((u128)value * ratio[0]) >> 64);
+    uint64_t qq = (uint64_t)(((u128)value * ratio[1] + tmp) >> 64);
+    return value - qq * q;
+}
+
+/* BarrettModulus::lazy_reduce_2limb: Barrett of the 128-bit (hi, lo), [0, 2q).
+ * primus_modulus/src/barrett/ops.rs:42-74 */
+uint64_t po_barrett_lazy_reduce_2limb(uint64_t lo, uint64_t hi, uint64_t q,
+                                      const uint64_t ratio[2]) {
+    uint64_t ah = (uint64_t)(((u128)lo * ratio[0]) >> 64);
+    u128 b = (u128)lo * ratio[1] + ah; /* carrying_mul */
+    u128 c = (u128)hi * ratio[0];
+    uint64_t d = hi * ratio[1];
+    uint64_t b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
+    uint64_t c0 = (uint64_t)c, c1 = (uint64_t)(c >> 64);
+    uint64_t carry = (uint64_t)(b0 + c0 < b0);
+    uint64_t bch = b1 + c1 + carry;
+    uint64_t qq = d + bch;
+    return lo - qq * q;
+}
+
+static inline uint64_t reduce_once(uint64_t x, uint64_t m) { return x >= m ? x - m : x; }
+
+/* ReduceMul::reduce_mul = reduce(widening_mul) = reduce_once(lazy_reduce_2limb).
+ * primus_modulus/src/barrett/ops.rs:158-175, 310-316 */
+uint64_t po_barrett_reduce_mul(uint64_t a, uint64_t b, uint64_t q, const uint64_t ratio[2]) {
+    u128 p = (u128)a * b;
+    return reduce_once(po_barrett_lazy_reduce_2limb((uint64_t)p, (uint64_t)(p >> 64), q, ratio), q);
+}
+
+/* ReduceMulAdd::reduce_mul_add = reduce(a.carrying_mul(b, c)).
+ * primus_modulus/src/barrett/ops.rs:331-337 */
+uint64_t po_barrett_reduce_mul_add(uint64_t a, uint64_t b, uint64_t c, uint64_t q,
+                                   const uint64_t ratio[2]) {
+    u128 p = (u128)a * b + c;
+    return reduce_once(po_barrett_lazy_reduce_2limb((uint64_t)p, (uint64_t)(p >> 64), q, ratio), q);
+}
+
+/* ReduceExp by square-and-multiply; any exact method yields the same residue.
+ * primus_modulus/src/barrett/ops.rs:399-456 */
+uint64_t po_pow_mod(uint64_t base, uint64_t exp, uint64_t q) {
+    u128 r = 1 % q, b = base % q;
+    while (exp) {
+        if (exp & 1) r = (r * b) % q;
+        b = (b * b) % q;
+        exp >>= 1;
+    }
+    return (uint64_t)r;
+}
+
+/* UintModulus::reduce_inv via the extended Euclidean algorithm (the reference
+ * calls primus_gcd's gcdinv); returns 0 when a is not invertible.
+ * primus_modulus/src/unsigned_integer/ops.rs, primus_gcd/src/lib.rs */
+uint64_t po_inv_mod(uint64_t a, uint64_t q) {
+    __int128 t = 0, nt = 1, r = q, nr = a % q;
+    while (nr != 0) {
+        __int128 qt = r / nr, tmp;
+        tmp = t - qt * nt; t = nt; nt = tmp;
+        tmp = r - qt * nr; r = nr; nr = tmp;
+    }
+    if (r != 1) return 0;
+    if (t < 0) t += q;
+    return (uint64_t)t;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Primitive root, bit reversal                                              */
+/* ------------------------------------------------------------------------- */
+
+/* ReverseLsbs::reverse_lsbs: primus_ntt/src/reverse.rs:128-141 */
+uint64_t po_reverse_lsbs(uint64_t x, uint32_t bits) {
+    if (x == 0 || bits == 0) return 0;
+    uint64_t r = 0;
+    for (uint32_t i = 0; i < 64; ++i) r |= ((x >> i) & 1ull) << (63 - i);
+    return r >> (64 - bits);
+}
+
+/* PrimitiveRoot::is_primitive_root: w != 0 and w^(2^(log_degree-1)) == q-1.
+ * primus_ntt/src/root.rs:27-44 */
+int po_is_primitive_root(uint64_t w, uint32_t log_degree, uint64_t q) {
+    if (w == 0 || log_degree == 0) return 0;
+    u128 x = w;
+    for (uint32_t i = 0; i + 1 < log_degree; ++i) x = (x * x) % q;
+    return (uint64_t)x == q - 1;
+}
+
+/* PrimitiveRoot::try_minimal_primitive_root: the smallest primitive 2^log_degree-th
+ * root of unity mod q.  primus_ntt/src/root.rs:46-111.
+ *
+ * The reference draws up to 200 random candidates r in [2, q-1] and tests
+ * w = r^((q-1)/2^log_degree); then it minimises over the odd powers of w.  The
+ * odd powers of any primitive root are exactly the set of primitive roots, so
+ * the minimum does not depend on which candidate succeeded.  Here candidates
+ * are r = 2, 3, ... (at most 1 << 20 of them), which makes the search
+ * deterministic; for a prime q the first quadratic non-residue succeeds. */
+int po_minimal_primitive_root(uint32_t log_degree, uint64_t q, uint64_t *root) {
+    if (log_degree >= 64 || q < 2) return PO_ERR_INVALID;
+    uint64_t qm1 = q - 1;
+    uint64_t quotient = qm1 >> log_degree;
+    if ((quotient << log_degree) != qm1) return PO_ERR_NO_PRIMITIVE_ROOT;
+    uint64_t w = 0;
+    int found = 0;
+    for (uint64_t r = 2; r <= qm1 && r < (2ull + (1ull << 20)); ++r) {
+        w = po_pow_mod(r, quotient, q);
+        if (po_is_primitive_root(w, log_degree, q)) { found = 1; break; }
+    }
+    if (!found) return PO_ERR_NO_PRIMITIVE_ROOT;
+    /* root.rs:97-110: walk w, w^3, w^5, ... (2^log_degree steps), keep the minimum */
+    uint64_t best = w;
+    uint64_t g2 = (uint64_t)(((u128)w * w) % q);
+    uint64_t cur = w;
+    uint64_t degree = 1ull << log_degree;
+    for (uint64_t i = 0; i < degree; ++i) {
+        if (cur < best) best = cur;
+        cur = (uint64_t)(((u128)g2 * cur) % q);
+    }
+    *root = best;
+    return PO_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* UintNttTable<u64>                                                         */
+/* ------------------------------------------------------------------------- */
+
+struct po_table {
+    uint64_t q, root, inv_root;
+    uint32_t log_n;
+    size_t n;
+    uint64_t inv_n, inv_n_q;
+    uint64_t *rp, *rp_q;   /* root_powers (bit-reversed), N */
+    uint64_t *irp, *irp_q; /* inv_root_powers (scrambled), N */
+    uint64_t *op, *op_q;   /* ordinal_root_powers, 2N */
+    uint64_t *rev;         /* reverse_lsbs, N */
+};
+
+void po_table_free(po_table *t) {
+    if (!t) return;
+    free(t->rp); free(t->rp_q); free(t->irp); free(t->irp_q);
+    free(t->op); free(t->op_q); free(t->rev);
+    free(t);
+}
+
+/* UintNttTable::new.  primus_ntt/src/ntt/primitive.rs:114-185 */
+int po_table_new(uint32_t log_n, uint64_t q, po_table **out) {
+    *out = NULL;
+    uint64_t ratio[2];
+    if (po_barrett_ratio(q, ratio) != PO_OK) return PO_ERR_INVALID; /* BarrettModulus::new */
+    /* log_n = 0 makes the reference's inverse loop bound `log_n - 1` wrap; it is
+     * rejected here, as are tables too large to hold in memory. */
+    if (log_n == 0 || log_n > 30) return PO_ERR_INVALID;
+    uint64_t root;
+    int st = po_minimal_primitive_root(log_n + 1, q, &root);
+    if (st != PO_OK) return st;
+    size_t n = (size_t)1 << log_n;
+    if ((uint64_t)n >= q) return PO_ERR_DEGREE_TOO_LARGE; /* primitive.rs:164-166 */
+    po_table *t = (po_table *)calloc(1, sizeof(po_table));
+    if (!t) return PO_ERR_NTT_TABLE;
+    t->q = q; t->root = root; t->log_n = log_n; t->n = n;
+    t->rp = malloc(n * 8); t->rp_q = malloc(n * 8);
+    t->irp = malloc(n * 8); t->irp_q = malloc(n * 8);
+    t->op = malloc(2 * n * 8); t->op_q = malloc(2 * n * 8);
+    t->rev = malloc(n * 8);
+    if (!t->rp || !t->rp_q || !t->irp || !t->irp_q || !t->op || !t->op_q || !t->rev) {
+        po_table_free(t);
+        return PO_ERR_NTT_TABLE;
+    }
+    /* ordinal_root_powers[k] = root^k, k < 2N (primitive.rs:128-137) */
+    uint64_t root_q = po_shoup_quotient(root, q);
+    t->op[0] = 1 % q; t->op_q[0] = po_shoup_quotient(t->op[0], q);
+    t->op[1] = root; t->op_q[1] = root_q;
+    uint64_t power = root;
+    for (size_t k = 2; k < 2 * n; ++k) {
+        power = po_shoup_mul(root, root_q, power, q);
+        t->op[k] = power; t->op_q[k] = po_shoup_quotient(power, q);
+    }
+    t->inv_root = t->op[2 * n - 1]; /* primitive.rs:139 */
+    for (size_t i = 0; i < n; ++i) t->rev[i] = po_reverse_lsbs(i, log_n); /* :143 */
+    /* root_powers[rev(k)] = root^k (primitive.rs:145-149) */
+    for (size_t k = 0; k < n; ++k) { t->rp[t->rev[k]] = t->op[k]; t->rp_q[t->rev[k]] = t->op_q[k]; }
+    t->rp[0] = t->op[0]; t->rp_q[0] = t->op_q[0];
+    /* inv_root_powers[rev(k)+1] = root^{-(k+1)} = ordinal[2N-1-k], k < N-1 (:151-159) */
+    t->irp[0] = t->op[0]; t->irp_q[0] = t->op_q[0];
+    for (size_t k = 0; k + 1 < n; ++k) {
+        size_t src = 2 * n - 1 - k;
+        t->irp[t->rev[k] + 1] = t->op[src];
+        t->irp_q[t->rev[k] + 1] = t->op_q[src];
+    }
+    /* inv_n = N^{-1} mod q (:161-168) */
+    t->inv_n = po_inv_mod((uint64_t)n % q, q);
+    t->inv_n_q = po_shoup_quotient(t->inv_n, q);
+    *out = t;
+    return PO_OK;
+}
+
+size_t po_table_n(const po_table *t) { return t->n; }
+uint64_t po_table_modulus(const po_table *t) { return t->q; }
+uint64_t po_table_root(const po_table *t) { return t->root; }
+uint64_t po_table_inv_root(const po_table *t) { return t->inv_root; }
+void po_table_inv_n(const po_table *t, uint64_t *value, uint64_t *quotient) {
+    *value = t->inv_n; *quotient = t->inv_n_q;
+}
+void po_table_powers(const po_table *t, int which, uint64_t *values, uint64_t *quotients) {
+    const uint64_t *v = which == 0 ? t->rp : which == 1 ? t->irp : t->op;
+    const uint64_t *w = which == 0 ? t->rp_q : which == 1 ? t->irp_q : t->op_q;
+    size_t cnt = which == 2 ? 2 * t->n : t->n;
+    if (values) memcpy(values, v, cnt * 8);
+    if (quotients) memcpy(quotients, w, cnt * 8);
+}
+
+/* lazy_transform_slice: Cooley-Tukey, normal in, bit-reversed out, [0,4q) -> [0,4q).
+ * primus_ntt/src/ntt/primitive.rs:211-232 */
+void po_lazy_transform(const po_table *t, uint64_t *poly) {
+    const uint64_t q = t->q, q2 = q << 1;
+    size_t ri = 1;
+    for (int s = (int)t->log_n - 1; s >= 0; --s) {
+        size_t gap = (size_t)1 << s;
+        for (size_t base = 0; base < t->n; base += gap << 1) {
+            uint64_t w = t->rp[ri], wq = t->rp_q[ri];
+            ++ri;
+            for (size_t i = base; i < base + gap; ++i) {
+                uint64_t u = reduce_once(poly[i], q2);
+                uint64_t v = po_shoup_lazy_mul(w, wq, poly[i + gap], q);
+                poly[i] = u + v;
+                poly[i + gap] = u + q2 - v;
+            }
+        }
+    }
+}
+
+/* transform_slice: lazy transform then reduce [0,4q) -> [0,q).
+ * primus_ntt/src/ntt/primitive.rs:234-244 */
+void po_transform(const po_table *t, uint64_t *poly) {
+    po_lazy_transform(t, poly);
+    const uint64_t q = t->q, q2 = q << 1;
+    for (size_t i = 0; i < t->n; ++i) poly[i] = reduce_once(reduce_once(poly[i], q2), q);
+}
+
+/* lazy_inverse_transform_slice: Gentleman-Sande, bit-reversed in, normal out,
+ * [0,2q) -> [0,q); the last stage folds N^{-1}.  primus_ntt/src/ntt/primitive.rs:246-286 */
+void po_lazy_inverse_transform(const po_table *t, uint64_t *values) {
+    const uint64_t q = t->q, q2 = q << 1;
+    const uint32_t log_n = t->log_n;
+    size_t ri = 1;
+    for (uint32_t s = 0; s + 1 < log_n; ++s) {
+        size_t gap = (size_t)1 << s;
+        for (size_t base = 0; base < t->n; base += gap << 1) {
+            uint64_t w = t->irp[ri], wq = t->irp_q[ri];
+            ++ri;
+            for (size_t i = base; i < base + gap; ++i) {
+                uint64_t u = values[i], v = values[i + gap];
+                values[i] = reduce_once(u + v, q2);
+                values[i + gap] = po_shoup_lazy_mul(w, wq, u + q2 - v, q);
+            }
+        }
+    }
+    size_t gap = (size_t)1 << (log_n - 1);
+    uint64_t s = t->inv_n, sq = t->inv_n_q;
+    uint64_t sr = po_shoup_mul(t->irp[ri], t->irp_q[ri], s, q);
+    uint64_t srq = po_shoup_quotient(sr, q);
+    for (size_t i = 0; i < gap; ++i) {
+        uint64_t u = values[i], v = values[i + gap];
+        values[i] = po_shoup_mul(s, sq, u + v, q);
+        values[i + gap] = po_shoup_mul(sr, srq, u + q2 - v, q);
+    }
+}
+
+/* inverse_transform_slice: lazy inverse then reduce_once.  primitive.rs:288-296 */
+void po_inverse_transform(const po_table *t, uint64_t *values) {
+    po_lazy_inverse_transform(t, values);
+    for (size_t i = 0; i < t->n; ++i) values[i] = reduce_once(values[i], t->q);
+}
+
+/* transform_monomial: out[k] = coeff * psi^(((2 rev(k) + 1) * degree) mod 2N).
+ * primus_ntt/src/ntt/primitive.rs:298-347 */
+void po_transform_monomial(const po_table *t, uint64_t coeff, size_t degree, uint64_t *values) {
+    size_t n = t->n;
+    if (coeff == 0) { memset(values, 0, n * 8); return; }
+    if (degree == 0) { for (size_t i = 0; i < n; ++i) values[i] = coeff; return; }
+    size_t mask = (2 * n) - 1;
+    for (size_t k = 0; k < n; ++k) {
+        size_t idx = ((2 * t->rev[k] + 1) * degree) & mask;
+        if (coeff == 1) values[k] = t->op[idx];
+        else if (coeff == t->q - 1) values[k] = t->op[idx ^ n];
+        else values[k] = po_shoup_mul(t->op[idx], t->op_q[idx], coeff, t->q);
+    }
+}
+
+/* transform_coeff_one_monomial.  primitive.rs:349-372 */
+void po_transform_coeff_one_monomial(const po_table *t, size_t degree, uint64_t *values) {
+    size_t n = t->n;
+    if (degree == 0) { for (size_t i = 0; i < n; ++i) values[i] = 1; return; }
+    size_t mask = (2 * n) - 1;
+    for (size_t k = 0; k < n; ++k) values[k] = t->op[((2 * t->rev[k] + 1) * degree) & mask];
+}
+
+/* transform_coeff_minus_one_monomial.  primitive.rs:374-397 */
+void po_transform_coeff_minus_one_monomial(const po_table *t, size_t degree, uint64_t *values) {
+    size_t n = t->n;
+    if (degree == 0) { for (size_t i = 0; i < n; ++i) values[i] = t->q - 1; return; }
+    size_t mask = (2 * n) - 1;
+    for (size_t k = 0; k < n; ++k) values[k] = t->op[(((2 * t->rev[k] + 1) * degree) & mask) ^ n];
+}
+
+/* ------------------------------------------------------------------------- */
+/* Element-wise and schoolbook                                               */
+/* ------------------------------------------------------------------------- */
+
+/* NttPolynomial::mul_assign / mul_inplace (Barrett reduce_mul per element).
+ * primus_poly/src/ntt/mul.rs:87-119 */
+void po_pointwise_mul(const uint64_t *a, const uint64_t *b, uint64_t *out, size_t n, uint64_t q) {
+    uint64_t ratio[2];
+    po_barrett_ratio(q, ratio);
+    for (size_t i = 0; i < n; ++i) out[i] = po_barrett_reduce_mul(a[i], b[i], q, ratio);
+}
+
+/* NttPolynomial::add_mul_assign: acc = (a*b + acc) mod q.
+ * primus_poly/src/ntt/mod.rs:97-108 -> primus_modulus/src/barrett/slice.rs:175-188 */
+void po_pointwise_mul_add(uint64_t *acc, const uint64_t *a, const uint64_t *b, size_t n,
+                          uint64_t q) {
+    uint64_t ratio[2];
+    po_barrett_ratio(q, ratio);
+    for (size_t i = 0; i < n; ++i) acc[i] = po_barrett_reduce_mul_add(a[i], b[i], acc[i], q, ratio);
+}
+
+/* Per-element Shoup multiply x[i] = w[i]*x[i] mod q (canonical).
+ * primus_poly/src/dcrt/mul.rs:119-155, primus_factor/src/shoup_factor/mod.rs:121-124 */
+void po_shoup_pointwise(const uint64_t *w, const uint64_t *wq, uint64_t *x, size_t n, uint64_t q) {
+    for (size_t i = 0; i < n; ++i) x[i] = po_shoup_mul(w[i], wq[i], x[i], q);
+}
+
+/* Polynomial::naive_mul_inplace: c = a*b mod (X^N + 1, q), c starts at zero.
+ * primus_poly/src/poly/mul.rs:111-142 */
+void po_schoolbook_negacyclic(const uint64_t *a, const uint64_t *b, uint64_t *c, size_t n,
+                              uint64_t q) {
+    uint64_t ratio[2];
+    po_barrett_ratio(q, ratio);
+    memset(c, 0, n * 8);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j <= i; ++j) c[i] = po_barrett_reduce_mul_add(a[j], b[i - j], c[i], q, ratio);
+    for (size_t i = n; i + 1 < 2 * n; ++i) {
+        size_t k = i - n;
+        for (size_t j = i - n + 1; j < n; ++j) {
+            uint64_t p = po_barrett_reduce_mul(a[j], b[i - j], q, ratio);
+            c[k] = c[k] >= p ? c[k] - p : c[k] + q - p; /* reduce_sub_assign, ops.rs:268-276 */
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Batched CPU baseline                                                      */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    po_table *const *tables;
+    uint32_t L;
+    uint64_t *polys;
+    size_t begin, end;
+    const uint64_t *operand, *operand_q;
+} batch_job;
+
+static void *batch_worker(void *arg) {
+    batch_job *j = (batch_job *)arg;
+    for (size_t i = j->begin; i < j->end; ++i) {
+        const po_table *t = j->tables[i % j->L];
+        size_t n = t->n;
+        uint64_t *p = j->polys + i * n;
+        const uint64_t *w = j->operand + (i % j->L) * n;
+        const uint64_t *wq = j->operand_q + (i % j->L) * n;
+        po_transform(t, p);
+        po_shoup_pointwise(w, wq, p, n, t->q);
+        po_inverse_transform(t, p);
+    }
+    return NULL;
+}
+
+/* UintCrtNttTable::transform_slice / inverse_transform_slice over limb-major
+ * chunks (primus_ntt/src/dcrt/primitive.rs:117-141) with a Shoup pointwise
+ * multiply between them, split over host threads. */
+void po_fwd_mul_inv_batch(po_table *const *tables, uint32_t L, uint64_t *polys, size_t count,
+                          const uint64_t *operand, const uint64_t *operand_q, int threads) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t tid[256];
+    batch_job jobs[256];
+    size_t per = (count + threads - 1) / threads;
+    int started = 0;
+    for (int k = 0; k < threads; ++k) {
+        size_t b = (size_t)k * per, e = b + per;
+        if (b >= count) break;
+        if (e > count) e = count;
+        jobs[k] = (batch_job){tables, L, polys, b, e, operand, operand_q};
+        if (threads == 1) { batch_worker(&jobs[k]); continue; }
+        pthread_create(&tid[k], NULL, batch_worker, &jobs[k]);
+        ++started;
+    }
+    for (int k = 0; k < started; ++k) pthread_join(tid[k], NULL);
+}

@@ -1,0 +1,94 @@
+/*
+ * primus_oracle.h -- CPU restatement of Primus's negacyclic NTT and the
+ * modular arithmetic under it.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * library (libprimus_hip.so) is compared against, and the timed CPU baseline
+ * in bench.py.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  No product code path calls into it.
+ *
+ * Every function cites the reference file:line it restates.  The reference
+ * is a Rust workspace (serendipity-crypto/primus); paths are relative to its
+ * root.
+ */
+#ifndef PRIMUS_ORACLE_H
+#define PRIMUS_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: identical numbering to include/primus_hip.h. */
+#define PO_OK 0
+#define PO_ERR_NO_PRIMITIVE_ROOT 1  /* NttError::NoPrimitiveRoot    primus_ntt/src/error.rs:9 */
+#define PO_ERR_DEGREE_CONVERSION 2  /* NttError::DegreeConversionErr primus_ntt/src/error.rs:17 */
+#define PO_ERR_DEGREE_TOO_LARGE 3   /* NttError::DegreeTooLarge     primus_ntt/src/error.rs:25 */
+#define PO_ERR_NTT_TABLE 4          /* NttError::NttTableErr        primus_ntt/src/error.rs:33 */
+#define PO_ERR_INVALID 5            /* bad argument (BarrettModulus::new panics) */
+
+/* ---- Shoup factor: primus_factor/src/shoup_factor/mod.rs:32-125 ---- */
+uint64_t po_shoup_quotient(uint64_t w, uint64_t q);
+uint64_t po_shoup_lazy_mul(uint64_t w, uint64_t wq, uint64_t b, uint64_t q);
+uint64_t po_shoup_mul(uint64_t w, uint64_t wq, uint64_t b, uint64_t q);
+
+/* ---- MultiplyFactor (HEXL Barrett factor): primus_factor/src/multiply.rs:4-76 ---- */
+uint64_t po_multiply_factor(uint64_t op, uint32_t bit_shift, uint64_t q);
+uint64_t po_multiply_mod_lazy(uint64_t x, uint64_t y, uint64_t y_barrett, uint32_t bit_shift, uint64_t q);
+
+/* ---- BarrettModulus<u64>: primus_modulus/src/barrett/{mod.rs,ops.rs} ---- */
+int po_barrett_ratio(uint64_t q, uint64_t ratio[2]);
+uint64_t po_barrett_lazy_reduce(uint64_t value, uint64_t q, const uint64_t ratio[2]);
+uint64_t po_barrett_lazy_reduce_2limb(uint64_t lo, uint64_t hi, uint64_t q, const uint64_t ratio[2]);
+uint64_t po_barrett_reduce_mul(uint64_t a, uint64_t b, uint64_t q, const uint64_t ratio[2]);
+uint64_t po_barrett_reduce_mul_add(uint64_t a, uint64_t b, uint64_t c, uint64_t q,
+                                   const uint64_t ratio[2]);
+uint64_t po_pow_mod(uint64_t base, uint64_t exp, uint64_t q);
+uint64_t po_inv_mod(uint64_t a, uint64_t q);
+
+/* ---- PrimitiveRoot / ReverseLsbs: primus_ntt/src/{root.rs,reverse.rs} ---- */
+int po_is_primitive_root(uint64_t w, uint32_t log_degree, uint64_t q);
+int po_minimal_primitive_root(uint32_t log_degree, uint64_t q, uint64_t *root);
+uint64_t po_reverse_lsbs(uint64_t x, uint32_t bits);
+
+/* ---- UintNttTable<u64>: primus_ntt/src/ntt/primitive.rs:34-398 ---- */
+typedef struct po_table po_table;
+int po_table_new(uint32_t log_n, uint64_t q, po_table **out);
+void po_table_free(po_table *t);
+size_t po_table_n(const po_table *t);
+uint64_t po_table_modulus(const po_table *t);
+uint64_t po_table_root(const po_table *t);
+uint64_t po_table_inv_root(const po_table *t);
+void po_table_inv_n(const po_table *t, uint64_t *value, uint64_t *quotient);
+/* which: 0 = root_powers (N), 1 = inv_root_powers (N), 2 = ordinal_root_powers (2N) */
+void po_table_powers(const po_table *t, int which, uint64_t *values, uint64_t *quotients);
+
+void po_lazy_transform(const po_table *t, uint64_t *poly);
+void po_transform(const po_table *t, uint64_t *poly);
+void po_lazy_inverse_transform(const po_table *t, uint64_t *values);
+void po_inverse_transform(const po_table *t, uint64_t *values);
+void po_transform_monomial(const po_table *t, uint64_t coeff, size_t degree, uint64_t *values);
+void po_transform_coeff_one_monomial(const po_table *t, size_t degree, uint64_t *values);
+void po_transform_coeff_minus_one_monomial(const po_table *t, size_t degree, uint64_t *values);
+
+/* ---- primus_poly element-wise ops and the schoolbook product ---- */
+void po_pointwise_mul(const uint64_t *a, const uint64_t *b, uint64_t *out, size_t n, uint64_t q);
+void po_pointwise_mul_add(uint64_t *acc, const uint64_t *a, const uint64_t *b, size_t n, uint64_t q);
+void po_shoup_pointwise(const uint64_t *w, const uint64_t *wq, uint64_t *x, size_t n, uint64_t q);
+void po_schoolbook_negacyclic(const uint64_t *a, const uint64_t *b, uint64_t *c, size_t n,
+                              uint64_t q);
+
+/* ---- Batched CPU baselines (the timed CPU leg) ---- */
+/* polys: count limb-polynomials of N coefficients, limb-major: poly i uses table[i % L].
+ * operand / operand_q: L*N Shoup-prepared NTT-domain operand.  Runs forward,
+ * Shoup pointwise multiply and inverse for each limb-polynomial on `threads`
+ * host threads. */
+void po_fwd_mul_inv_batch(po_table *const *tables, uint32_t L, uint64_t *polys, size_t count,
+                          const uint64_t *operand, const uint64_t *operand_q, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,140 @@
+"""Loader and build step of libprimus_hip.so (the C ABI in include/primus_hip.h).
+
+There is no CPU fallback: if the library is missing, loading raises.
+"""
+import ctypes
+import os
+import subprocess
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libprimus_hip.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "primus_hip.h")
+CSRC = os.path.join(PKG_DIR, "csrc")
+SOURCES = ["capi.hip", "ntt_kernels.hip", "fhe_kernels.hip", "serving.cpp"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # keep packed-FP32 instructions out of all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose=False):
+    """Compile every HIP source into primus_amd/libprimus_hip.so for gfx950."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc] + HIPCC_FLAGS + ["-pthread", "-o", tmp] + srcs
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+    if verbose:
+        print(res.stdout + res.stderr)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+_lib = None
+
+
+class PrimusHipError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__("primus_hip status %d (%s)%s" % (code, _status_string(code), (": " + msg) if msg else ""))
+        self.code = code
+
+
+def _status_string(code):
+    try:
+        return lib().ph_status_string(code).decode()
+    except Exception:  # pragma: no cover
+        return "?"
+
+
+def lib():
+    """The loaded libprimus_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libprimus_hip.so is not built (%s); run __graft_entry__.build() first" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        msg = lib().ph_last_error().decode(errors="replace")
+        raise PrimusHipError(code, msg)
+
+
+def _declare(L):
+    u64, u32, sz, vp, i = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int
+    p64 = ctypes.POINTER(ctypes.c_uint64)
+    pvp = ctypes.POINTER(ctypes.c_void_p)
+    sigs = {
+        "ph_status_string": (ctypes.c_char_p, [i]),
+        "ph_last_error": (ctypes.c_char_p, []),
+        "ph_ntt_table_new": (i, [u32, u64, pvp]),
+        "ph_ntt_table_free": (None, [vp]),
+        "ph_ntt_table_poly_length": (sz, [vp]),
+        "ph_ntt_table_log_n": (u32, [vp]),
+        "ph_ntt_table_modulus": (u64, [vp]),
+        "ph_ntt_table_root": (u64, [vp]),
+        "ph_ntt_table_inv_root": (u64, [vp]),
+        "ph_ntt_table_inv_n": (i, [vp, p64, p64]),
+        "ph_ntt_table_powers": (i, [vp, i, p64, p64]),
+        "ph_lazy_transform_slice": (i, [vp, p64, sz]),
+        "ph_transform_slice": (i, [vp, p64, sz]),
+        "ph_lazy_inverse_transform_slice": (i, [vp, p64, sz]),
+        "ph_inverse_transform_slice": (i, [vp, p64, sz]),
+        "ph_transform_monomial": (i, [vp, u64, sz, p64, sz]),
+        "ph_transform_coeff_one_monomial": (i, [vp, sz, p64, sz]),
+        "ph_transform_coeff_minus_one_monomial": (i, [vp, sz, p64, sz]),
+        "ph_dcrt_table_new": (i, [u32, p64, u32, pvp]),
+        "ph_dcrt_table_free": (None, [vp]),
+        "ph_dcrt_poly_length": (sz, [vp]),
+        "ph_dcrt_moduli_count": (sz, [vp]),
+        "ph_dcrt_crt_poly_length": (sz, [vp]),
+        "ph_dcrt_ntt_table": (vp, [vp, u32]),
+        "ph_dcrt_lazy_transform_slice": (i, [vp, p64, sz]),
+        "ph_dcrt_transform_slice": (i, [vp, p64, sz]),
+        "ph_dcrt_lazy_inverse_transform_slice": (i, [vp, p64, sz]),
+        "ph_dcrt_inverse_transform_slice": (i, [vp, p64, sz]),
+        "ph_dcrt_transform_device": (i, [vp, vp, sz, i, vp]),
+        "ph_dcrt_inverse_transform_device": (i, [vp, vp, sz, vp]),
+        "ph_ntt_transform_device": (i, [vp, vp, sz, i, vp]),
+        "ph_ntt_inverse_transform_device": (i, [vp, vp, sz, vp]),
+        "ph_operand_new": (i, [vp, p64, pvp]),
+        "ph_operand_free": (None, [vp]),
+        "ph_dcrt_fwd_mul_inv_device": (i, [vp, vp, sz, vp, vp]),
+        "ph_dcrt_negacyclic_mul_device": (i, [vp, vp, vp, sz, vp]),
+        "ph_dcrt_pointwise_mul_device": (i, [vp, vp, vp, vp, sz, vp]),
+        "ph_dcrt_pointwise_mul_add_device": (i, [vp, vp, vp, vp, sz, vp]),
+        "ph_device_alloc": (i, [sz, pvp]),
+        "ph_device_free": (i, [vp]),
+        "ph_memcpy_htod": (i, [vp, vp, sz]),
+        "ph_memcpy_dtoh": (i, [vp, vp, sz]),
+        "ph_stream_synchronize": (i, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        if not hasattr(L, name):
+            continue
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def header_functions():
+    """Names of the functions include/primus_hip.h declares."""
+    import re
+
+    text = open(HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ph_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,102 @@
+// arith.hpp -- 64-bit modular arithmetic for gfx950 (device) and for table
+// set-up (host).
+//
+// All arithmetic is exact unsigned integer work.  The device forms compile to
+// 32-bit VALU chains (v_mul_hi_u32 / v_mad_u64_u32 / v_mul_lo_u32): CDNA has
+// no 64x64 multiplier.  Semantics follow the reference's reduction contract:
+//   * Shoup lazy multiply: primus_factor/src/shoup_factor/mod.rs:108-113
+//     (result in [0, 2q) for any 64-bit b), full: :121-124.
+//   * Barrett 2-limb reduction of a 128-bit product:
+//     primus_modulus/src/barrett/ops.rs:42-74, reduce_mul :310-316,
+//     reduce_mul_add :331-337.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ph {
+
+// ---------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+
+__device__ __forceinline__ uint64_t sub_if_ge(uint64_t x, uint64_t m) { return x >= m ? x - m : x; }
+
+// w*b - floor(wq*b / 2^64)*q (mod 2^64), in [0, 2q); wq = floor(w*2^64/q).
+__device__ __forceinline__ uint64_t shoup_lazy(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
+    return w * b - mulhi64(wq, b) * q;
+}
+
+__device__ __forceinline__ uint64_t shoup_full(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
+    return sub_if_ge(shoup_lazy(w, wq, b, q), q);
+}
+
+// Canonical (a*b + c) mod q by the 2-limb Barrett quotient with ratio
+// floor(2^128/q) = (r1:r0).  Inputs a, b < 2^64 with a*b + c < 2^128.
+__device__ __forceinline__ uint64_t barrett_reduce128(uint64_t lo, uint64_t hi, uint64_t q,
+                                                      uint64_t r0, uint64_t r1) {
+    uint64_t ah = mulhi64(lo, r0);
+    uint64_t b_lo = lo * r1;
+    uint64_t b_hi = mulhi64(lo, r1);
+    b_lo += ah;
+    b_hi += (b_lo < ah);
+    uint64_t c_lo = hi * r0;
+    uint64_t c_hi = mulhi64(hi, r0);
+    uint64_t d = hi * r1;
+    uint64_t s = b_lo + c_lo;
+    uint64_t bch = b_hi + c_hi + (s < b_lo);
+    uint64_t qq = d + bch;
+    return sub_if_ge(lo - qq * q, q);
+}
+
+__device__ __forceinline__ uint64_t barrett_mul(uint64_t a, uint64_t b, uint64_t q, uint64_t r0,
+                                                uint64_t r1) {
+    return barrett_reduce128(a * b, mulhi64(a, b), q, r0, r1);
+}
+
+__device__ __forceinline__ uint64_t barrett_mul_add(uint64_t a, uint64_t b, uint64_t c, uint64_t q,
+                                                    uint64_t r0, uint64_t r1) {
+    uint64_t lo = a * b;
+    uint64_t hi = mulhi64(a, b);
+    lo += c;
+    hi += (lo < c);
+    return barrett_reduce128(lo, hi, q, r0, r1);
+}
+
+// (a + b) mod q for a, b < q.
+__device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) {
+    return sub_if_ge(a + b, q);
+}
+
+// ---------------------------------------------------------------------------
+// host (table construction only)
+// ---------------------------------------------------------------------------
+
+typedef unsigned __int128 u128;
+
+inline uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)(((u128)a * b) % q); }
+
+inline uint64_t h_powmod(uint64_t b, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    b %= q;
+    while (e) {
+        if (e & 1) r = h_mulmod(r, b, q);
+        b = h_mulmod(b, b, q);
+        e >>= 1;
+    }
+    return r;
+}
+
+inline uint64_t h_shoup_quotient(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+
+inline void h_barrett_ratio(uint64_t q, uint64_t *r0, uint64_t *r1) {
+    u128 all = ~(u128)0;
+    u128 r = all / q;
+    if (all % q == q - 1) r += 1;
+    *r0 = (uint64_t)r;
+    *r1 = (uint64_t)(r >> 64);
+}
+
+}  // namespace ph

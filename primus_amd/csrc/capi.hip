@@ -1,0 +1,535 @@
+// capi.hip -- the extern "C" boundary declared in include/primus_hip.h.
+//
+// Table construction happens once on the host (exact 128-bit arithmetic) and
+// uploads the Shoup twiddle tables, which then stay resident in HBM for the
+// table's lifetime.  Transforms run on the GPU only: there is no host
+// fallback, and a missing GPU is reported as PH_ERR_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/primus_hip.h"
+#include "arith.hpp"
+#include "ntt_launch.hpp"
+
+using namespace ph;
+
+struct ph_ntt_table {
+    uint32_t log_n;
+    uint64_t n, q, root, inv_root, inv_n, inv_n_q;
+    std::vector<Shoup> fwd, inv, ord;  // host copies (accessors)
+    Shoup *d_fwd = nullptr, *d_inv = nullptr, *d_ord = nullptr;
+    DevTable dev{};                    // device view (pointers into d_*)
+    DevTable *d_tab = nullptr;         // the view itself in device memory (L = 1)
+};
+
+struct ph_dcrt_table {
+    uint32_t log_n, L;
+    std::vector<ph_ntt_table *> tables;
+    DevTable *d_tabs = nullptr;
+};
+
+struct ph_operand {
+    uint32_t L;
+    uint64_t n;
+    Shoup *d = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(PH_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PH_HIP(call)                                      \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+uint64_t rev_bits(uint64_t x, uint32_t bits) {
+    uint64_t r = 0;
+    for (uint32_t i = 0; i < bits; ++i) r |= ((x >> i) & 1ull) << (bits - 1 - i);
+    return r;
+}
+
+// w^(2^(log_degree-1)) == q - 1, primus_ntt/src/root.rs:27-44.
+bool is_primitive_root(uint64_t w, uint32_t log_degree, uint64_t q) {
+    if (w == 0 || log_degree == 0) return false;
+    uint64_t x = w;
+    for (uint32_t i = 0; i + 1 < log_degree; ++i) x = h_mulmod(x, x, q);
+    return x == q - 1;
+}
+
+// try_minimal_primitive_root (primus_ntt/src/root.rs:46-111): the smallest
+// primitive 2^log_degree-th root.  Candidates r = 2, 3, ... replace the
+// reference's random draws; the minimum over odd powers does not depend on
+// which primitive root is found first.
+int minimal_primitive_root(uint32_t log_degree, uint64_t q, uint64_t *root) {
+    uint64_t qm1 = q - 1;
+    uint64_t quotient = qm1 >> log_degree;
+    if ((quotient << log_degree) != qm1) return PH_ERR_NO_PRIMITIVE_ROOT;
+    uint64_t w = 0;
+    bool found = false;
+    for (uint64_t r = 2; r <= qm1 && r < (2ull + (1ull << 20)); ++r) {
+        w = h_powmod(r, quotient, q);
+        if (is_primitive_root(w, log_degree, q)) {
+            found = true;
+            break;
+        }
+    }
+    if (!found) return PH_ERR_NO_PRIMITIVE_ROOT;
+    uint64_t best = w, g2 = h_mulmod(w, w, q), cur = w;
+    const uint64_t degree = 1ull << log_degree;
+    for (uint64_t i = 0; i < degree; ++i) {
+        if (cur < best) best = cur;
+        cur = h_mulmod(g2, cur, q);
+    }
+    *root = best;
+    return PH_OK;
+}
+
+uint64_t inv_mod(uint64_t a, uint64_t q) {
+    __int128 t = 0, nt = 1, r = q, nr = a % q;
+    while (nr != 0) {
+        __int128 k = r / nr, tmp;
+        tmp = t - k * nt; t = nt; nt = tmp;
+        tmp = r - k * nr; r = nr; nr = tmp;
+    }
+    if (r != 1) return 0;
+    if (t < 0) t += q;
+    return (uint64_t)t;
+}
+
+Shoup mk(uint64_t w, uint64_t q) { return Shoup{w, h_shoup_quotient(w, q)}; }
+
+void free_table(ph_ntt_table *t) {
+    if (!t) return;
+    if (t->d_fwd) (void)hipFree(t->d_fwd);
+    if (t->d_inv) (void)hipFree(t->d_inv);
+    if (t->d_ord) (void)hipFree(t->d_ord);
+    if (t->d_tab) (void)hipFree(t->d_tab);
+    delete t;
+}
+
+int upload(void **dst, const void *src, size_t bytes) {
+    PH_HIP(hipMalloc(dst, bytes));
+    PH_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return PH_OK;
+}
+
+// UintNttTable::new, primus_ntt/src/ntt/primitive.rs:114-185.
+int build_table(uint32_t log_n, uint64_t q, ph_ntt_table **out) {
+    if (q <= 1 || (64 - __builtin_clzll(q)) >= 63)
+        return fail(PH_ERR_INVALID, "modulus must satisfy 1 < q < 2^62 (BarrettModulus::new)");
+    if (log_n < kMinLogN || log_n > kMaxLogN)
+        return fail(PH_ERR_UNSUPPORTED, "log_n must be in [1, 17]");
+    uint64_t root = 0;
+    int st = minimal_primitive_root(log_n + 1, q, &root);
+    if (st != PH_OK) return fail(st, "no primitive 2N-th root of unity modulo q");
+    const uint64_t n = 1ull << log_n;
+    if (n >= q) return fail(PH_ERR_DEGREE_TOO_LARGE, "degree must be less than the modulus");
+    ph_ntt_table *t = new (std::nothrow) ph_ntt_table();
+    if (!t) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    t->log_n = log_n;
+    t->n = n;
+    t->q = q;
+    t->root = root;
+    try {
+        t->ord.resize(2 * n);
+        t->fwd.resize(n);
+        t->inv.resize(n);
+    } catch (...) {
+        delete t;
+        return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    }
+    // ordinal_root_powers[k] = root^k (primitive.rs:128-137)
+    uint64_t p = 1 % q;
+    for (uint64_t k = 0; k < 2 * n; ++k) {
+        t->ord[k] = mk(p, q);
+        p = h_mulmod(p, root, q);
+    }
+    t->inv_root = t->ord[2 * n - 1].w;
+    // root_powers[rev(k)] = root^k (:145-149)
+    for (uint64_t k = 0; k < n; ++k) t->fwd[rev_bits(k, log_n)] = t->ord[k];
+    // inv_root_powers[0] = 1, inv_root_powers[rev(k)+1] = root^-(k+1) (:151-159)
+    t->inv[0] = t->ord[0];
+    for (uint64_t k = 0; k + 1 < n; ++k) t->inv[rev_bits(k, log_n) + 1] = t->ord[2 * n - 1 - k];
+    t->inv_n = inv_mod(n % q, q);
+    t->inv_n_q = h_shoup_quotient(t->inv_n, q);
+
+    int rc = upload((void **)&t->d_fwd, t->fwd.data(), n * sizeof(Shoup));
+    if (rc == PH_OK) rc = upload((void **)&t->d_inv, t->inv.data(), n * sizeof(Shoup));
+    if (rc == PH_OK) rc = upload((void **)&t->d_ord, t->ord.data(), 2 * n * sizeof(Shoup));
+    if (rc != PH_OK) {
+        free_table(t);
+        return rc;
+    }
+    DevTable &d = t->dev;
+    d.q = q;
+    d.q2 = q << 1;
+    d.inv_n = t->inv_n;
+    d.inv_n_q = t->inv_n_q;
+    // scaled root of the last inverse stage: N^-1 * inv_root_powers[N-1] (:272-277)
+    const Shoup last = t->inv[n - 1];
+    d.sr = (uint64_t)(((u128)last.w * t->inv_n) % q);
+    d.sr_q = h_shoup_quotient(d.sr, q);
+    h_barrett_ratio(q, &d.br0, &d.br1);
+    d.fwd = t->d_fwd;
+    d.inv = t->d_inv;
+    d.ord = t->d_ord;
+    d.log_n = log_n;
+    rc = upload((void **)&t->d_tab, &d, sizeof(DevTable));
+    if (rc != PH_OK) {
+        free_table(t);
+        return rc;
+    }
+    *out = t;
+    return PH_OK;
+}
+
+// One pinned staging buffer and one stream per host thread for the host-slice
+// entry points.
+struct Staging {
+    void *buf = nullptr;
+    size_t cap = 0;
+    hipStream_t stream = nullptr;
+};
+thread_local Staging g_staging;
+
+int staging(size_t bytes, void **buf, hipStream_t *stream) {
+    Staging &s = g_staging;
+    if (!s.stream) PH_HIP(hipStreamCreate(&s.stream));
+    if (s.cap < bytes) {
+        if (s.buf) PH_HIP(hipFree(s.buf));
+        s.buf = nullptr;
+        s.cap = 0;
+        PH_HIP(hipMalloc(&s.buf, bytes));
+        s.cap = bytes;
+    }
+    *buf = s.buf;
+    *stream = s.stream;
+    return PH_OK;
+}
+
+int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, uint64_t *d_data,
+           const uint64_t *d_data2, const Shoup *operand, uint64_t count, hipStream_t s) {
+    if (count == 0) return PH_OK;
+    KArgs a{};
+    a.data = d_data;
+    a.data2 = d_data2;
+    a.operand = operand;
+    a.tabs = d_tabs;
+    a.L = L;
+    void *scratch = nullptr;
+    if (needs_scratch(op, log_n)) PH_HIP(hipMallocAsync(&scratch, (count << log_n) * 8, s));
+    hipError_t e = launch_transform(op, a, log_n, count, scratch, s);
+    if (scratch) (void)hipFreeAsync(scratch, s);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return PH_OK;
+}
+
+// Host slice -> device -> op -> host slice, synchronous.
+int host_roundtrip(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, uint64_t *poly,
+                   size_t len) {
+    void *buf;
+    hipStream_t s;
+    int rc = staging(len * 8, &buf, &s);
+    if (rc != PH_OK) return rc;
+    PH_HIP(hipMemcpyAsync(buf, poly, len * 8, hipMemcpyHostToDevice, s));
+    rc = run_op(op, d_tabs, L, log_n, (uint64_t *)buf, nullptr, nullptr, len >> log_n, s);
+    if (rc != PH_OK) return rc;
+    PH_HIP(hipMemcpyAsync(poly, buf, len * 8, hipMemcpyDeviceToHost, s));
+    PH_HIP(hipStreamSynchronize(s));
+    return PH_OK;
+}
+
+int check_slice(const void *table, const void *p, size_t len, size_t want) {
+    if (!table || (!p && len)) return fail(PH_ERR_INVALID, "null table or slice");
+    if (len != want) return fail(PH_ERR_INVALID, "slice length does not match the table");
+    return PH_OK;
+}
+
+int monomial(const ph_ntt_table *t, uint64_t coeff, size_t degree, uint64_t *values, size_t len,
+             int kind) {
+    int rc = check_slice(t, values, len, t ? t->n : 0);
+    if (rc != PH_OK) return rc;
+    void *buf;
+    hipStream_t s;
+    rc = staging(len * 8, &buf, &s);
+    if (rc != PH_OK) return rc;
+    hipError_t e = launch_monomial(t->d_tab, t->log_n, (uint64_t *)buf, coeff, (uint64_t)degree, kind, s);
+    if (e != hipSuccess) return hip_fail(e, "monomial kernel");
+    PH_HIP(hipMemcpyAsync(values, buf, len * 8, hipMemcpyDeviceToHost, s));
+    PH_HIP(hipStreamSynchronize(s));
+    return PH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ph_status_string(int status) {
+    switch (status) {
+        case PH_OK: return "ok";
+        case PH_ERR_NO_PRIMITIVE_ROOT: return "no primitive root";
+        case PH_ERR_DEGREE_CONVERSION: return "degree conversion";
+        case PH_ERR_DEGREE_TOO_LARGE: return "degree too large";
+        case PH_ERR_NTT_TABLE: return "ntt table";
+        case PH_ERR_INVALID: return "invalid argument";
+        case PH_ERR_DEVICE: return "device error";
+        case PH_ERR_UNSUPPORTED: return "unsupported ring degree";
+        case PH_ERR_NO_MEMORY: return "out of memory";
+    }
+    return "unknown status";
+}
+
+const char *ph_last_error(void) { return g_last_error.c_str(); }
+
+int ph_ntt_table_new(uint32_t log_n, uint64_t modulus, ph_ntt_table **out) {
+    if (!out) return fail(PH_ERR_INVALID, "null output");
+    *out = nullptr;
+    return build_table(log_n, modulus, out);
+}
+
+void ph_ntt_table_free(ph_ntt_table *table) { free_table(table); }
+
+size_t ph_ntt_table_poly_length(const ph_ntt_table *t) { return t ? t->n : 0; }
+uint32_t ph_ntt_table_log_n(const ph_ntt_table *t) { return t ? t->log_n : 0; }
+uint64_t ph_ntt_table_modulus(const ph_ntt_table *t) { return t ? t->q : 0; }
+uint64_t ph_ntt_table_root(const ph_ntt_table *t) { return t ? t->root : 0; }
+uint64_t ph_ntt_table_inv_root(const ph_ntt_table *t) { return t ? t->inv_root : 0; }
+
+int ph_ntt_table_inv_n(const ph_ntt_table *t, uint64_t *value, uint64_t *quotient) {
+    if (!t) return fail(PH_ERR_INVALID, "null table");
+    if (value) *value = t->inv_n;
+    if (quotient) *quotient = t->inv_n_q;
+    return PH_OK;
+}
+
+int ph_ntt_table_powers(const ph_ntt_table *t, int which, uint64_t *values, uint64_t *quotients) {
+    if (!t || which < 0 || which > 2) return fail(PH_ERR_INVALID, "bad table or selector");
+    const std::vector<Shoup> &v = which == 0 ? t->fwd : which == 1 ? t->inv : t->ord;
+    for (size_t i = 0; i < v.size(); ++i) {
+        if (values) values[i] = v[i].w;
+        if (quotients) quotients[i] = v[i].wq;
+    }
+    return PH_OK;
+}
+
+int ph_lazy_transform_slice(const ph_ntt_table *t, uint64_t *poly, size_t len) {
+    int rc = check_slice(t, poly, len, t ? t->n : 0);
+    return rc != PH_OK ? rc : host_roundtrip(Op::FwdLazy, t->d_tab, 1, t->log_n, poly, len);
+}
+
+int ph_transform_slice(const ph_ntt_table *t, uint64_t *poly, size_t len) {
+    int rc = check_slice(t, poly, len, t ? t->n : 0);
+    return rc != PH_OK ? rc : host_roundtrip(Op::FwdCanon, t->d_tab, 1, t->log_n, poly, len);
+}
+
+int ph_lazy_inverse_transform_slice(const ph_ntt_table *t, uint64_t *values, size_t len) {
+    int rc = check_slice(t, values, len, t ? t->n : 0);
+    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, values, len);
+}
+
+int ph_inverse_transform_slice(const ph_ntt_table *t, uint64_t *values, size_t len) {
+    int rc = check_slice(t, values, len, t ? t->n : 0);
+    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, values, len);
+}
+
+int ph_transform_monomial(const ph_ntt_table *t, uint64_t coeff, size_t degree, uint64_t *values,
+                          size_t len) {
+    return monomial(t, coeff, degree, values, len, 0);
+}
+
+int ph_transform_coeff_one_monomial(const ph_ntt_table *t, size_t degree, uint64_t *values, size_t len) {
+    return monomial(t, 0, degree, values, len, 1);
+}
+
+int ph_transform_coeff_minus_one_monomial(const ph_ntt_table *t, size_t degree, uint64_t *values,
+                                          size_t len) {
+    return monomial(t, 0, degree, values, len, 2);
+}
+
+int ph_dcrt_table_new(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_count, ph_dcrt_table **out) {
+    if (!out || !moduli || moduli_count == 0) return fail(PH_ERR_INVALID, "null moduli or output");
+    *out = nullptr;
+    ph_dcrt_table *d = new (std::nothrow) ph_dcrt_table();
+    if (!d) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    d->log_n = log_n;
+    d->L = moduli_count;
+    std::vector<DevTable> views;
+    for (uint32_t i = 0; i < moduli_count; ++i) {
+        ph_ntt_table *t = nullptr;
+        int rc = build_table(log_n, moduli[i], &t);
+        if (rc != PH_OK) {
+            ph_dcrt_table_free(d);
+            return rc;
+        }
+        d->tables.push_back(t);
+        views.push_back(t->dev);
+    }
+    int rc = upload((void **)&d->d_tabs, views.data(), views.size() * sizeof(DevTable));
+    if (rc != PH_OK) {
+        ph_dcrt_table_free(d);
+        return rc;
+    }
+    *out = d;
+    return PH_OK;
+}
+
+void ph_dcrt_table_free(ph_dcrt_table *d) {
+    if (!d) return;
+    for (ph_ntt_table *t : d->tables) free_table(t);
+    if (d->d_tabs) (void)hipFree(d->d_tabs);
+    delete d;
+}
+
+size_t ph_dcrt_poly_length(const ph_dcrt_table *d) { return d ? (size_t)1 << d->log_n : 0; }
+size_t ph_dcrt_moduli_count(const ph_dcrt_table *d) { return d ? d->L : 0; }
+size_t ph_dcrt_crt_poly_length(const ph_dcrt_table *d) { return d ? ((size_t)d->L << d->log_n) : 0; }
+
+const ph_ntt_table *ph_dcrt_ntt_table(const ph_dcrt_table *d, uint32_t i) {
+    return (d && i < d->L) ? d->tables[i] : nullptr;
+}
+
+static int dcrt_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len, Op op) {
+    int rc = check_slice(d, poly, len, d ? ((size_t)d->L << d->log_n) : 0);
+    return rc != PH_OK ? rc : host_roundtrip(op, d->d_tabs, d->L, d->log_n, poly, len);
+}
+
+int ph_dcrt_lazy_transform_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len) {
+    return dcrt_slice(d, poly, len, Op::FwdLazy);
+}
+int ph_dcrt_transform_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len) {
+    return dcrt_slice(d, poly, len, Op::FwdCanon);
+}
+int ph_dcrt_lazy_inverse_transform_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len) {
+    return dcrt_slice(d, poly, len, Op::Inv);
+}
+int ph_dcrt_inverse_transform_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len) {
+    return dcrt_slice(d, poly, len, Op::Inv);
+}
+
+int ph_dcrt_transform_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t count, int lazy, void *stream) {
+    if (!d || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
+    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, d->d_tabs, d->L, d->log_n, d_polys, nullptr, nullptr,
+                  count, (hipStream_t)stream);
+}
+
+int ph_dcrt_inverse_transform_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t count, void *stream) {
+    if (!d || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
+    return run_op(Op::Inv, d->d_tabs, d->L, d->log_n, d_polys, nullptr, nullptr, count, (hipStream_t)stream);
+}
+
+int ph_ntt_transform_device(const ph_ntt_table *t, uint64_t *d_polys, size_t count, int lazy, void *stream) {
+    if (!t || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
+    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, t->d_tab, 1, t->log_n, d_polys, nullptr, nullptr, count,
+                  (hipStream_t)stream);
+}
+
+int ph_ntt_inverse_transform_device(const ph_ntt_table *t, uint64_t *d_polys, size_t count, void *stream) {
+    if (!t || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
+    return run_op(Op::Inv, t->d_tab, 1, t->log_n, d_polys, nullptr, nullptr, count, (hipStream_t)stream);
+}
+
+int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_operand **out) {
+    if (!d || !host_values || !out) return fail(PH_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const uint64_t n = 1ull << d->log_n;
+    std::vector<Shoup> h((size_t)d->L * n);
+    for (uint32_t l = 0; l < d->L; ++l) {
+        const uint64_t q = d->tables[l]->q;
+        for (uint64_t i = 0; i < n; ++i) {
+            uint64_t v = host_values[l * n + i];
+            if (v >= q) return fail(PH_ERR_INVALID, "operand values must be canonical (< q)");
+            h[l * n + i] = mk(v, q);
+        }
+    }
+    ph_operand *op = new (std::nothrow) ph_operand();
+    if (!op) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    op->L = d->L;
+    op->n = n;
+    int rc = upload((void **)&op->d, h.data(), h.size() * sizeof(Shoup));
+    if (rc != PH_OK) {
+        delete op;
+        return rc;
+    }
+    *out = op;
+    return PH_OK;
+}
+
+void ph_operand_free(ph_operand *op) {
+    if (!op) return;
+    if (op->d) (void)hipFree(op->d);
+    delete op;
+}
+
+int ph_dcrt_fwd_mul_inv_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t count, const ph_operand *op,
+                               void *stream) {
+    if (!d || !op || (!d_polys && count)) return fail(PH_ERR_INVALID, "null argument");
+    if (op->L != d->L || op->n != (1ull << d->log_n)) return fail(PH_ERR_INVALID, "operand shape mismatch");
+    return run_op(Op::FwdMulInv, d->d_tabs, d->L, d->log_n, d_polys, nullptr, op->d, count, (hipStream_t)stream);
+}
+
+int ph_dcrt_negacyclic_mul_device(const ph_dcrt_table *d, uint64_t *d_a, const uint64_t *d_b, size_t count,
+                                  void *stream) {
+    if (!d || ((!d_a || !d_b) && count)) return fail(PH_ERR_INVALID, "null argument");
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d_a, d_b, nullptr, count, (hipStream_t)stream);
+}
+
+int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *d, const uint64_t *d_a, const uint64_t *d_b,
+                                 uint64_t *d_out, size_t count, void *stream) {
+    if (!d || ((!d_a || !d_b || !d_out) && count)) return fail(PH_ERR_INVALID, "null argument");
+    hipError_t e = launch_pointwise(d->d_tabs, d->L, d->log_n, (uint64_t)count << d->log_n, d_a, d_b, d_out, 0,
+                                    (hipStream_t)stream);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "pointwise kernel");
+}
+
+int ph_dcrt_pointwise_mul_add_device(const ph_dcrt_table *d, uint64_t *d_acc, const uint64_t *d_a,
+                                     const uint64_t *d_b, size_t count, void *stream) {
+    if (!d || ((!d_a || !d_b || !d_acc) && count)) return fail(PH_ERR_INVALID, "null argument");
+    hipError_t e = launch_pointwise(d->d_tabs, d->L, d->log_n, (uint64_t)count << d->log_n, d_a, d_b, d_acc, 1,
+                                    (hipStream_t)stream);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "pointwise kernel");
+}
+
+int ph_device_alloc(size_t bytes, void **d_ptr) {
+    if (!d_ptr) return fail(PH_ERR_INVALID, "null output");
+    PH_HIP(hipMalloc(d_ptr, bytes));
+    return PH_OK;
+}
+
+int ph_device_free(void *d_ptr) {
+    PH_HIP(hipFree(d_ptr));
+    return PH_OK;
+}
+
+int ph_memcpy_htod(void *d_dst, const void *h_src, size_t bytes) {
+    PH_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+    return PH_OK;
+}
+
+int ph_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes) {
+    PH_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+    return PH_OK;
+}
+
+int ph_stream_synchronize(void *stream) {
+    PH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return PH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,426 @@
+// ntt_kernels.hip -- gfx950 kernels of the negacyclic NTT and the fused
+// transform pipelines built on it.
+//
+// Two kernel shapes cover ring degrees 2^1 .. 2^17:
+//
+// * chunk_kernel<LOGC, LOGT, MODE>: one workgroup of 2^LOGT threads owns a
+//   contiguous chunk of 2^LOGC coefficients in LDS (plus one pad word per 16)
+//   and runs the chunk's stages as radix-2^LOGE register rounds, E = 2^LOGE
+//   coefficients per thread.  For N <= 2^13 the chunk is the whole
+//   polynomial (single pass, one HBM round trip); the forward's last round and
+//   the inverse's first round have the same register layout, so a fused
+//   forward -> pointwise -> inverse needs no LDS exchange in the middle.
+// * column_kernel<LOGR, MODE>: for N >= 2^14 the first log2(N/4096) forward
+//   stages (the last ones of the inverse) act on columns k = i*4096 + j;
+//   each thread runs one column's 2^LOGR coefficients in registers, and the
+//   4096-wide rows of a wave are read and written coalesced.  Together with a
+//   chunk pass of 4096 this is the two-pass (four-step) schedule.
+//
+// MODE bits select what a chunk does between its load and its store.
+#include "ntt_device.hpp"
+#include "ntt_launch.hpp"
+
+namespace ph {
+
+enum : int {
+    M_FWD = 1,        // forward stages of this chunk
+    M_INV = 2,        // inverse stages of this chunk
+    M_MUL_SHOUP = 4,  // multiply by a resident Shoup operand between them
+    M_MUL_POLY = 8,   // multiply by the transform of a second polynomial (Barrett)
+    M_CANON = 16,     // forward output reduced to [0, q)
+    M_INV_LAST = 32,  // the chunk holds the inverse's final stage (single pass)
+};
+
+template <int LOGC, int LOGT>
+struct Geo {
+    static constexpr int C = 1 << LOGC;
+    static constexpr int T = 1 << LOGT;
+    static constexpr int LOGE = LOGC - LOGT;
+    static constexpr int E = 1 << LOGE;
+    static constexpr int REM = LOGC % LOGE;
+    static constexpr int NR = LOGC / LOGE + (REM ? 1 : 0);
+    static constexpr int LDS = C + (C >> 4);
+    // forward rounds: a short round first (if any), then full rounds
+    static constexpr int fr(int i) { return (REM != 0 && i == 0) ? REM : LOGE; }
+    static constexpr int fs(int i) { return i == 0 ? 0 : (REM != 0 ? REM + (i - 1) * LOGE : i * LOGE); }
+    // inverse rounds: full rounds, then the short one
+    static constexpr int ir(int i) { return (REM != 0 && i == NR - 1) ? REM : LOGE; }
+    static constexpr int is(int i) { return i * LOGE; }
+    static_assert(LOGE >= 1, "a thread needs at least two coefficients");
+};
+
+// Local slot of register (m, j) in the forward round covering local stages [S, S+R).
+template <int LOGC, int LOGT, int S, int R>
+__device__ __forceinline__ uint32_t fwd_slot(uint32_t t, int m, int j) {
+    constexpr int OB = LOGC - S - R;
+    const uint32_t gi = t + ((uint32_t)m << LOGT);
+    const uint32_t c = gi >> OB;
+    const uint32_t o = gi & ((1u << OB) - 1u);
+    return (c << (LOGC - S)) + ((uint32_t)j << OB) + o;
+}
+
+// Local slot of register (m, j) in the inverse round covering local stages [P, P+R).
+template <int LOGC, int LOGT, int P, int R>
+__device__ __forceinline__ uint32_t inv_slot(uint32_t t, int m, int j) {
+    const uint32_t gi = t + ((uint32_t)m << LOGT);
+    const uint32_t c = gi >> P;
+    const uint32_t o = gi & ((1u << P) - 1u);
+    return (c << (P + R)) + ((uint32_t)j << P) + o;
+}
+
+template <int LOGC, int LOGT, int I>
+struct FwdRounds {
+    using G = Geo<LOGC, LOGT>;
+    static constexpr int S = G::fs(I), R = G::fr(I);
+
+    __device__ __forceinline__ static void store(uint64_t *lds, const uint64_t *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))] = x[(m << R) + j];
+    }
+    __device__ __forceinline__ static void load(const uint64_t *lds, uint64_t *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))];
+    }
+    __device__ __forceinline__ static void load_global(const uint64_t *__restrict__ g, uint64_t *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = g[fwd_slot<LOGC, LOGT, S, R>(t, m, j)];
+    }
+
+    // Runs rounds I .. NR-1.  st0: global stage of local stage 0; b: chunk index
+    // of this workgroup inside the polynomial.
+    __device__ __forceinline__ static void run(uint64_t *x, uint64_t *lds, uint32_t t, uint32_t st0,
+                                               uint32_t b, const Shoup *__restrict__ tw, uint64_t q,
+                                               uint64_t q2) {
+        if constexpr (I > 0) {
+            __syncthreads();
+            FwdRounds<LOGC, LOGT, I - 1>::store(lds, x, t);
+            __syncthreads();
+            load(lds, x, t);
+        }
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m) {
+            const uint32_t gi = t + ((uint32_t)m << LOGT);
+            const uint32_t c = (b << S) + (gi >> (LOGC - S - R));
+            ct_round<R>(x + (m << R), st0 + S, c, tw, q, q2);
+        }
+        if constexpr (I + 1 < G::NR) FwdRounds<LOGC, LOGT, I + 1>::run(x, lds, t, st0, b, tw, q, q2);
+    }
+};
+
+template <int LOGC, int LOGT, int I, bool LAST>
+struct InvRounds {
+    using G = Geo<LOGC, LOGT>;
+    static constexpr int P = G::is(I), R = G::ir(I);
+
+    __device__ __forceinline__ static void store(uint64_t *lds, const uint64_t *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))] = x[(m << R) + j];
+    }
+    __device__ __forceinline__ static void load(const uint64_t *lds, uint64_t *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))];
+    }
+    __device__ __forceinline__ static void store_global(uint64_t *__restrict__ g, const uint64_t *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) g[inv_slot<LOGC, LOGT, P, R>(t, m, j)] = x[(m << R) + j];
+    }
+
+    __device__ __forceinline__ static void run(uint64_t *x, uint64_t *lds, uint32_t t, uint32_t n,
+                                               uint32_t b, const Shoup *__restrict__ itw,
+                                               const DevTable &tab) {
+        if constexpr (I > 0) {
+            __syncthreads();
+            InvRounds<LOGC, LOGT, I - 1, LAST>::store(lds, x, t);
+            __syncthreads();
+            load(lds, x, t);
+        }
+        constexpr bool last_here = LAST && (I + 1 == G::NR);
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m) {
+            const uint32_t gi = t + ((uint32_t)m << LOGT);
+            const uint32_t c = (b << (LOGC - P - R)) + (gi >> P);
+            gs_round<R, last_here>(x + (m << R), P, c, n, itw, tab);
+        }
+        if constexpr (I + 1 < G::NR) InvRounds<LOGC, LOGT, I + 1, LAST>::run(x, lds, t, n, b, itw, tab);
+    }
+};
+
+template <int LOGC, int LOGT, int MODE>
+__global__ __launch_bounds__(1 << LOGT) void chunk_kernel(KArgs a) {
+    using G = Geo<LOGC, LOGT>;
+    constexpr int LE = G::LOGE;
+    __shared__ uint64_t lds[G::LDS];
+    const uint32_t t = threadIdx.x;
+    const uint64_t blk = blockIdx.x;
+    const uint64_t poly = blk / a.blocks_per_poly;
+    const uint32_t b = (uint32_t)(blk - poly * a.blocks_per_poly);
+    const uint32_t limb = (uint32_t)(poly % a.L);
+    const DevTable tab = a.tabs[limb];
+    const uint32_t n = 1u << tab.log_n;
+    const uint32_t st0 = tab.log_n - LOGC;
+    const uint64_t off = poly * n + (uint64_t)b * G::C;
+    uint64_t *data = a.data + off;
+    uint64_t x[G::E];
+
+    if constexpr ((MODE & M_FWD) != 0) {
+        if constexpr ((MODE & M_MUL_POLY) != 0) {
+            // transform the second operand first and keep it in registers
+            uint64_t y[G::E];
+            FwdRounds<LOGC, LOGT, 0>::load_global(a.data2 + off, y, t);
+            FwdRounds<LOGC, LOGT, 0>::run(y, lds, t, st0, b, tab.fwd, tab.q, tab.q2);
+            FwdRounds<LOGC, LOGT, 0>::load_global(data, x, t);
+            FwdRounds<LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab.fwd, tab.q, tab.q2);
+            // NttPolynomial::mul_assign on canonical values (primus_poly/src/ntt/mul.rs:87-95)
+#pragma unroll
+            for (int j = 0; j < G::E; ++j) {
+                uint64_t xa = sub_if_ge(sub_if_ge(x[j], tab.q2), tab.q);
+                uint64_t yb = sub_if_ge(sub_if_ge(y[j], tab.q2), tab.q);
+                x[j] = barrett_mul(xa, yb, tab.q, tab.br0, tab.br1);
+            }
+        } else {
+            FwdRounds<LOGC, LOGT, 0>::load_global(data, x, t);
+            FwdRounds<LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab.fwd, tab.q, tab.q2);
+        }
+        // after the last forward round thread t holds local slots t*E .. t*E+E-1
+        if constexpr ((MODE & M_MUL_SHOUP) != 0) {
+            const Shoup *op = a.operand + (uint64_t)limb * n + (uint64_t)b * G::C + ((uint64_t)t << LE);
+#pragma unroll
+            for (int j = 0; j < G::E; ++j) {
+                const Shoup w = op[j];
+                x[j] = shoup_full(w.w, w.wq, x[j], tab.q);
+            }
+        }
+        if constexpr ((MODE & M_INV) == 0) {
+            uint64_t *dst = data + ((uint64_t)t << LE);
+#pragma unroll
+            for (int j = 0; j < G::E; ++j) {
+                uint64_t v = x[j];
+                if constexpr ((MODE & M_CANON) != 0) v = sub_if_ge(sub_if_ge(v, tab.q2), tab.q);
+                dst[j] = v;
+            }
+            return;
+        }
+    } else {
+        const uint64_t *src = data + ((uint64_t)t << LE);
+#pragma unroll
+        for (int j = 0; j < G::E; ++j) x[j] = src[j];
+    }
+
+    if constexpr ((MODE & M_INV) != 0) {
+        constexpr bool LAST = (MODE & M_INV_LAST) != 0;
+        InvRounds<LOGC, LOGT, 0, LAST>::run(x, lds, t, n, b, tab.inv, tab);
+        InvRounds<LOGC, LOGT, G::NR - 1, LAST>::store_global(data, x, t);
+    }
+}
+
+// Column pass of the two-pass schedule: forward stages [0, LOGR) or inverse
+// stages [logN-LOGR, logN) on columns k = i*C + j, C = N >> LOGR.
+template <int LOGR, int MODE>
+__global__ __launch_bounds__(256) void column_kernel(KArgs a) {
+    const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const DevTable tab0 = a.tabs[0];
+    const uint32_t logc = tab0.log_n - LOGR;
+    const uint64_t poly = gid >> logc;
+    const uint32_t j = (uint32_t)(gid & ((1u << logc) - 1u));
+    const uint32_t limb = (uint32_t)(poly % a.L);
+    const DevTable tab = a.tabs[limb];
+    const uint32_t n = 1u << tab.log_n;
+    const uint64_t off = poly * n + j;
+    const uint64_t *src = a.data2 + off;
+    uint64_t *dst = a.data + off;
+    uint64_t x[1 << LOGR];
+#pragma unroll
+    for (int i = 0; i < (1 << LOGR); ++i) x[i] = src[(uint64_t)i << logc];
+    if constexpr ((MODE & M_FWD) != 0) {
+        ct_round<LOGR>(x, 0, 0, tab.fwd, tab.q, tab.q2);
+    } else {
+        gs_round<LOGR, true>(x, logc, 0, n, tab.inv, tab);
+    }
+#pragma unroll
+    for (int i = 0; i < (1 << LOGR); ++i) dst[(uint64_t)i << logc] = x[i];
+}
+
+// ---------------------------------------------------------------------------
+// element-wise kernels
+// ---------------------------------------------------------------------------
+
+// transform_monomial family, primus_ntt/src/ntt/primitive.rs:298-397.
+// kind 0: coeff * X^degree, 1: X^degree, 2: -X^degree.
+__global__ void monomial_kernel(const DevTable *tabs, uint64_t *out, uint64_t coeff,
+                                uint64_t degree, int kind) {
+    const DevTable t = tabs[0];
+    const uint32_t n = 1u << t.log_n;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint64_t mask = 2ull * n - 1ull;
+    const uint64_t r = __brev(k) >> (32 - t.log_n);
+    const uint64_t idx = ((2ull * r + 1ull) * degree) & mask;
+    uint64_t v;
+    if (kind == 1) {
+        v = degree == 0 ? 1ull : t.ord[idx].w;
+    } else if (kind == 2) {
+        v = degree == 0 ? t.q - 1 : t.ord[idx ^ n].w;
+    } else {
+        if (coeff == 0) v = 0;
+        else if (degree == 0) v = coeff;
+        else if (coeff == 1) v = t.ord[idx].w;
+        else if (coeff == t.q - 1) v = t.ord[idx ^ n].w;
+        else v = shoup_full(t.ord[idx].w, t.ord[idx].wq, coeff, t.q);
+    }
+    out[k] = v;
+}
+
+// out = a * b (mod q) or out = acc + a * b (mod q), limb-major [count][N],
+// limb = poly % L.  primus_poly/src/ntt/mul.rs:87-119, primus_poly/src/ntt/mod.rs:97-108.
+__global__ void pointwise_kernel(const DevTable *tabs, uint32_t L, uint32_t log_n, uint64_t total,
+                                 const uint64_t *a, const uint64_t *b, uint64_t *out, int mac) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t limb = (uint32_t)((i >> log_n) % L);
+    const DevTable t = tabs[limb];
+    uint64_t v = mac ? barrett_mul_add(a[i], b[i], out[i], t.q, t.br0, t.br1)
+                     : barrett_mul(a[i], b[i], t.q, t.br0, t.br1);
+    out[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+template <int LOGC, int LOGT, int MODE>
+static hipError_t launch_chunk(const KArgs &a, uint64_t count, hipStream_t s) {
+    const uint64_t blocks = count * a.blocks_per_poly;
+    if (blocks == 0) return hipSuccess;
+    chunk_kernel<LOGC, LOGT, MODE><<<dim3((uint32_t)blocks), dim3(1u << LOGT), 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int LOGR, int MODE>
+static hipError_t launch_column(const KArgs &a, uint32_t log_n, uint64_t count, hipStream_t s) {
+    const uint64_t threads = count << (log_n - LOGR);
+    if (threads == 0) return hipSuccess;
+    column_kernel<LOGR, MODE><<<dim3((uint32_t)(threads / 256u)), dim3(256), 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// Single-pass chunk launch for log_n = LOGN <= 13 with the mode chosen at run time.
+template <int LOGN>
+static hipError_t single_pass(Op op, KArgs a, uint64_t count, hipStream_t s) {
+    constexpr int LOGT = LOGN > 4 ? LOGN - 4 : 0;
+    a.blocks_per_poly = 1;
+    switch (op) {
+        case Op::FwdLazy: return launch_chunk<LOGN, LOGT, M_FWD>(a, count, s);
+        case Op::FwdCanon: return launch_chunk<LOGN, LOGT, M_FWD | M_CANON>(a, count, s);
+        case Op::Inv: return launch_chunk<LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
+        case Op::FwdMulInv:
+            return launch_chunk<LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count, s);
+        case Op::PolyMul:
+            return launch_chunk<LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+// Two-pass schedule for log_n = 12 + LOGR: columns of 2^LOGR, chunks of 4096.
+template <int LOGR>
+static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStream_t s) {
+    constexpr int LOGC = 12, LOGT = 8;
+    const uint32_t log_n = LOGC + LOGR;
+    a.blocks_per_poly = 1u << LOGR;
+    hipError_t e = hipSuccess;
+    KArgs col = a;
+    switch (op) {
+        case Op::FwdLazy:
+        case Op::FwdCanon:
+            col.data2 = a.data;
+            e = launch_column<LOGR, M_FWD>(col, log_n, count, s);
+            if (e != hipSuccess) return e;
+            if (op == Op::FwdLazy) return launch_chunk<LOGC, LOGT, M_FWD>(a, count, s);
+            return launch_chunk<LOGC, LOGT, M_FWD | M_CANON>(a, count, s);
+        case Op::Inv:
+            e = launch_chunk<LOGC, LOGT, M_INV>(a, count, s);
+            if (e != hipSuccess) return e;
+            col.data2 = a.data;
+            return launch_column<LOGR, M_INV>(col, log_n, count, s);
+        case Op::FwdMulInv:
+            col.data2 = a.data;
+            e = launch_column<LOGR, M_FWD>(col, log_n, count, s);
+            if (e != hipSuccess) return e;
+            e = launch_chunk<LOGC, LOGT, M_FWD | M_MUL_SHOUP | M_INV>(a, count, s);
+            if (e != hipSuccess) return e;
+            return launch_column<LOGR, M_INV>(col, log_n, count, s);
+        case Op::PolyMul: {
+            // the second operand's column pass goes to scratch, leaving the caller's b intact
+            KArgs colb = a;
+            colb.data = (uint64_t *)scratch;
+            colb.data2 = a.data2;
+            e = launch_column<LOGR, M_FWD>(colb, log_n, count, s);
+            if (e != hipSuccess) return e;
+            col.data2 = a.data;
+            e = launch_column<LOGR, M_FWD>(col, log_n, count, s);
+            if (e != hipSuccess) return e;
+            KArgs ch = a;
+            ch.data2 = (const uint64_t *)scratch;
+            e = launch_chunk<LOGC, LOGT, M_FWD | M_MUL_POLY | M_INV>(ch, count, s);
+            if (e != hipSuccess) return e;
+            return launch_column<LOGR, M_INV>(col, log_n, count, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+bool needs_scratch(Op op, uint32_t log_n) { return op == Op::PolyMul && log_n > kMaxSinglePassLogN; }
+
+hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch,
+                            hipStream_t s) {
+    switch (log_n) {
+        case 1: return single_pass<1>(op, a, count, s);
+        case 2: return single_pass<2>(op, a, count, s);
+        case 3: return single_pass<3>(op, a, count, s);
+        case 4: return single_pass<4>(op, a, count, s);
+        case 5: return single_pass<5>(op, a, count, s);
+        case 6: return single_pass<6>(op, a, count, s);
+        case 7: return single_pass<7>(op, a, count, s);
+        case 8: return single_pass<8>(op, a, count, s);
+        case 9: return single_pass<9>(op, a, count, s);
+        case 10: return single_pass<10>(op, a, count, s);
+        case 11: return single_pass<11>(op, a, count, s);
+        case 12: return single_pass<12>(op, a, count, s);
+        case 13: return single_pass<13>(op, a, count, s);
+        case 14: return two_pass<2>(op, a, count, scratch, s);
+        case 15: return two_pass<3>(op, a, count, scratch, s);
+        case 16: return two_pass<4>(op, a, count, scratch, s);
+        case 17: return two_pass<5>(op, a, count, scratch, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, uint64_t coeff,
+                           uint64_t degree, int kind, hipStream_t s) {
+    const uint32_t n = 1u << log_n;
+    const uint32_t tpb = n < 256 ? n : 256;
+    monomial_kernel<<<dim3((n + tpb - 1) / tpb), dim3(tpb), 0, s>>>(tabs, out, coeff, degree, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_pointwise(const DevTable *tabs, uint32_t L, uint32_t log_n, uint64_t total,
+                            const uint64_t *a, const uint64_t *b, uint64_t *out, int mac,
+                            hipStream_t s) {
+    if (total == 0) return hipSuccess;
+    const uint64_t blocks = (total + 255) / 256;
+    pointwise_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(tabs, L, log_n, total, a, b, out, mac);
+    return hipGetLastError();
+}
+
+}  // namespace ph

@@ -1,0 +1,42 @@
+// ntt_launch.hpp -- host-side view of the NTT kernels (ntt_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ntt_device.hpp"
+
+namespace ph {
+
+constexpr uint32_t kMinLogN = 1;
+constexpr uint32_t kMaxLogN = 17;
+constexpr uint32_t kMaxSinglePassLogN = 13;
+
+// Kernel arguments, passed by value.
+struct KArgs {
+    uint64_t *data;          // in/out limb-polynomials, [count][N]
+    const uint64_t *data2;   // second operand (poly-multiply) / column-pass source
+    const Shoup *operand;    // Shoup-prepared NTT-domain operand, [L][N]
+    const DevTable *tabs;    // L device tables; limb-polynomial i uses tabs[i % L]
+    uint32_t L;
+    uint32_t blocks_per_poly;
+};
+
+enum class Op : int {
+    FwdLazy,    // lazy_transform_slice:          [0,4q) -> [0,4q)
+    FwdCanon,   // transform_slice:               [0,4q) -> [0,q)
+    Inv,        // (lazy_)inverse_transform_slice: [0,2q) -> [0,q)
+    FwdMulInv,  // forward, Shoup multiply by a resident operand, inverse
+    PolyMul,    // negacyclic product: forward both, Barrett multiply, inverse
+};
+
+bool needs_scratch(Op op, uint32_t log_n);
+hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch,
+                            hipStream_t s);
+hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, uint64_t coeff,
+                           uint64_t degree, int kind, hipStream_t s);
+hipError_t launch_pointwise(const DevTable *tabs, uint32_t L, uint32_t log_n, uint64_t total,
+                            const uint64_t *a, const uint64_t *b, uint64_t *out, int mac,
+                            hipStream_t s);
+
+}  // namespace ph

@@ -1,0 +1,60 @@
+"""CPU-side checks of the C ABI: the library loads, exports every function the
+header declares, and rejects bad arguments before touching the GPU."""
+import ctypes
+import os
+
+import pytest
+
+import primus_amd as pa
+from primus_amd import _lib
+
+
+def test_library_built_and_loads():
+    assert os.path.exists(_lib.LIB_PATH), "run __graft_entry__.build() first"
+    pa.lib()
+
+
+def test_every_header_function_is_exported():
+    names = pa.header_functions()
+    assert len(names) >= 40
+    L = pa.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_status_strings():
+    L = pa.lib()
+    assert L.ph_status_string(0) == b"ok"
+    assert L.ph_status_string(1) == b"no primitive root"
+    assert L.ph_status_string(3) == b"degree too large"
+    assert L.ph_status_string(99) == b"unknown status"
+
+
+@pytest.mark.parametrize("q", [0, 1, 1 << 62, (1 << 64) - 1])
+def test_bad_modulus_rejected_on_host(q):
+    with pytest.raises(ValueError):
+        pa.UintNttTable.new(10, q)
+
+
+def test_no_primitive_root_on_host():
+    # 2N = 8192 does not divide q - 1 = 12288: NttError::NoPrimitiveRoot
+    with pytest.raises(pa.NoPrimitiveRoot):
+        pa.UintNttTable.new(12, 12289)
+    with pytest.raises(pa.NoPrimitiveRoot):
+        pa.UintCrtNttTable.new(12, [12289, 1125899906826241])
+
+
+@pytest.mark.parametrize("log_n", [0, 18, 30])
+def test_unsupported_degree(log_n):
+    with pytest.raises(pa.PrimusHipError) as e:
+        pa.UintNttTable.new(log_n, 576460752303421441)
+    assert e.value.code == 7
+
+
+def test_null_arguments():
+    L = pa.lib()
+    out = ctypes.c_void_p()
+    assert L.ph_ntt_table_new(10, 2147473409, None) == 5
+    assert L.ph_transform_slice(None, None, 0) == 5
+    assert L.ph_dcrt_table_new(10, None, 0, ctypes.byref(out)) == 5
+    assert L.ph_ntt_table_poly_length(None) == 0

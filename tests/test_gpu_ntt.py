@@ -1,0 +1,232 @@
+"""GPU parity of the HIP NTT path against the oracle and the golden vectors.
+
+Every comparison is bit-exact: the lazy forward output too, since the kernels
+run the reference's Harvey butterflies one for one.
+"""
+import numpy as np
+import pytest
+
+import primus_amd as pa
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CFG2_PRIMES = [1125899906826241, 1125899906629633, 1125899906424833, 1125899906260993]
+
+
+def _rand(q, shape, seed):
+    return np.random.default_rng(seed).integers(0, q, size=shape, dtype=np.uint64)
+
+
+# ---------------------------------------------------------------- golden vectors
+
+def test_golden_slices(golden):
+    for label, c in golden.items():
+        q, log_n = int(c["q"][0]), int(c["log_n"][0])
+        print("golden", label, flush=True)
+        t = pa.UintNttTable.new(log_n, q)
+        assert t.root() == int(c["psi"][0]), label
+        x = c["a"].copy()
+        t.transform_slice(x)
+        np.testing.assert_array_equal(x, c["fwd_a"], err_msg=label)
+        t.inverse_transform_slice(x)
+        np.testing.assert_array_equal(x, c["a"], err_msg=label)
+        if "inv_in" in c:
+            y = c["inv_in"].copy()
+            t.inverse_transform_slice(y)
+            np.testing.assert_array_equal(y, c["inv_out"], err_msg=label)
+        coeff, deg = (int(v) for v in c["mono_params"])
+        m = np.zeros(1 << log_n, dtype=np.uint64)
+        t.transform_monomial(coeff, deg, m)
+        np.testing.assert_array_equal(m, c["mono"], err_msg=label)
+        t.transform_coeff_one_monomial(deg, m)
+        np.testing.assert_array_equal(m, c["mono_one"], err_msg=label)
+        t.transform_coeff_minus_one_monomial(deg, m)
+        np.testing.assert_array_equal(m, c["mono_minus_one"], err_msg=label)
+
+
+def test_golden_negacyclic_device(golden):
+    for label, c in golden.items():
+        if "prod" not in c:
+            continue
+        q, log_n = int(c["q"][0]), int(c["log_n"][0])
+        tab = pa.UintCrtNttTable.new(log_n, [q])
+        da, db = pa.DeviceBuffer.from_array(c["a"]), pa.DeviceBuffer.from_array(c["b"])
+        pa.negacyclic_mul_device(tab, da.ptr, db.ptr, 1)
+        pa.synchronize()
+        np.testing.assert_array_equal(da.download(), c["prod"], err_msg=label)
+        np.testing.assert_array_equal(db.download(), c["b"], err_msg=label)
+
+
+# ---------------------------------------------------------------- every ring degree
+
+@pytest.mark.parametrize("log_n", list(range(1, 18)))
+def test_all_degrees_bit_exact(log_n):
+    n = 1 << log_n
+    q = pa.ntt_primes_below(59, 2 * n, 1)[0]
+    print("degree", log_n, q, flush=True)
+    t = pa.UintNttTable.new(log_n, q)
+    ot = O.Table(log_n, q)
+    assert t.root() == ot.root and t.inv_root() == ot.inv_root and t.inv_n() == ot.inv_n
+    rp, rpq = t.root_powers()
+    orp, orpq = ot.powers(0)
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(rpq, orpq)
+    rng = np.random.default_rng(log_n)
+    # lazy forward from [0, 4q): bit-identical lazy values
+    a = (rng.integers(0, q, size=n, dtype=np.uint64)
+         + rng.integers(0, 4, size=n, dtype=np.uint64) * np.uint64(q)).astype(np.uint64)
+    x = a.copy()
+    t.lazy_transform_slice(x)
+    np.testing.assert_array_equal(x, ot.lazy_transform(a))
+    x = a.copy()
+    t.transform_slice(x)
+    np.testing.assert_array_equal(x, ot.transform(a))
+    # inverse from [0, 2q)
+    b = (rng.integers(0, q, size=n, dtype=np.uint64)
+         + rng.integers(0, 2, size=n, dtype=np.uint64) * np.uint64(q)).astype(np.uint64)
+    y = b.copy()
+    t.lazy_inverse_transform_slice(y)
+    np.testing.assert_array_equal(y, ot.lazy_inverse_transform(b))
+    y = b.copy()
+    t.inverse_transform_slice(y)
+    np.testing.assert_array_equal(y, ot.inverse_transform(b))
+
+
+@pytest.mark.parametrize("q", [17, 12289, 2147473409, 4294957057, 1152921504606830593,
+                               4611686018427322369])
+def test_moduli_edge_cases(q):
+    log_n = min(10, ((q - 1) & -(q - 1)).bit_length() - 2)
+    n = 1 << log_n
+    t = pa.UintNttTable.new(log_n, q)
+    ot = O.Table(log_n, q)
+    for a in [np.zeros(n, dtype=np.uint64), np.full(n, q - 1, dtype=np.uint64),
+              np.full(n, 4 * q - 1, dtype=np.uint64), _rand(q, n, 7)]:
+        x = a.copy()
+        t.transform_slice(x)
+        np.testing.assert_array_equal(x, ot.transform(a))
+        x = a.copy()
+        t.lazy_transform_slice(x)
+        np.testing.assert_array_equal(x, ot.lazy_transform(a))
+    b = np.full(n, 2 * q - 1, dtype=np.uint64)
+    y = b.copy()
+    t.inverse_transform_slice(y)
+    np.testing.assert_array_equal(y, ot.inverse_transform(b))
+
+
+def test_slice_length_checked():
+    t = pa.UintNttTable.new(10, 2147473409)
+    with pytest.raises(ValueError):
+        t.transform_slice(np.zeros(512, dtype=np.uint64))
+
+
+# ---------------------------------------------------------------- DCRT and batched forms
+
+def test_dcrt_slices():
+    log_n = 12
+    tab = pa.UintCrtNttTable.new(log_n, CFG2_PRIMES)
+    assert tab.moduli_count() == 4 and tab.crt_poly_length() == 4 << log_n
+    x = np.concatenate([_rand(q, 1 << log_n, i) for i, q in enumerate(CFG2_PRIMES)])
+    want = np.concatenate([O.Table(log_n, q).transform(x[i << log_n:(i + 1) << log_n])
+                           for i, q in enumerate(CFG2_PRIMES)])
+    y = x.copy()
+    tab.transform_slice(y)
+    np.testing.assert_array_equal(y, want)
+    tab.inverse_transform_slice(y)
+    np.testing.assert_array_equal(y, x)
+
+
+def _fmi_reference(tables, polys, op):
+    out = polys.copy()
+    L = len(tables)
+    for i in range(polys.shape[0]):
+        t = tables[i % L]
+        out[i] = t.inverse_transform(O.pointwise_mul(t.transform(polys[i]), op[i % L], t.q))
+    return out
+
+
+@pytest.mark.parametrize("log_n,L,count", [(12, 4, 1024), (10, 3, 30), (13, 2, 8), (14, 2, 4),
+                                           (16, 1, 2), (17, 2, 2), (5, 2, 6)])
+def test_fwd_mul_inv_batch(log_n, L, count):
+    n = 1 << log_n
+    moduli = pa.ntt_primes_below(50, 2 * n, L)
+    print("fwd_mul_inv", log_n, L, count, flush=True)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    rng = np.random.default_rng(log_n * 100 + L)
+    polys = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    op_vals = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in moduli])
+    op = pa.Operand(tab, op_vals)
+    d = pa.DeviceBuffer.from_array(polys)
+    pa.fwd_mul_inv_device(tab, d.ptr, count, op)
+    pa.synchronize()
+    got = d.download((count, n))
+    want = _fmi_reference([O.Table(log_n, q) for q in moduli], polys, op_vals)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("log_n,L,count", [(10, 1, 64), (12, 3, 6), (13, 1, 3), (14, 1, 2), (15, 2, 2),
+                                           (16, 1, 1), (3, 1, 5)])
+def test_negacyclic_batch(log_n, L, count):
+    n = 1 << log_n
+    moduli = pa.ntt_primes_below(59, 2 * n, L)
+    print("negacyclic", log_n, L, count, flush=True)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    rng = np.random.default_rng(log_n)
+    a = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    b = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    da, db = pa.DeviceBuffer.from_array(a), pa.DeviceBuffer.from_array(b)
+    pa.negacyclic_mul_device(tab, da.ptr, db.ptr, count)
+    pa.synchronize()
+    got = da.download((count, n))
+    tables = [O.Table(log_n, q) for q in moduli]
+    for i in range(count):
+        t = tables[i % L]
+        want = t.inverse_transform(O.pointwise_mul(t.transform(a[i]), t.transform(b[i]), t.q))
+        np.testing.assert_array_equal(got[i], want)
+        if n <= 256:
+            np.testing.assert_array_equal(got[i], O.schoolbook(a[i], b[i], t.q))
+    np.testing.assert_array_equal(db.download((count, n)), b)
+
+
+def test_device_forward_inverse_batch():
+    log_n, L, count = 11, 3, 12
+    n = 1 << log_n
+    moduli = pa.ntt_primes_below(62, 2 * n, L)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    tables = [O.Table(log_n, q) for q in moduli]
+    rng = np.random.default_rng(3)
+    x = np.stack([rng.integers(0, 4 * moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    d = pa.DeviceBuffer.from_array(x)
+    pa.transform_device(tab, d.ptr, count, lazy=True)
+    pa.synchronize()
+    lz = d.download((count, n))
+    for i in range(count):
+        np.testing.assert_array_equal(lz[i], tables[i % L].lazy_transform(x[i]))
+    d.upload(x)
+    pa.transform_device(tab, d.ptr, count, lazy=False)
+    pa.inverse_transform_device(tab, d.ptr, count)
+    pa.synchronize()
+    back = d.download((count, n))
+    for i in range(count):
+        np.testing.assert_array_equal(back[i], x[i] % np.uint64(moduli[i % L]))
+
+
+def test_pointwise_device():
+    log_n, L, count = 10, 2, 6
+    n = 1 << log_n
+    moduli = pa.ntt_primes_below(62, 2 * n, L)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    rng = np.random.default_rng(9)
+    a = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    b = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    c = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    da, db, dc = (pa.DeviceBuffer.from_array(v) for v in (a, b, c))
+    dout = pa.DeviceBuffer(count * n)
+    pa.pointwise_mul_device(tab, da.ptr, db.ptr, dout.ptr, count)
+    pa.pointwise_mul_add_device(tab, dc.ptr, da.ptr, db.ptr, count)
+    pa.synchronize()
+    out, acc = dout.download((count, n)), dc.download((count, n))
+    for i in range(count):
+        q = moduli[i % L]
+        np.testing.assert_array_equal(out[i], O.pointwise_mul(a[i], b[i], q))
+        np.testing.assert_array_equal(acc[i], O.pointwise_mul_add(c[i], a[i], b[i], q))
