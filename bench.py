@@ -119,47 +119,58 @@ def main():
     sp = ctypes.c_void_p(stream.cuda_stream)
     ptrs = [ctypes.c_void_p(bufs[r].data_ptr()) for r in range(ROTATE)]
 
-    def step(i):
-        pn._check(pa.lib().ph_dcrt_fwd_mul_inv_device(table.handle, ptrs[i % ROTATE], COUNT, op.handle, sp))
+    def launcher(tab, operand):
+        def step(i):
+            pn._check(pa.lib().ph_dcrt_fwd_mul_inv_device(tab.handle, ptrs[i % ROTATE], COUNT, operand.handle, sp))
+        return step
+
+    step = launcher(table, op)
 
     # correctness gate on the first limb-polynomials before any timing
-    step(0)
-    stream.synchronize()
     from oracle import oracle as O
 
     otabs = [O.Table(LOG_N, q) for q in moduli]
-    got = bufs[0, :8].cpu().numpy().view(np.uint64)
-    for i in range(8):
-        t = otabs[i % LIMBS]
-        want = t.inverse_transform(O.pointwise_mul(t.transform(polys[i]), operand[i % LIMBS], t.q))
-        if not np.array_equal(got[i], want):
-            print("bench: GPU output differs from the oracle; refusing to time", file=sys.stderr)
-            return 1
-    bufs[0].copy_(host)
-    torch.cuda.synchronize()
+    int_table = pa.UintCrtNttTable.new(LOG_N, moduli, integer_only=True)
+    int_op = pa.Operand(int_table, operand)
+    int_step = launcher(int_table, int_op)
+    for st in (step, int_step):
+        st(0)
+        stream.synchronize()
+        got = bufs[0, :8].cpu().numpy().view(np.uint64)
+        for i in range(8):
+            t = otabs[i % LIMBS]
+            want = t.inverse_transform(O.pointwise_mul(t.transform(polys[i]), operand[i % LIMBS], t.q))
+            if not np.array_equal(got[i], want):
+                print("bench: GPU output differs from the oracle; refusing to time", file=sys.stderr)
+                return 1
+        bufs[0].copy_(host)
+        torch.cuda.synchronize()
 
-    with torch.cuda.stream(stream):
-        for i in range(args.warmup):
-            step(i)
-    stream.synchronize()
+    def timed(st, K, W):
+        for i in range(W):
+            st(i)
+        stream.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        evs[0].record(stream)
+        for i in range(K):
+            st(i)
+            evs[i + 1].record(stream)
+        stream.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(K)])  # ms per launch
+        return per, evs[0].elapsed_time(evs[K]), wall
 
     K = args.steps
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evs[0].record(stream)
-    for i in range(K):
-        step(i)
-        evs[i + 1].record(stream)
-    stream.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(K)])  # ms per launch
-    total_ms = evs[0].elapsed_time(evs[K])
+    per, total_ms, wall = timed(step, K, args.warmup)
     ms_per_step = total_ms / K
     value = NTT_PER_STEP / (ms_per_step * 1e-3)
     kernel_ms = float(np.mean(per))
     achieved = BYTES_PER_STEP / (kernel_ms * 1e-3) / 1e9
+    int_per, int_total, _ = timed(int_step, max(K // 4, 10), args.warmup)
+    int_ms = int_total / max(K // 4, 10)
 
     # plain path: one launch per limb-polynomial, per-limb single-modulus tables
     limb_tabs = [pa.UintCrtNttTable.new(LOG_N, [q]) for q in moduli]
@@ -216,6 +227,13 @@ def main():
             "p99_ms": float(np.percentile(plain_per, 99)),
         },
         "batched_vs_plain": plain_ms / ms_per_step,
+        "arith": table.arith(),
+        "int_path": {
+            "desc": "same fused kernel on 64-bit integer Shoup arithmetic (32-bit multiply chains)",
+            "ms_per_batch": int_ms,
+            "ntt_per_s": NTT_PER_STEP / (int_ms * 1e-3),
+            "hbm_frac": BYTES_PER_STEP / (int_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+        },
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -223,7 +241,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "chunk_kernel<12,8,FWD|MUL_SHOUP|INV> (fused fwd -> mul -> inv)",
+            "kernel": "chunk_kernel<F64Arith,12,8,FWD|MUL_SHOUP|INV|INV_LAST> (fused fwd -> mul -> inv)",
             "kernel_ms": kernel_ms,
             "bytes_per_launch": BYTES_PER_STEP,
         },

@@ -50,6 +50,14 @@ extern "C" {
 #define PH_ERR_UNSUPPORTED 7       /* ring degree outside 2^1 .. 2^17 */
 #define PH_ERR_NO_MEMORY 8         /* host or device allocation failed */
 
+/* Table creation flags.  By default canonical-output transforms over moduli
+ * q < 2^50 run on exact binary64 FMA arithmetic (full-rate on gfx950) and all
+ * others on 64-bit integer Shoup arithmetic built from 32-bit multiplies;
+ * both give identical canonical results.  Lazy-output entry points always use
+ * the integer butterflies, whose lazy values match the reference's.
+ * PH_TABLE_INTEGER_ONLY forces the integer arithmetic everywhere. */
+#define PH_TABLE_INTEGER_ONLY 1u
+
 typedef struct ph_ntt_table ph_ntt_table;   /* UintNttTable<u64> */
 typedef struct ph_dcrt_table ph_dcrt_table; /* UintCrtNttTable<u64> */
 typedef struct ph_operand ph_operand;       /* device-resident Shoup-prepared NTT-domain operand */
@@ -65,7 +73,13 @@ const char *ph_last_error(void);
  * psi = minimal primitive 2N-th root (primus_ntt/src/root.rs:89-111), Shoup
  * twiddle tables, N^-1.  `modulus` plays BarrettModulus::new (q in (1, 2^62)). */
 int ph_ntt_table_new(uint32_t log_n, uint64_t modulus, ph_ntt_table **out);
+/* ph_ntt_table_new with PH_TABLE_* flags (no reference counterpart: the
+ * reference has one CPU arithmetic). */
+int ph_ntt_table_new_ex(uint32_t log_n, uint64_t modulus, uint32_t flags, ph_ntt_table **out);
 void ph_ntt_table_free(ph_ntt_table *table);
+/* 1 if canonical transforms of this table run on binary64 FMA arithmetic, 0 if
+ * on integer arithmetic, -1 for a null table. */
+int ph_ntt_table_arith(const ph_ntt_table *table);
 
 /* NttTable::poly_length (primitive.rs:188-190) */
 size_t ph_ntt_table_poly_length(const ph_ntt_table *table);
@@ -102,7 +116,11 @@ int ph_transform_coeff_minus_one_monomial(const ph_ntt_table *table, size_t degr
 
 /* DcrtTable::new -> UintCrtNttTable::new (primus_ntt/src/dcrt/primitive.rs:24-43) */
 int ph_dcrt_table_new(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_count, ph_dcrt_table **out);
+int ph_dcrt_table_new_ex(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_count, uint32_t flags,
+                         ph_dcrt_table **out);
 void ph_dcrt_table_free(ph_dcrt_table *table);
+/* As ph_ntt_table_arith, for all limbs together. */
+int ph_dcrt_table_arith(const ph_dcrt_table *table);
 /* DcrtTable::{poly_length, moduli_count, crt_poly_length} (dcrt/primitive.rs:57-70) */
 size_t ph_dcrt_poly_length(const ph_dcrt_table *table);
 size_t ph_dcrt_moduli_count(const ph_dcrt_table *table);

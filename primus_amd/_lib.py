@@ -21,6 +21,9 @@ HIPCC_FLAGS = [
     "-shared",
     # keep packed-FP32 instructions out of all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the binary64 modular arithmetic relies on every product and FMA being
+    # exactly the one written: no contraction of separate multiplies and adds
+    "-ffp-contract=off",
 ]
 
 
@@ -82,6 +85,10 @@ def _declare(L):
         "ph_status_string": (ctypes.c_char_p, [i]),
         "ph_last_error": (ctypes.c_char_p, []),
         "ph_ntt_table_new": (i, [u32, u64, pvp]),
+        "ph_ntt_table_new_ex": (i, [u32, u64, u32, pvp]),
+        "ph_ntt_table_arith": (i, [vp]),
+        "ph_dcrt_table_new_ex": (i, [u32, p64, u32, u32, pvp]),
+        "ph_dcrt_table_arith": (i, [vp]),
         "ph_ntt_table_free": (None, [vp]),
         "ph_ntt_table_poly_length": (sz, [vp]),
         "ph_ntt_table_log_n": (u32, [vp]),

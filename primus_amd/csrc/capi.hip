@@ -24,12 +24,15 @@ struct ph_ntt_table {
     uint64_t n, q, root, inv_root, inv_n, inv_n_q;
     std::vector<Shoup> fwd, inv, ord;  // host copies (accessors)
     Shoup *d_fwd = nullptr, *d_inv = nullptr, *d_ord = nullptr;
+    ShoupF *d_fwd_f = nullptr, *d_inv_f = nullptr;  // binary64 twiddles (q < 2^50)
+    bool f64 = false;                                // canonical ops may use F64Arith
     DevTable dev{};                    // device view (pointers into d_*)
     DevTable *d_tab = nullptr;         // the view itself in device memory (L = 1)
 };
 
 struct ph_dcrt_table {
     uint32_t log_n, L;
+    bool f64 = false;
     std::vector<ph_ntt_table *> tables;
     DevTable *d_tabs = nullptr;
 };
@@ -38,6 +41,7 @@ struct ph_operand {
     uint32_t L;
     uint64_t n;
     Shoup *d = nullptr;
+    ShoupF *d_f = nullptr;
 };
 
 namespace {
@@ -120,6 +124,8 @@ void free_table(ph_ntt_table *t) {
     if (t->d_fwd) (void)hipFree(t->d_fwd);
     if (t->d_inv) (void)hipFree(t->d_inv);
     if (t->d_ord) (void)hipFree(t->d_ord);
+    if (t->d_fwd_f) (void)hipFree(t->d_fwd_f);
+    if (t->d_inv_f) (void)hipFree(t->d_inv_f);
     if (t->d_tab) (void)hipFree(t->d_tab);
     delete t;
 }
@@ -130,8 +136,11 @@ int upload(void **dst, const void *src, size_t bytes) {
     return PH_OK;
 }
 
+// binary64 twiddle {w, w/q}: w/q from the Shoup quotient, |wp - w/q| <= 2^-54 + 2^-64.
+ShoupF mkf(const Shoup &s) { return ShoupF{(double)s.w, (double)s.wq * 0x1p-64}; }
+
 // UintNttTable::new, primus_ntt/src/ntt/primitive.rs:114-185.
-int build_table(uint32_t log_n, uint64_t q, ph_ntt_table **out) {
+int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) {
     if (q <= 1 || (64 - __builtin_clzll(q)) >= 63)
         return fail(PH_ERR_INVALID, "modulus must satisfy 1 < q < 2^62 (BarrettModulus::new)");
     if (log_n < kMinLogN || log_n > kMaxLogN)
@@ -191,6 +200,30 @@ int build_table(uint32_t log_n, uint64_t q, ph_ntt_table **out) {
     d.inv = t->d_inv;
     d.ord = t->d_ord;
     d.log_n = log_n;
+    t->f64 = q < kF64MaxModulus && !(flags & PH_TABLE_INTEGER_ONLY);
+    if (t->f64) {
+        std::vector<ShoupF> ff(n), fi(n);
+        for (uint64_t k = 0; k < n; ++k) {
+            ff[k] = mkf(t->fwd[k]);
+            fi[k] = mkf(t->inv[k]);
+        }
+        rc = upload((void **)&t->d_fwd_f, ff.data(), n * sizeof(ShoupF));
+        if (rc == PH_OK) rc = upload((void **)&t->d_inv_f, fi.data(), n * sizeof(ShoupF));
+        if (rc != PH_OK) {
+            free_table(t);
+            return rc;
+        }
+        d.qd = (double)q;
+        d.qinv = 1.0 / (double)q;
+        ShoupF in = mkf(Shoup{t->inv_n, t->inv_n_q});
+        ShoupF sr = mkf(Shoup{d.sr, d.sr_q});
+        d.inv_n_f = in.w;
+        d.inv_n_p = in.wp;
+        d.sr_f = sr.w;
+        d.sr_p = sr.wp;
+        d.fwd_f = t->d_fwd_f;
+        d.inv_f = t->d_inv_f;
+    }
     rc = upload((void **)&t->d_tab, &d, sizeof(DevTable));
     if (rc != PH_OK) {
         free_table(t);
@@ -224,32 +257,34 @@ int staging(size_t bytes, void **buf, hipStream_t *stream) {
     return PH_OK;
 }
 
-int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, uint64_t *d_data,
-           const uint64_t *d_data2, const Shoup *operand, uint64_t count, hipStream_t s) {
+int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, uint64_t *d_data,
+           const uint64_t *d_data2, const ph_operand *operand, uint64_t count, hipStream_t s) {
     if (count == 0) return PH_OK;
     KArgs a{};
     a.data = d_data;
     a.data2 = d_data2;
-    a.operand = operand;
+    a.operand = operand ? operand->d : nullptr;
+    a.operand_f = operand ? operand->d_f : nullptr;
+    if (op == Op::FwdMulInv && f64 && !a.operand_f) f64 = false;
     a.tabs = d_tabs;
     a.L = L;
     void *scratch = nullptr;
     if (needs_scratch(op, log_n)) PH_HIP(hipMallocAsync(&scratch, (count << log_n) * 8, s));
-    hipError_t e = launch_transform(op, a, log_n, count, scratch, s);
+    hipError_t e = launch_transform(op, a, log_n, count, scratch, f64, s);
     if (scratch) (void)hipFreeAsync(scratch, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return PH_OK;
 }
 
 // Host slice -> device -> op -> host slice, synchronous.
-int host_roundtrip(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, uint64_t *poly,
+int host_roundtrip(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, uint64_t *poly,
                    size_t len) {
     void *buf;
     hipStream_t s;
     int rc = staging(len * 8, &buf, &s);
     if (rc != PH_OK) return rc;
     PH_HIP(hipMemcpyAsync(buf, poly, len * 8, hipMemcpyHostToDevice, s));
-    rc = run_op(op, d_tabs, L, log_n, (uint64_t *)buf, nullptr, nullptr, len >> log_n, s);
+    rc = run_op(op, d_tabs, L, log_n, f64, (uint64_t *)buf, nullptr, nullptr, len >> log_n, s);
     if (rc != PH_OK) return rc;
     PH_HIP(hipMemcpyAsync(poly, buf, len * 8, hipMemcpyDeviceToHost, s));
     PH_HIP(hipStreamSynchronize(s));
@@ -299,10 +334,16 @@ const char *ph_status_string(int status) {
 const char *ph_last_error(void) { return g_last_error.c_str(); }
 
 int ph_ntt_table_new(uint32_t log_n, uint64_t modulus, ph_ntt_table **out) {
+    return ph_ntt_table_new_ex(log_n, modulus, 0, out);
+}
+
+int ph_ntt_table_new_ex(uint32_t log_n, uint64_t modulus, uint32_t flags, ph_ntt_table **out) {
     if (!out) return fail(PH_ERR_INVALID, "null output");
     *out = nullptr;
-    return build_table(log_n, modulus, out);
+    return build_table(log_n, modulus, flags, out);
 }
+
+int ph_ntt_table_arith(const ph_ntt_table *t) { return t ? (t->f64 ? 1 : 0) : -1; }
 
 void ph_ntt_table_free(ph_ntt_table *table) { free_table(table); }
 
@@ -331,22 +372,22 @@ int ph_ntt_table_powers(const ph_ntt_table *t, int which, uint64_t *values, uint
 
 int ph_lazy_transform_slice(const ph_ntt_table *t, uint64_t *poly, size_t len) {
     int rc = check_slice(t, poly, len, t ? t->n : 0);
-    return rc != PH_OK ? rc : host_roundtrip(Op::FwdLazy, t->d_tab, 1, t->log_n, poly, len);
+    return rc != PH_OK ? rc : host_roundtrip(Op::FwdLazy, t->d_tab, 1, t->log_n, t->f64, poly, len);
 }
 
 int ph_transform_slice(const ph_ntt_table *t, uint64_t *poly, size_t len) {
     int rc = check_slice(t, poly, len, t ? t->n : 0);
-    return rc != PH_OK ? rc : host_roundtrip(Op::FwdCanon, t->d_tab, 1, t->log_n, poly, len);
+    return rc != PH_OK ? rc : host_roundtrip(Op::FwdCanon, t->d_tab, 1, t->log_n, t->f64, poly, len);
 }
 
 int ph_lazy_inverse_transform_slice(const ph_ntt_table *t, uint64_t *values, size_t len) {
     int rc = check_slice(t, values, len, t ? t->n : 0);
-    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, values, len);
+    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, t->f64, values, len);
 }
 
 int ph_inverse_transform_slice(const ph_ntt_table *t, uint64_t *values, size_t len) {
     int rc = check_slice(t, values, len, t ? t->n : 0);
-    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, values, len);
+    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, t->f64, values, len);
 }
 
 int ph_transform_monomial(const ph_ntt_table *t, uint64_t coeff, size_t degree, uint64_t *values,
@@ -364,6 +405,11 @@ int ph_transform_coeff_minus_one_monomial(const ph_ntt_table *t, size_t degree, 
 }
 
 int ph_dcrt_table_new(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_count, ph_dcrt_table **out) {
+    return ph_dcrt_table_new_ex(log_n, moduli, moduli_count, 0, out);
+}
+
+int ph_dcrt_table_new_ex(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_count, uint32_t flags,
+                         ph_dcrt_table **out) {
     if (!out || !moduli || moduli_count == 0) return fail(PH_ERR_INVALID, "null moduli or output");
     *out = nullptr;
     ph_dcrt_table *d = new (std::nothrow) ph_dcrt_table();
@@ -371,15 +417,17 @@ int ph_dcrt_table_new(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_co
     d->log_n = log_n;
     d->L = moduli_count;
     std::vector<DevTable> views;
+    d->f64 = true;
     for (uint32_t i = 0; i < moduli_count; ++i) {
         ph_ntt_table *t = nullptr;
-        int rc = build_table(log_n, moduli[i], &t);
+        int rc = build_table(log_n, moduli[i], flags, &t);
         if (rc != PH_OK) {
             ph_dcrt_table_free(d);
             return rc;
         }
         d->tables.push_back(t);
         views.push_back(t->dev);
+        d->f64 = d->f64 && t->f64;
     }
     int rc = upload((void **)&d->d_tabs, views.data(), views.size() * sizeof(DevTable));
     if (rc != PH_OK) {
@@ -397,6 +445,8 @@ void ph_dcrt_table_free(ph_dcrt_table *d) {
     delete d;
 }
 
+int ph_dcrt_table_arith(const ph_dcrt_table *d) { return d ? (d->f64 ? 1 : 0) : -1; }
+
 size_t ph_dcrt_poly_length(const ph_dcrt_table *d) { return d ? (size_t)1 << d->log_n : 0; }
 size_t ph_dcrt_moduli_count(const ph_dcrt_table *d) { return d ? d->L : 0; }
 size_t ph_dcrt_crt_poly_length(const ph_dcrt_table *d) { return d ? ((size_t)d->L << d->log_n) : 0; }
@@ -407,7 +457,7 @@ const ph_ntt_table *ph_dcrt_ntt_table(const ph_dcrt_table *d, uint32_t i) {
 
 static int dcrt_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len, Op op) {
     int rc = check_slice(d, poly, len, d ? ((size_t)d->L << d->log_n) : 0);
-    return rc != PH_OK ? rc : host_roundtrip(op, d->d_tabs, d->L, d->log_n, poly, len);
+    return rc != PH_OK ? rc : host_roundtrip(op, d->d_tabs, d->L, d->log_n, d->f64, poly, len);
 }
 
 int ph_dcrt_lazy_transform_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len) {
@@ -425,24 +475,24 @@ int ph_dcrt_inverse_transform_slice(const ph_dcrt_table *d, uint64_t *poly, size
 
 int ph_dcrt_transform_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t count, int lazy, void *stream) {
     if (!d || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
-    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, d->d_tabs, d->L, d->log_n, d_polys, nullptr, nullptr,
+    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, d->d_tabs, d->L, d->log_n, d->f64, d_polys, nullptr, nullptr,
                   count, (hipStream_t)stream);
 }
 
 int ph_dcrt_inverse_transform_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t count, void *stream) {
     if (!d || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
-    return run_op(Op::Inv, d->d_tabs, d->L, d->log_n, d_polys, nullptr, nullptr, count, (hipStream_t)stream);
+    return run_op(Op::Inv, d->d_tabs, d->L, d->log_n, d->f64, d_polys, nullptr, nullptr, count, (hipStream_t)stream);
 }
 
 int ph_ntt_transform_device(const ph_ntt_table *t, uint64_t *d_polys, size_t count, int lazy, void *stream) {
     if (!t || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
-    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, t->d_tab, 1, t->log_n, d_polys, nullptr, nullptr, count,
+    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, t->d_tab, 1, t->log_n, t->f64, d_polys, nullptr, nullptr, count,
                   (hipStream_t)stream);
 }
 
 int ph_ntt_inverse_transform_device(const ph_ntt_table *t, uint64_t *d_polys, size_t count, void *stream) {
     if (!t || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
-    return run_op(Op::Inv, t->d_tab, 1, t->log_n, d_polys, nullptr, nullptr, count, (hipStream_t)stream);
+    return run_op(Op::Inv, t->d_tab, 1, t->log_n, t->f64, d_polys, nullptr, nullptr, count, (hipStream_t)stream);
 }
 
 int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_operand **out) {
@@ -450,12 +500,14 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
     *out = nullptr;
     const uint64_t n = 1ull << d->log_n;
     std::vector<Shoup> h((size_t)d->L * n);
+    std::vector<ShoupF> hf(d->f64 ? (size_t)d->L * n : 0);
     for (uint32_t l = 0; l < d->L; ++l) {
         const uint64_t q = d->tables[l]->q;
         for (uint64_t i = 0; i < n; ++i) {
             uint64_t v = host_values[l * n + i];
             if (v >= q) return fail(PH_ERR_INVALID, "operand values must be canonical (< q)");
             h[l * n + i] = mk(v, q);
+            if (d->f64) hf[l * n + i] = mkf(h[l * n + i]);
         }
     }
     ph_operand *op = new (std::nothrow) ph_operand();
@@ -463,8 +515,9 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
     op->L = d->L;
     op->n = n;
     int rc = upload((void **)&op->d, h.data(), h.size() * sizeof(Shoup));
+    if (rc == PH_OK && d->f64) rc = upload((void **)&op->d_f, hf.data(), hf.size() * sizeof(ShoupF));
     if (rc != PH_OK) {
-        delete op;
+        ph_operand_free(op);
         return rc;
     }
     *out = op;
@@ -474,6 +527,7 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
 void ph_operand_free(ph_operand *op) {
     if (!op) return;
     if (op->d) (void)hipFree(op->d);
+    if (op->d_f) (void)hipFree(op->d_f);
     delete op;
 }
 
@@ -481,13 +535,13 @@ int ph_dcrt_fwd_mul_inv_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t
                                void *stream) {
     if (!d || !op || (!d_polys && count)) return fail(PH_ERR_INVALID, "null argument");
     if (op->L != d->L || op->n != (1ull << d->log_n)) return fail(PH_ERR_INVALID, "operand shape mismatch");
-    return run_op(Op::FwdMulInv, d->d_tabs, d->L, d->log_n, d_polys, nullptr, op->d, count, (hipStream_t)stream);
+    return run_op(Op::FwdMulInv, d->d_tabs, d->L, d->log_n, d->f64, d_polys, nullptr, op, count, (hipStream_t)stream);
 }
 
 int ph_dcrt_negacyclic_mul_device(const ph_dcrt_table *d, uint64_t *d_a, const uint64_t *d_b, size_t count,
                                   void *stream) {
     if (!d || ((!d_a || !d_b) && count)) return fail(PH_ERR_INVALID, "null argument");
-    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d_a, d_b, nullptr, count, (hipStream_t)stream);
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, (hipStream_t)stream);
 }
 
 int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *d, const uint64_t *d_a, const uint64_t *d_b,

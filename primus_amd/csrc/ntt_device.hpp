@@ -3,9 +3,7 @@
 // The transform is the reference's Harvey-lazy Cooley-Tukey forward
 // (normal -> bit-reversed, primus_ntt/src/ntt/primitive.rs:211-232) and
 // Gentleman-Sande inverse (bit-reversed -> normal, :246-286), butterfly for
-// butterfly: every butterfly pairs the same two slots with the same twiddle
-// and computes the same lazy value, so lazy outputs are bit-identical to the
-// CPU table and canonical outputs follow.
+// butterfly: every butterfly pairs the same two slots with the same twiddle.
 //
 // Only the schedule differs.  A thread holds a group of 2^R slots in VGPRs and
 // runs R consecutive stages on them (radix-2^R); groups are re-formed between
@@ -15,11 +13,28 @@
 //   inverse round, global stages [P, P+R):  k = (c << (P+R)) + (j << P) + o
 //
 // with j < 2^R the register index, c the chunk the group lives in at the
-// round's first stage and o the offset inside it.  The twiddle of a
-// butterfly is the one the CPU loop would fetch for that chunk:
+// round's first (forward) or last (inverse) stage and o the offset inside it.
+// The twiddle of a butterfly is the one the CPU loop would fetch:
 //   forward stage s, chunk c_s:  root_powers[2^s + c_s]
 //   inverse stage p, chunk c_p:  inv_root_powers[1 + N - (N >> p) + c_p]
 //   (the inverse's last stage, gap N/2, folds N^-1 as in primitive.rs:270-285).
+//
+// Two arithmetic policies run the same schedule:
+//
+// * IntArith: uint64 lanes, the reference's Shoup butterflies exactly
+//   (lazy [0,4q) forward, [0,2q) inverse); 32-bit multiply chains; any
+//   q < 2^62.  Lazy outputs are bit-identical to the CPU table.
+// * F64Arith: for q < 2^50, coefficients are exact integers held in binary64
+//   lanes in a signed range, and a*w mod q is computed with full-rate FMAs:
+//       h = x*w;  l = fma(x, w, -h)            (x*w = h + l exactly)
+//       k = fma(x, w/q, C) - C                 (C = 1.5*2^52: k = round(x*(w/q)))
+//       r = fma(-k, q, h) + l                  (both steps exact)
+//   For |x| <= 2^51 and |w/q - wp| <= 2^-54 + 2^-64, |x*w/q - k| <= 0.6251,
+//   so r = x*w - k*q with |r| <= 0.6251 q; every intermediate is an integer of
+//   magnitude < 2^53 and is therefore exact.  Values entering a product are
+//   kept below 2^51 by centred reductions (a - round(a/q)*q, |.| <= q/2 + 1).
+//   The arithmetic is exact, so canonical outputs equal the CPU table's;
+//   intermediate (lazy) values differ, so lazy entry points use IntArith.
 #pragma once
 
 #include "arith.hpp"
@@ -31,6 +46,11 @@ struct alignas(16) Shoup {
     uint64_t wq;  // floor(w * 2^64 / q)
 };
 
+struct alignas(16) ShoupF {
+    double w;   // value (exact integer)
+    double wp;  // w / q rounded to binary64
+};
+
 // Per-modulus device view of one UintNttTable.
 struct DevTable {
     uint64_t q, q2;
@@ -40,76 +60,175 @@ struct DevTable {
     const Shoup *fwd;         // root_powers, N entries (bit-reversed order)
     const Shoup *inv;         // inv_root_powers, N entries (scrambled order)
     const Shoup *ord;         // ordinal_root_powers, 2N entries
+    // binary64 view, valid when q < 2^50 (fwd_f != nullptr)
+    double qd, qinv;
+    double inv_n_f, inv_n_p, sr_f, sr_p;
+    const ShoupF *fwd_f;
+    const ShoupF *inv_f;
     uint32_t log_n;
     uint32_t pad_;
 };
 
-// Harvey forward butterfly, primitive.rs:225-228.
-__device__ __forceinline__ void ct_bf(uint64_t &a, uint64_t &b, uint64_t w, uint64_t wq, uint64_t q,
-                                      uint64_t q2) {
-    uint64_t u = sub_if_ge(a, q2);
-    uint64_t v = shoup_lazy(w, wq, b, q);
-    a = u + v;
-    b = u + q2 - v;
-}
+// ---------------------------------------------------------------------------
+// IntArith
+// ---------------------------------------------------------------------------
 
-// Gentleman-Sande inverse butterfly, primitive.rs:262-265.
-__device__ __forceinline__ void gs_bf(uint64_t &a, uint64_t &b, uint64_t w, uint64_t wq, uint64_t q,
-                                      uint64_t q2) {
-    uint64_t u = a, v = b;
-    a = sub_if_ge(u + v, q2);
-    b = shoup_lazy(w, wq, u + q2 - v, q);
-}
+struct IntArith {
+    using T = uint64_t;
+    using Tw = Shoup;
 
-// Last inverse stage with N^-1 folded in, primitive.rs:270-285.
-__device__ __forceinline__ void gs_last_bf(uint64_t &a, uint64_t &b, const DevTable &t) {
-    uint64_t u = a, v = b;
-    a = shoup_full(t.inv_n, t.inv_n_q, u + v, t.q);
-    b = shoup_full(t.sr, t.sr_q, u + t.q2 - v, t.q);
-}
+    __device__ __forceinline__ static const Tw *fwd(const DevTable &t) { return t.fwd; }
+    __device__ __forceinline__ static const Tw *inv(const DevTable &t) { return t.inv; }
+
+    // Harvey forward butterfly, primitive.rs:225-228.
+    __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
+        uint64_t u = sub_if_ge(a, t.q2);
+        uint64_t v = shoup_lazy(w.w, w.wq, b, t.q);
+        a = u + v;
+        b = u + t.q2 - v;
+    }
+    // Gentleman-Sande inverse butterfly, primitive.rs:262-265.
+    __device__ __forceinline__ static void gs(T &a, T &b, const Tw &w, const DevTable &t) {
+        uint64_t u = a, v = b;
+        a = sub_if_ge(u + v, t.q2);
+        b = shoup_lazy(w.w, w.wq, u + t.q2 - v, t.q);
+    }
+    // Last inverse stage with N^-1 folded in, primitive.rs:270-285.
+    __device__ __forceinline__ static void gs_last(T &a, T &b, const DevTable &t) {
+        uint64_t u = a, v = b;
+        a = shoup_full(t.inv_n, t.inv_n_q, u + v, t.q);
+        b = shoup_full(t.sr, t.sr_q, u + t.q2 - v, t.q);
+    }
+    // u64 in [0, 4q) -> lane value
+    __device__ __forceinline__ static T load(uint64_t v, const DevTable &) { return v; }
+    // lane value (forward output, [0,4q)) -> canonical u64
+    __device__ __forceinline__ static uint64_t canon_fwd(T v, const DevTable &t) {
+        return sub_if_ge(sub_if_ge(v, t.q2), t.q);
+    }
+    // lane value (inverse output, [0,q)) -> u64
+    __device__ __forceinline__ static uint64_t out_inv(T v, const DevTable &) { return v; }
+    // forward output * resident operand (canonical)
+    __device__ __forceinline__ static T mul_operand(T x, const Shoup &w, const DevTable &t) {
+        return shoup_full(w.w, w.wq, x, t.q);
+    }
+    // forward output * forward output, canonical (NttPolynomial::mul_assign, Barrett)
+    __device__ __forceinline__ static T mul_pair(T x, T y, const DevTable &t) {
+        return barrett_mul(canon_fwd(x, t), canon_fwd(y, t), t.q, t.br0, t.br1);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// F64Arith (q < 2^50)
+// ---------------------------------------------------------------------------
+
+struct F64Arith {
+    using T = double;
+    using Tw = ShoupF;
+    static constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
+
+    __device__ __forceinline__ static const Tw *fwd(const DevTable &t) { return t.fwd_f; }
+    __device__ __forceinline__ static const Tw *inv(const DevTable &t) { return t.inv_f; }
+
+    // x*w mod q in the centred range |r| <= 0.6251 q, for |x| <= 2^51.
+    __device__ __forceinline__ static double mulmod(double x, double w, double wp, double q) {
+        const double h = x * w;
+        const double l = __fma_rn(x, w, -h);
+        const double k = __fma_rn(x, wp, kRound) - kRound;
+        const double t = __fma_rn(-k, q, h);
+        return t + l;
+    }
+    // a - round(a/q)*q, |result| <= q/2 + 1, for |a| < 2^52.
+    __device__ __forceinline__ static double reduce(double a, const DevTable &t) {
+        const double k = __fma_rn(a, t.qinv, kRound) - kRound;
+        return __fma_rn(-k, t.qd, a);
+    }
+    __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
+        const double u = reduce(a, t);                // |u| <= q/2 + 1
+        const double v = mulmod(b, w.w, w.wp, t.qd);  // |b| <= 1.126 q < 2^51
+        a = u + v;                                    // |a| <= 1.126 q
+        b = u - v;
+    }
+    __device__ __forceinline__ static void gs(T &a, T &b, const Tw &w, const DevTable &t) {
+        const double u = a, v = b;                        // |u|, |v| <= 0.6251 q
+        a = reduce(u + v, t);                             // <= q/2 + 1
+        b = mulmod(u - v, w.w, w.wp, t.qd);               // <= 0.6251 q
+    }
+    __device__ __forceinline__ static void gs_last(T &a, T &b, const DevTable &t) {
+        const double u = a, v = b;
+        a = mulmod(u + v, t.inv_n_f, t.inv_n_p, t.qd);
+        b = mulmod(u - v, t.sr_f, t.sr_p, t.qd);
+    }
+    // u64 in [0, 4q) -> centred lane value, |x| <= q/2 + 1
+    __device__ __forceinline__ static T load(uint64_t v, const DevTable &t) {
+        return reduce((double)v, t);
+    }
+    // centred |x| <= q -> [0, q)
+    __device__ __forceinline__ static uint64_t to_canon(double x, const DevTable &t) {
+        const double r = x < 0.0 ? x + t.qd : x;
+        return (uint64_t)r;
+    }
+    __device__ __forceinline__ static uint64_t canon_fwd(T v, const DevTable &t) {
+        return to_canon(reduce(v, t), t);
+    }
+    __device__ __forceinline__ static uint64_t out_inv(T v, const DevTable &t) {
+        return to_canon(v, t);  // |v| <= 0.6251 q
+    }
+    __device__ __forceinline__ static T mul_operand(T x, const ShoupF &w, const DevTable &t) {
+        return mulmod(x, w.w, w.wp, t.qd);
+    }
+    // x*y mod q for |x|, |y| <= 1.126 q: y/q is formed on the fly (two more roundings).
+    __device__ __forceinline__ static T mul_pair(T x, T y, const DevTable &t) {
+        const double yr = reduce(y, t);
+        return mulmod(reduce(x, t), yr, yr * t.qinv, t.qd);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// register rounds
+// ---------------------------------------------------------------------------
 
 // R forward stages on one register group x[0 .. 2^R), global stages [S, S+R),
 // chunk c at stage S.
-template <int R>
-__device__ __forceinline__ void ct_round(uint64_t *x, uint32_t S, uint32_t c,
-                                         const Shoup *__restrict__ tw, uint64_t q, uint64_t q2) {
+template <class A, int R>
+__device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t c,
+                                         const typename A::Tw *__restrict__ tw, const DevTable &t) {
 #pragma unroll
     for (int u = 0; u < R; ++u) {
         const int half = 1 << (R - u - 1);
         const uint32_t base = (1u << (S + u)) + (c << u);
 #pragma unroll
         for (int sub = 0; sub < (1 << u); ++sub) {
-            const Shoup w = tw[base + sub];
+            const typename A::Tw w = tw[base + sub];
 #pragma unroll
             for (int jj = 0; jj < half; ++jj) {
                 const int j0 = sub * 2 * half + jj;
-                ct_bf(x[j0], x[j0 + half], w.w, w.wq, q, q2);
+                A::ct(x[j0], x[j0 + half], w, t);
             }
         }
     }
 }
 
 // R inverse stages on one register group, global inverse stages [P, P+R)
-// (gaps 2^P .. 2^(P+R-1)), chunk c at stage P+R-1's grouping (k >> (P+R)).
-// LAST: the round ends with the transform's final stage (gap N/2).
-template <int R, bool LAST>
-__device__ __forceinline__ void gs_round(uint64_t *x, uint32_t P, uint32_t c, uint32_t n,
-                                         const Shoup *__restrict__ itw, const DevTable &t) {
+// (gaps 2^P .. 2^(P+R-1)), chunk c = k >> (P+R).  LAST: the round ends with the
+// transform's final stage (gap N/2).
+template <class A, int R, bool LAST>
+__device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t c, uint32_t n,
+                                         const typename A::Tw *__restrict__ itw, const DevTable &t) {
 #pragma unroll
     for (int u = 0; u < R; ++u) {
         const int half = 1 << u;
         if (LAST && u == R - 1) {
 #pragma unroll
-            for (int jj = 0; jj < half; ++jj) gs_last_bf(x[jj], x[jj + half], t);
+            for (int jj = 0; jj < half; ++jj) A::gs_last(x[jj], x[jj + half], t);
         } else {
             const uint32_t base = 1u + n - (n >> (P + u)) + (c << (R - u - 1));
 #pragma unroll
             for (int sub = 0; sub < (1 << (R - u - 1)); ++sub) {
-                const Shoup w = itw[base + sub];
+                const typename A::Tw w = itw[base + sub];
 #pragma unroll
                 for (int jj = 0; jj < half; ++jj) {
                     const int j0 = sub * 2 * half + jj;
-                    gs_bf(x[j0], x[j0 + half], w.w, w.wq, t.q, t.q2);
+                    A::gs(x[j0], x[j0 + half], w, t);
                 }
             }
         }

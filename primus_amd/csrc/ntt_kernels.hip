@@ -3,7 +3,7 @@
 //
 // Two kernel shapes cover ring degrees 2^1 .. 2^17:
 //
-// * chunk_kernel<LOGC, LOGT, MODE>: one workgroup of 2^LOGT threads owns a
+// * chunk_kernel<A, LOGC, LOGT, MODE>: one workgroup of 2^LOGT threads owns a
 //   contiguous chunk of 2^LOGC coefficients in LDS (plus one pad word per 16)
 //   and runs the chunk's stages as radix-2^LOGE register rounds, E = 2^LOGE
 //   coefficients per thread.  For N <= 2^13 the chunk is the whole
@@ -16,7 +16,8 @@
 //   4096-wide rows of a wave are read and written coalesced.  Together with a
 //   chunk pass of 4096 this is the two-pass (four-step) schedule.
 //
-// MODE bits select what a chunk does between its load and its store.
+// A is the arithmetic policy (IntArith or F64Arith, ntt_device.hpp).  MODE
+// bits select what a chunk does between its load and its store.
 #include "ntt_device.hpp"
 #include "ntt_launch.hpp"
 
@@ -26,7 +27,7 @@ enum : int {
     M_FWD = 1,        // forward stages of this chunk
     M_INV = 2,        // inverse stages of this chunk
     M_MUL_SHOUP = 4,  // multiply by a resident Shoup operand between them
-    M_MUL_POLY = 8,   // multiply by the transform of a second polynomial (Barrett)
+    M_MUL_POLY = 8,   // multiply by the transform of a second polynomial
     M_CANON = 16,     // forward output reduced to [0, q)
     M_INV_LAST = 32,  // the chunk holds the inverse's final stage (single pass)
 };
@@ -68,38 +69,40 @@ __device__ __forceinline__ uint32_t inv_slot(uint32_t t, int m, int j) {
     return (c << (P + R)) + ((uint32_t)j << P) + o;
 }
 
-template <int LOGC, int LOGT, int I>
+template <class A, int LOGC, int LOGT, int I>
 struct FwdRounds {
     using G = Geo<LOGC, LOGT>;
+    using T = typename A::T;
     static constexpr int S = G::fs(I), R = G::fr(I);
 
-    __device__ __forceinline__ static void store(uint64_t *lds, const uint64_t *x, uint32_t t) {
+    __device__ __forceinline__ static void store(T *lds, const T *x, uint32_t t) {
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j) lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))] = x[(m << R) + j];
     }
-    __device__ __forceinline__ static void load(const uint64_t *lds, uint64_t *x, uint32_t t) {
+    __device__ __forceinline__ static void load(const T *lds, T *x, uint32_t t) {
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))];
     }
-    __device__ __forceinline__ static void load_global(const uint64_t *__restrict__ g, uint64_t *x, uint32_t t) {
+    __device__ __forceinline__ static void load_global(const uint64_t *__restrict__ g, T *x, uint32_t t,
+                                                       const DevTable &tab) {
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = g[fwd_slot<LOGC, LOGT, S, R>(t, m, j)];
+            for (int j = 0; j < (1 << R); ++j)
+                x[(m << R) + j] = A::load(g[fwd_slot<LOGC, LOGT, S, R>(t, m, j)], tab);
     }
 
     // Runs rounds I .. NR-1.  st0: global stage of local stage 0; b: chunk index
     // of this workgroup inside the polynomial.
-    __device__ __forceinline__ static void run(uint64_t *x, uint64_t *lds, uint32_t t, uint32_t st0,
-                                               uint32_t b, const Shoup *__restrict__ tw, uint64_t q,
-                                               uint64_t q2) {
+    __device__ __forceinline__ static void run(T *x, T *lds, uint32_t t, uint32_t st0, uint32_t b,
+                                               const DevTable &tab) {
         if constexpr (I > 0) {
             __syncthreads();
-            FwdRounds<LOGC, LOGT, I - 1>::store(lds, x, t);
+            FwdRounds<A, LOGC, LOGT, I - 1>::store(lds, x, t);
             __syncthreads();
             load(lds, x, t);
         }
@@ -107,42 +110,44 @@ struct FwdRounds {
         for (int m = 0; m < (G::E >> R); ++m) {
             const uint32_t gi = t + ((uint32_t)m << LOGT);
             const uint32_t c = (b << S) + (gi >> (LOGC - S - R));
-            ct_round<R>(x + (m << R), st0 + S, c, tw, q, q2);
+            ct_round<A, R>(x + (m << R), st0 + S, c, A::fwd(tab), tab);
         }
-        if constexpr (I + 1 < G::NR) FwdRounds<LOGC, LOGT, I + 1>::run(x, lds, t, st0, b, tw, q, q2);
+        if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1>::run(x, lds, t, st0, b, tab);
     }
 };
 
-template <int LOGC, int LOGT, int I, bool LAST>
+template <class A, int LOGC, int LOGT, int I, bool LAST>
 struct InvRounds {
     using G = Geo<LOGC, LOGT>;
+    using T = typename A::T;
     static constexpr int P = G::is(I), R = G::ir(I);
 
-    __device__ __forceinline__ static void store(uint64_t *lds, const uint64_t *x, uint32_t t) {
+    __device__ __forceinline__ static void store(T *lds, const T *x, uint32_t t) {
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j) lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))] = x[(m << R) + j];
     }
-    __device__ __forceinline__ static void load(const uint64_t *lds, uint64_t *x, uint32_t t) {
+    __device__ __forceinline__ static void load(const T *lds, T *x, uint32_t t) {
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))];
     }
-    __device__ __forceinline__ static void store_global(uint64_t *__restrict__ g, const uint64_t *x, uint32_t t) {
+    __device__ __forceinline__ static void store_global(uint64_t *__restrict__ g, const T *x, uint32_t t,
+                                                        const DevTable &tab) {
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j) g[inv_slot<LOGC, LOGT, P, R>(t, m, j)] = x[(m << R) + j];
+            for (int j = 0; j < (1 << R); ++j)
+                g[inv_slot<LOGC, LOGT, P, R>(t, m, j)] = A::out_inv(x[(m << R) + j], tab);
     }
 
-    __device__ __forceinline__ static void run(uint64_t *x, uint64_t *lds, uint32_t t, uint32_t n,
-                                               uint32_t b, const Shoup *__restrict__ itw,
+    __device__ __forceinline__ static void run(T *x, T *lds, uint32_t t, uint32_t n, uint32_t b,
                                                const DevTable &tab) {
         if constexpr (I > 0) {
             __syncthreads();
-            InvRounds<LOGC, LOGT, I - 1, LAST>::store(lds, x, t);
+            InvRounds<A, LOGC, LOGT, I - 1, LAST>::store(lds, x, t);
             __syncthreads();
             load(lds, x, t);
         }
@@ -151,17 +156,21 @@ struct InvRounds {
         for (int m = 0; m < (G::E >> R); ++m) {
             const uint32_t gi = t + ((uint32_t)m << LOGT);
             const uint32_t c = (b << (LOGC - P - R)) + (gi >> P);
-            gs_round<R, last_here>(x + (m << R), P, c, n, itw, tab);
+            gs_round<A, R, last_here>(x + (m << R), P, c, n, A::inv(tab), tab);
         }
-        if constexpr (I + 1 < G::NR) InvRounds<LOGC, LOGT, I + 1, LAST>::run(x, lds, t, n, b, itw, tab);
+        if constexpr (I + 1 < G::NR) InvRounds<A, LOGC, LOGT, I + 1, LAST>::run(x, lds, t, n, b, tab);
     }
 };
 
-template <int LOGC, int LOGT, int MODE>
+__device__ __forceinline__ const Shoup *operand_of(const KArgs &a, const IntArith *) { return a.operand; }
+__device__ __forceinline__ const ShoupF *operand_of(const KArgs &a, const F64Arith *) { return a.operand_f; }
+
+template <class A, int LOGC, int LOGT, int MODE>
 __global__ __launch_bounds__(1 << LOGT) void chunk_kernel(KArgs a) {
     using G = Geo<LOGC, LOGT>;
+    using T = typename A::T;
     constexpr int LE = G::LOGE;
-    __shared__ uint64_t lds[G::LDS];
+    __shared__ T lds[G::LDS];
     const uint32_t t = threadIdx.x;
     const uint64_t blk = blockIdx.x;
     const uint64_t poly = blk / a.blocks_per_poly;
@@ -172,66 +181,57 @@ __global__ __launch_bounds__(1 << LOGT) void chunk_kernel(KArgs a) {
     const uint32_t st0 = tab.log_n - LOGC;
     const uint64_t off = poly * n + (uint64_t)b * G::C;
     uint64_t *data = a.data + off;
-    uint64_t x[G::E];
+    T x[G::E];
 
     if constexpr ((MODE & M_FWD) != 0) {
         if constexpr ((MODE & M_MUL_POLY) != 0) {
             // transform the second operand first and keep it in registers
-            uint64_t y[G::E];
-            FwdRounds<LOGC, LOGT, 0>::load_global(a.data2 + off, y, t);
-            FwdRounds<LOGC, LOGT, 0>::run(y, lds, t, st0, b, tab.fwd, tab.q, tab.q2);
-            FwdRounds<LOGC, LOGT, 0>::load_global(data, x, t);
-            FwdRounds<LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab.fwd, tab.q, tab.q2);
-            // NttPolynomial::mul_assign on canonical values (primus_poly/src/ntt/mul.rs:87-95)
+            T y[G::E];
+            FwdRounds<A, LOGC, LOGT, 0>::load_global(a.data2 + off, y, t, tab);
+            FwdRounds<A, LOGC, LOGT, 0>::run(y, lds, t, st0, b, tab);
+            FwdRounds<A, LOGC, LOGT, 0>::load_global(data, x, t, tab);
+            FwdRounds<A, LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab);
+            // NttPolynomial::mul_assign (primus_poly/src/ntt/mul.rs:87-95)
 #pragma unroll
-            for (int j = 0; j < G::E; ++j) {
-                uint64_t xa = sub_if_ge(sub_if_ge(x[j], tab.q2), tab.q);
-                uint64_t yb = sub_if_ge(sub_if_ge(y[j], tab.q2), tab.q);
-                x[j] = barrett_mul(xa, yb, tab.q, tab.br0, tab.br1);
-            }
+            for (int j = 0; j < G::E; ++j) x[j] = A::mul_pair(x[j], y[j], tab);
         } else {
-            FwdRounds<LOGC, LOGT, 0>::load_global(data, x, t);
-            FwdRounds<LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab.fwd, tab.q, tab.q2);
+            FwdRounds<A, LOGC, LOGT, 0>::load_global(data, x, t, tab);
+            FwdRounds<A, LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab);
         }
         // after the last forward round thread t holds local slots t*E .. t*E+E-1
         if constexpr ((MODE & M_MUL_SHOUP) != 0) {
-            const Shoup *op = a.operand + (uint64_t)limb * n + (uint64_t)b * G::C + ((uint64_t)t << LE);
+            const auto *op = operand_of(a, (const A *)nullptr) + (uint64_t)limb * n + (uint64_t)b * G::C + ((uint64_t)t << LE);
 #pragma unroll
-            for (int j = 0; j < G::E; ++j) {
-                const Shoup w = op[j];
-                x[j] = shoup_full(w.w, w.wq, x[j], tab.q);
-            }
+            for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], op[j], tab);
         }
         if constexpr ((MODE & M_INV) == 0) {
             uint64_t *dst = data + ((uint64_t)t << LE);
 #pragma unroll
             for (int j = 0; j < G::E; ++j) {
-                uint64_t v = x[j];
-                if constexpr ((MODE & M_CANON) != 0) v = sub_if_ge(sub_if_ge(v, tab.q2), tab.q);
-                dst[j] = v;
+                if constexpr ((MODE & M_CANON) != 0) dst[j] = A::canon_fwd(x[j], tab);
+                else dst[j] = (uint64_t)x[j];  // IntArith only: lazy [0,4q)
             }
             return;
         }
     } else {
         const uint64_t *src = data + ((uint64_t)t << LE);
 #pragma unroll
-        for (int j = 0; j < G::E; ++j) x[j] = src[j];
+        for (int j = 0; j < G::E; ++j) x[j] = A::load(src[j], tab);
     }
 
     if constexpr ((MODE & M_INV) != 0) {
         constexpr bool LAST = (MODE & M_INV_LAST) != 0;
-        InvRounds<LOGC, LOGT, 0, LAST>::run(x, lds, t, n, b, tab.inv, tab);
-        InvRounds<LOGC, LOGT, G::NR - 1, LAST>::store_global(data, x, t);
+        InvRounds<A, LOGC, LOGT, 0, LAST>::run(x, lds, t, n, b, tab);
+        InvRounds<A, LOGC, LOGT, G::NR - 1, LAST>::store_global(data, x, t, tab);
     }
 }
 
-// Column pass of the two-pass schedule: forward stages [0, LOGR) or inverse
-// stages [logN-LOGR, logN) on columns k = i*C + j, C = N >> LOGR.
+// Column pass of the two-pass schedule (IntArith): forward stages [0, LOGR) or
+// inverse stages [logN-LOGR, logN) on columns k = i*C + j, C = N >> LOGR.
 template <int LOGR, int MODE>
 __global__ __launch_bounds__(256) void column_kernel(KArgs a) {
     const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    const DevTable tab0 = a.tabs[0];
-    const uint32_t logc = tab0.log_n - LOGR;
+    const uint32_t logc = a.tabs[0].log_n - LOGR;
     const uint64_t poly = gid >> logc;
     const uint32_t j = (uint32_t)(gid & ((1u << logc) - 1u));
     const uint32_t limb = (uint32_t)(poly % a.L);
@@ -244,9 +244,9 @@ __global__ __launch_bounds__(256) void column_kernel(KArgs a) {
 #pragma unroll
     for (int i = 0; i < (1 << LOGR); ++i) x[i] = src[(uint64_t)i << logc];
     if constexpr ((MODE & M_FWD) != 0) {
-        ct_round<LOGR>(x, 0, 0, tab.fwd, tab.q, tab.q2);
+        ct_round<IntArith, LOGR>(x, 0, 0, tab.fwd, tab);
     } else {
-        gs_round<LOGR, true>(x, logc, 0, n, tab.inv, tab);
+        gs_round<IntArith, LOGR, true>(x, logc, 0, n, tab.inv, tab);
     }
 #pragma unroll
     for (int i = 0; i < (1 << LOGR); ++i) dst[(uint64_t)i << logc] = x[i];
@@ -299,11 +299,11 @@ __global__ void pointwise_kernel(const DevTable *tabs, uint32_t L, uint32_t log_
 // launchers
 // ---------------------------------------------------------------------------
 
-template <int LOGC, int LOGT, int MODE>
+template <class A, int LOGC, int LOGT, int MODE>
 static hipError_t launch_chunk(const KArgs &a, uint64_t count, hipStream_t s) {
     const uint64_t blocks = count * a.blocks_per_poly;
     if (blocks == 0) return hipSuccess;
-    chunk_kernel<LOGC, LOGT, MODE><<<dim3((uint32_t)blocks), dim3(1u << LOGT), 0, s>>>(a);
+    chunk_kernel<A, LOGC, LOGT, MODE><<<dim3((uint32_t)blocks), dim3(1u << LOGT), 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -317,17 +317,30 @@ static hipError_t launch_column(const KArgs &a, uint32_t log_n, uint64_t count, 
 
 // Single-pass chunk launch for log_n = LOGN <= 13 with the mode chosen at run time.
 template <int LOGN>
-static hipError_t single_pass(Op op, KArgs a, uint64_t count, hipStream_t s) {
+static hipError_t single_pass(Op op, KArgs a, uint64_t count, bool f64, hipStream_t s) {
     constexpr int LOGT = LOGN > 4 ? LOGN - 4 : 0;
     a.blocks_per_poly = 1;
+    if constexpr (LOGN >= 6) {
+        if (f64) {
+            switch (op) {
+                case Op::FwdLazy: break;  // lazy values are defined by the integer butterflies
+                case Op::FwdCanon: return launch_chunk<F64Arith, LOGN, LOGT, M_FWD | M_CANON>(a, count, s);
+                case Op::Inv: return launch_chunk<F64Arith, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
+                case Op::FwdMulInv:
+                    return launch_chunk<F64Arith, LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count, s);
+                case Op::PolyMul:
+                    return launch_chunk<F64Arith, LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
+            }
+        }
+    }
     switch (op) {
-        case Op::FwdLazy: return launch_chunk<LOGN, LOGT, M_FWD>(a, count, s);
-        case Op::FwdCanon: return launch_chunk<LOGN, LOGT, M_FWD | M_CANON>(a, count, s);
-        case Op::Inv: return launch_chunk<LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
+        case Op::FwdLazy: return launch_chunk<IntArith, LOGN, LOGT, M_FWD>(a, count, s);
+        case Op::FwdCanon: return launch_chunk<IntArith, LOGN, LOGT, M_FWD | M_CANON>(a, count, s);
+        case Op::Inv: return launch_chunk<IntArith, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
         case Op::FwdMulInv:
-            return launch_chunk<LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count, s);
+            return launch_chunk<IntArith, LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count, s);
         case Op::PolyMul:
-            return launch_chunk<LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
+            return launch_chunk<IntArith, LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
     }
     return hipErrorInvalidValue;
 }
@@ -336,6 +349,7 @@ static hipError_t single_pass(Op op, KArgs a, uint64_t count, hipStream_t s) {
 template <int LOGR>
 static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStream_t s) {
     constexpr int LOGC = 12, LOGT = 8;
+    using A = IntArith;
     const uint32_t log_n = LOGC + LOGR;
     a.blocks_per_poly = 1u << LOGR;
     hipError_t e = hipSuccess;
@@ -346,10 +360,10 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
             col.data2 = a.data;
             e = launch_column<LOGR, M_FWD>(col, log_n, count, s);
             if (e != hipSuccess) return e;
-            if (op == Op::FwdLazy) return launch_chunk<LOGC, LOGT, M_FWD>(a, count, s);
-            return launch_chunk<LOGC, LOGT, M_FWD | M_CANON>(a, count, s);
+            if (op == Op::FwdLazy) return launch_chunk<A, LOGC, LOGT, M_FWD>(a, count, s);
+            return launch_chunk<A, LOGC, LOGT, M_FWD | M_CANON>(a, count, s);
         case Op::Inv:
-            e = launch_chunk<LOGC, LOGT, M_INV>(a, count, s);
+            e = launch_chunk<A, LOGC, LOGT, M_INV>(a, count, s);
             if (e != hipSuccess) return e;
             col.data2 = a.data;
             return launch_column<LOGR, M_INV>(col, log_n, count, s);
@@ -357,7 +371,7 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
             col.data2 = a.data;
             e = launch_column<LOGR, M_FWD>(col, log_n, count, s);
             if (e != hipSuccess) return e;
-            e = launch_chunk<LOGC, LOGT, M_FWD | M_MUL_SHOUP | M_INV>(a, count, s);
+            e = launch_chunk<A, LOGC, LOGT, M_FWD | M_MUL_SHOUP | M_INV>(a, count, s);
             if (e != hipSuccess) return e;
             return launch_column<LOGR, M_INV>(col, log_n, count, s);
         case Op::PolyMul: {
@@ -372,7 +386,7 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
             if (e != hipSuccess) return e;
             KArgs ch = a;
             ch.data2 = (const uint64_t *)scratch;
-            e = launch_chunk<LOGC, LOGT, M_FWD | M_MUL_POLY | M_INV>(ch, count, s);
+            e = launch_chunk<A, LOGC, LOGT, M_FWD | M_MUL_POLY | M_INV>(ch, count, s);
             if (e != hipSuccess) return e;
             return launch_column<LOGR, M_INV>(col, log_n, count, s);
         }
@@ -382,22 +396,22 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
 
 bool needs_scratch(Op op, uint32_t log_n) { return op == Op::PolyMul && log_n > kMaxSinglePassLogN; }
 
-hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch,
+hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, bool f64,
                             hipStream_t s) {
     switch (log_n) {
-        case 1: return single_pass<1>(op, a, count, s);
-        case 2: return single_pass<2>(op, a, count, s);
-        case 3: return single_pass<3>(op, a, count, s);
-        case 4: return single_pass<4>(op, a, count, s);
-        case 5: return single_pass<5>(op, a, count, s);
-        case 6: return single_pass<6>(op, a, count, s);
-        case 7: return single_pass<7>(op, a, count, s);
-        case 8: return single_pass<8>(op, a, count, s);
-        case 9: return single_pass<9>(op, a, count, s);
-        case 10: return single_pass<10>(op, a, count, s);
-        case 11: return single_pass<11>(op, a, count, s);
-        case 12: return single_pass<12>(op, a, count, s);
-        case 13: return single_pass<13>(op, a, count, s);
+        case 1: return single_pass<1>(op, a, count, f64, s);
+        case 2: return single_pass<2>(op, a, count, f64, s);
+        case 3: return single_pass<3>(op, a, count, f64, s);
+        case 4: return single_pass<4>(op, a, count, f64, s);
+        case 5: return single_pass<5>(op, a, count, f64, s);
+        case 6: return single_pass<6>(op, a, count, f64, s);
+        case 7: return single_pass<7>(op, a, count, f64, s);
+        case 8: return single_pass<8>(op, a, count, f64, s);
+        case 9: return single_pass<9>(op, a, count, f64, s);
+        case 10: return single_pass<10>(op, a, count, f64, s);
+        case 11: return single_pass<11>(op, a, count, f64, s);
+        case 12: return single_pass<12>(op, a, count, f64, s);
+        case 13: return single_pass<13>(op, a, count, f64, s);
         case 14: return two_pass<2>(op, a, count, scratch, s);
         case 15: return two_pass<3>(op, a, count, scratch, s);
         case 16: return two_pass<4>(op, a, count, scratch, s);

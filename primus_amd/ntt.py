@@ -70,10 +70,14 @@ class UintNttTable:
         self._owner = owner  # a DCRT table that owns this one, if any
 
     @classmethod
-    def new(cls, log_n, modulus):
+    def new(cls, log_n, modulus, integer_only=False):
         h = ctypes.c_void_p()
-        _check(lib().ph_ntt_table_new(log_n, modulus, ctypes.byref(h)))
+        _check(lib().ph_ntt_table_new_ex(log_n, modulus, int(bool(integer_only)), ctypes.byref(h)))
         return cls(h)
+
+    def arith(self):
+        """'f64' if canonical transforms use binary64 FMA arithmetic, else 'int'."""
+        return "f64" if lib().ph_ntt_table_arith(self._h) == 1 else "int"
 
     def __del__(self):
         if getattr(self, "_owner", None) is None and getattr(self, "_h", None):
@@ -162,11 +166,15 @@ class UintCrtNttTable:
         self._h = handle
 
     @classmethod
-    def new(cls, log_n, moduli):
+    def new(cls, log_n, moduli, integer_only=False):
         arr = np.ascontiguousarray(np.array(moduli, dtype=np.uint64))
         h = ctypes.c_void_p()
-        _check(lib().ph_dcrt_table_new(log_n, arr.ctypes.data_as(_p64), arr.size, ctypes.byref(h)))
+        _check(lib().ph_dcrt_table_new_ex(log_n, arr.ctypes.data_as(_p64), arr.size, int(bool(integer_only)),
+                                          ctypes.byref(h)))
         return cls(h)
+
+    def arith(self):
+        return "f64" if lib().ph_dcrt_table_arith(self._h) == 1 else "int"
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -20,11 +20,13 @@ def _rand(q, shape, seed):
 
 # ---------------------------------------------------------------- golden vectors
 
-def test_golden_slices(golden):
+@pytest.mark.parametrize("integer_only", [False, True])
+def test_golden_slices(golden, integer_only):
     for label, c in golden.items():
         q, log_n = int(c["q"][0]), int(c["log_n"][0])
-        print("golden", label, flush=True)
-        t = pa.UintNttTable.new(log_n, q)
+        print("golden", label, integer_only, flush=True)
+        t = pa.UintNttTable.new(log_n, q, integer_only=integer_only)
+        assert t.arith() == ("f64" if (q < 1 << 50 and not integer_only) else "int")
         assert t.root() == int(c["psi"][0]), label
         x = c["a"].copy()
         t.transform_slice(x)
@@ -45,12 +47,13 @@ def test_golden_slices(golden):
         np.testing.assert_array_equal(m, c["mono_minus_one"], err_msg=label)
 
 
-def test_golden_negacyclic_device(golden):
+@pytest.mark.parametrize("integer_only", [False, True])
+def test_golden_negacyclic_device(golden, integer_only):
     for label, c in golden.items():
         if "prod" not in c:
             continue
         q, log_n = int(c["q"][0]), int(c["log_n"][0])
-        tab = pa.UintCrtNttTable.new(log_n, [q])
+        tab = pa.UintCrtNttTable.new(log_n, [q], integer_only=integer_only)
         da, db = pa.DeviceBuffer.from_array(c["a"]), pa.DeviceBuffer.from_array(c["b"])
         pa.negacyclic_mul_device(tab, da.ptr, db.ptr, 1)
         pa.synchronize()
@@ -93,8 +96,8 @@ def test_all_degrees_bit_exact(log_n):
     np.testing.assert_array_equal(y, ot.inverse_transform(b))
 
 
-@pytest.mark.parametrize("q", [17, 12289, 2147473409, 4294957057, 1152921504606830593,
-                               4611686018427322369])
+@pytest.mark.parametrize("q", [17, 12289, 2147473409, 4294957057, 1125899906826241, 1125899906260993,
+                               1152921504606830593, 4611686018427322369])
 def test_moduli_edge_cases(q):
     log_n = min(10, ((q - 1) & -(q - 1)).bit_length() - 2)
     n = 1 << log_n
@@ -145,13 +148,14 @@ def _fmi_reference(tables, polys, op):
     return out
 
 
+@pytest.mark.parametrize("integer_only", [False, True])
 @pytest.mark.parametrize("log_n,L,count", [(12, 4, 1024), (10, 3, 30), (13, 2, 8), (14, 2, 4),
-                                           (16, 1, 2), (17, 2, 2), (5, 2, 6)])
-def test_fwd_mul_inv_batch(log_n, L, count):
+                                           (16, 1, 2), (17, 2, 2), (5, 2, 6), (6, 2, 4)])
+def test_fwd_mul_inv_batch(log_n, L, count, integer_only):
     n = 1 << log_n
     moduli = pa.ntt_primes_below(50, 2 * n, L)
-    print("fwd_mul_inv", log_n, L, count, flush=True)
-    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    print("fwd_mul_inv", log_n, L, count, integer_only, flush=True)
+    tab = pa.UintCrtNttTable.new(log_n, moduli, integer_only=integer_only)
     rng = np.random.default_rng(log_n * 100 + L)
     polys = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
     op_vals = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in moduli])
@@ -164,11 +168,12 @@ def test_fwd_mul_inv_batch(log_n, L, count):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("bits", [59, 50, 31])
 @pytest.mark.parametrize("log_n,L,count", [(10, 1, 64), (12, 3, 6), (13, 1, 3), (14, 1, 2), (15, 2, 2),
-                                           (16, 1, 1), (3, 1, 5)])
-def test_negacyclic_batch(log_n, L, count):
+                                           (16, 1, 1), (3, 1, 5), (7, 2, 4)])
+def test_negacyclic_batch(log_n, L, count, bits):
     n = 1 << log_n
-    moduli = pa.ntt_primes_below(59, 2 * n, L)
+    moduli = pa.ntt_primes_below(bits, 2 * n, L)
     print("negacyclic", log_n, L, count, flush=True)
     tab = pa.UintCrtNttTable.new(log_n, moduli)
     rng = np.random.default_rng(log_n)
@@ -188,10 +193,11 @@ def test_negacyclic_batch(log_n, L, count):
     np.testing.assert_array_equal(db.download((count, n)), b)
 
 
-def test_device_forward_inverse_batch():
+@pytest.mark.parametrize("bits", [62, 50, 40])
+def test_device_forward_inverse_batch(bits):
     log_n, L, count = 11, 3, 12
     n = 1 << log_n
-    moduli = pa.ntt_primes_below(62, 2 * n, L)
+    moduli = pa.ntt_primes_below(bits, 2 * n, L)
     tab = pa.UintCrtNttTable.new(log_n, moduli)
     tables = [O.Table(log_n, q) for q in moduli]
     rng = np.random.default_rng(3)
@@ -204,6 +210,14 @@ def test_device_forward_inverse_batch():
         np.testing.assert_array_equal(lz[i], tables[i % L].lazy_transform(x[i]))
     d.upload(x)
     pa.transform_device(tab, d.ptr, count, lazy=False)
+    pa.synchronize()
+    canon = d.download((count, n))
+    for i in range(count):
+        np.testing.assert_array_equal(canon[i], tables[i % L].transform(x[i]))
+    # inverse from [0, 2q)
+    y = (canon + np.stack([rng.integers(0, 2, size=n, dtype=np.uint64) * np.uint64(moduli[i % L])
+                           for i in range(count)])).astype(np.uint64)
+    d.upload(y)
     pa.inverse_transform_device(tab, d.ptr, count)
     pa.synchronize()
     back = d.download((count, n))
@@ -230,3 +244,30 @@ def test_pointwise_device():
         q = moduli[i % L]
         np.testing.assert_array_equal(out[i], O.pointwise_mul(a[i], b[i], q))
         np.testing.assert_array_equal(acc[i], O.pointwise_mul_add(c[i], a[i], b[i], q))
+
+
+@pytest.mark.parametrize("q", [1125899906826241, 1125899906260993, 2147473409])
+def test_f64_extreme_operands(q):
+    """Largest magnitudes the binary64 path can meet: every input and operand q-1 (or 4q-1)."""
+    log_n = 12
+    n = 1 << log_n
+    tab = pa.UintCrtNttTable.new(log_n, [q])
+    assert tab.arith() == "f64"
+    ot = O.Table(log_n, q)
+    rng = np.random.default_rng(q % 997)
+    cases = [np.full(n, q - 1, dtype=np.uint64), np.full(n, 4 * q - 1, dtype=np.uint64),
+             np.where(rng.integers(0, 2, size=n) == 1, q - 1, 0).astype(np.uint64)]
+    for a in cases:
+        for opv in [np.full(n, q - 1, dtype=np.uint64), rng.integers(0, q, size=n, dtype=np.uint64)]:
+            op = pa.Operand(tab, opv)
+            d = pa.DeviceBuffer.from_array(a)
+            pa.fwd_mul_inv_device(tab, d.ptr, 1, op)
+            pa.synchronize()
+            want = ot.inverse_transform(O.pointwise_mul(ot.transform(a), opv, q))
+            np.testing.assert_array_equal(d.download(), want)
+        b = np.minimum(a, np.uint64(q - 1))
+        da, db = pa.DeviceBuffer.from_array(b), pa.DeviceBuffer.from_array(np.full(n, q - 1, dtype=np.uint64))
+        pa.negacyclic_mul_device(tab, da.ptr, db.ptr, 1)
+        pa.synchronize()
+        want = ot.inverse_transform(O.pointwise_mul(ot.transform(b), ot.transform(np.full(n, q - 1, dtype=np.uint64)), q))
+        np.testing.assert_array_equal(da.download(), want)
