@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="counter runs: only the fused batch kernel (--steps launches) and one 256 MiB "
+                         "device-to-device copy as the byte-count calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -108,6 +111,9 @@ def main():
     moduli, polys, operand = workload(rng)
     table = pa.UintCrtNttTable.new(LOG_N, moduli)
     op = pa.Operand(table, operand)
+    from primus_amd import tuning
+
+    shape, shape_ms = tuning.autotune(table, "fwd_mul_inv", COUNT, reps=20)
 
     dev = torch.device("cuda:0")
     host = torch.from_numpy(polys.view(np.int64))
@@ -132,6 +138,7 @@ def main():
     otabs = [O.Table(LOG_N, q) for q in moduli]
     int_table = pa.UintCrtNttTable.new(LOG_N, moduli, integer_only=True)
     int_op = pa.Operand(int_table, operand)
+    int_shape, _ = tuning.autotune(int_table, "fwd_mul_inv", COUNT, reps=10)
     int_step = launcher(int_table, int_op)
     for st in (step, int_step):
         st(0)
@@ -164,6 +171,17 @@ def main():
         return per, evs[0].elapsed_time(evs[K]), wall
 
     K = args.steps
+    if args.profile:
+        per, total_ms, _ = timed(step, K, args.warmup)
+        src = torch.empty(32 << 20, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # calibration: 256 MiB read + 256 MiB written
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": True, "kernel_ms": float(np.mean(per)), "launches": K + args.warmup,
+                          "calibration_copy_bytes": 2 * src.numel() * 8}))
+        return 0
     per, total_ms, wall = timed(step, K, args.warmup)
     ms_per_step = total_ms / K
     value = NTT_PER_STEP / (ms_per_step * 1e-3)
@@ -228,8 +246,10 @@ def main():
         },
         "batched_vs_plain": plain_ms / ms_per_step,
         "arith": table.arith(),
+        "autotune": {"coefficients_per_thread": 1 << shape, "best_ms": shape_ms},
         "int_path": {
             "desc": "same fused kernel on 64-bit integer Shoup arithmetic (32-bit multiply chains)",
+            "coefficients_per_thread": 1 << int_shape,
             "ms_per_batch": int_ms,
             "ntt_per_s": NTT_PER_STEP / (int_ms * 1e-3),
             "hbm_frac": BYTES_PER_STEP / (int_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,

@@ -173,6 +173,27 @@ int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *table, const uint64_t *d_a
 int ph_dcrt_pointwise_mul_add_device(const ph_dcrt_table *table, uint64_t *d_acc, const uint64_t *d_a,
                                      const uint64_t *d_b, size_t count, void *stream);
 
+/* ---- autotuning (no reference counterpart: launch-shape selection for the GPU) ---- */
+
+/* Operation codes for the tuning table. */
+#define PH_OP_FORWARD_LAZY 0
+#define PH_OP_FORWARD 1
+#define PH_OP_INVERSE 2
+#define PH_OP_FWD_MUL_INV 3
+#define PH_OP_NEGACYCLIC_MUL 4
+
+/* Process-wide tuning table: for (log_n, op, arith: 0 integer / 1 binary64) the
+ * log2 of coefficients per thread of the single-pass kernels (3 or 4; 0 = the
+ * default).  Results never depend on it. */
+int ph_tuning_set(uint32_t log_n, int op, int arith, int loge);
+int ph_tuning_get(uint32_t log_n, int op, int arith);
+int ph_tuning_save(const char *path);
+int ph_tuning_load(const char *path);
+/* Times every launch shape of `op` for this table on `count` limb-polynomials
+ * (reps timed launches each), stores the fastest in the tuning table and
+ * reports it. */
+int ph_autotune(const ph_dcrt_table *table, int op, size_t count, int reps, int *chosen, float *best_ms);
+
 /* ---- device memory helpers (for callers without their own HIP allocator) ---- */
 int ph_device_alloc(size_t bytes, void **d_ptr);
 int ph_device_free(void *d_ptr);

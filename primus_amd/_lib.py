@@ -129,6 +129,11 @@ def _declare(L):
         "ph_memcpy_htod": (i, [vp, vp, sz]),
         "ph_memcpy_dtoh": (i, [vp, vp, sz]),
         "ph_stream_synchronize": (i, [vp]),
+        "ph_tuning_set": (i, [u32, i, i, i]),
+        "ph_tuning_get": (i, [u32, i, i]),
+        "ph_tuning_save": (i, [ctypes.c_char_p]),
+        "ph_tuning_load": (i, [ctypes.c_char_p]),
+        "ph_autotune": (i, [vp, i, sz, i, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_float)]),
     }
     for name, (res, args) in sigs.items():
         if not hasattr(L, name):

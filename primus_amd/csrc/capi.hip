@@ -8,6 +8,8 @@
 
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <chrono>
 #include <mutex>
 #include <new>
 #include <string>
@@ -40,8 +42,8 @@ struct ph_dcrt_table {
 struct ph_operand {
     uint32_t L;
     uint64_t n;
-    Shoup *d = nullptr;
-    ShoupF *d_f = nullptr;
+    Shoup *d = nullptr;      // ShoupFactor form (integer path)
+    uint64_t *d_v = nullptr; // values alone (binary64 path)
 };
 
 namespace {
@@ -136,8 +138,13 @@ int upload(void **dst, const void *src, size_t bytes) {
     return PH_OK;
 }
 
-// binary64 twiddle {w, w/q}: w/q from the Shoup quotient, |wp - w/q| <= 2^-54 + 2^-64.
-ShoupF mkf(const Shoup &s) { return ShoupF{(double)s.w, (double)s.wq * 0x1p-64}; }
+// binary64 twiddle, centred: w' = w or w - q (|w'| <= q/2) and w'/q from the
+// Shoup quotient, |wp - w'/q| <= 2^-54 + 2^-64 (the subtraction of 1 is exact).
+ShoupF mkf(const Shoup &s, uint64_t q) {
+    const double frac = (double)s.wq * 0x1p-64;  // w/q
+    if (s.w > q / 2) return ShoupF{(double)s.w - (double)q, frac - 1.0};
+    return ShoupF{(double)s.w, frac};
+}
 
 // UintNttTable::new, primus_ntt/src/ntt/primitive.rs:114-185.
 int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) {
@@ -204,8 +211,8 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
     if (t->f64) {
         std::vector<ShoupF> ff(n), fi(n);
         for (uint64_t k = 0; k < n; ++k) {
-            ff[k] = mkf(t->fwd[k]);
-            fi[k] = mkf(t->inv[k]);
+            ff[k] = mkf(t->fwd[k], q);
+            fi[k] = mkf(t->inv[k], q);
         }
         rc = upload((void **)&t->d_fwd_f, ff.data(), n * sizeof(ShoupF));
         if (rc == PH_OK) rc = upload((void **)&t->d_inv_f, fi.data(), n * sizeof(ShoupF));
@@ -215,8 +222,8 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
         }
         d.qd = (double)q;
         d.qinv = 1.0 / (double)q;
-        ShoupF in = mkf(Shoup{t->inv_n, t->inv_n_q});
-        ShoupF sr = mkf(Shoup{d.sr, d.sr_q});
+        ShoupF in = mkf(Shoup{t->inv_n, t->inv_n_q}, q);
+        ShoupF sr = mkf(Shoup{d.sr, d.sr_q}, q);
         d.inv_n_f = in.w;
         d.inv_n_p = in.wp;
         d.sr_f = sr.w;
@@ -257,6 +264,16 @@ int staging(size_t bytes, void **buf, hipStream_t *stream) {
     return PH_OK;
 }
 
+// Autotuned launch shape per (log_n, op, arithmetic): log2 of the coefficients
+// each thread holds in single-pass kernels (0 = the default).  Written by
+// ph_tuning_set / ph_autotune / ph_tuning_load, read by every launch.
+std::atomic<int> g_tuning[kMaxLogN + 1][5][2];
+
+int tuning_of(uint32_t log_n, Op op, bool f64) {
+    if (log_n > kMaxLogN) return 0;
+    return g_tuning[log_n][(int)op][f64 ? 1 : 0].load(std::memory_order_relaxed);
+}
+
 int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, uint64_t *d_data,
            const uint64_t *d_data2, const ph_operand *operand, uint64_t count, hipStream_t s) {
     if (count == 0) return PH_OK;
@@ -264,13 +281,13 @@ int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, 
     a.data = d_data;
     a.data2 = d_data2;
     a.operand = operand ? operand->d : nullptr;
-    a.operand_f = operand ? operand->d_f : nullptr;
-    if (op == Op::FwdMulInv && f64 && !a.operand_f) f64 = false;
+    a.operand_v = operand ? operand->d_v : nullptr;
+    if (op == Op::FwdMulInv && f64 && !a.operand_v) f64 = false;
     a.tabs = d_tabs;
     a.L = L;
     void *scratch = nullptr;
     if (needs_scratch(op, log_n)) PH_HIP(hipMallocAsync(&scratch, (count << log_n) * 8, s));
-    hipError_t e = launch_transform(op, a, log_n, count, scratch, f64, s);
+    hipError_t e = launch_transform(op, a, log_n, count, scratch, f64, tuning_of(log_n, op, f64), s);
     if (scratch) (void)hipFreeAsync(scratch, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return PH_OK;
@@ -500,14 +517,14 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
     *out = nullptr;
     const uint64_t n = 1ull << d->log_n;
     std::vector<Shoup> h((size_t)d->L * n);
-    std::vector<ShoupF> hf(d->f64 ? (size_t)d->L * n : 0);
+
     for (uint32_t l = 0; l < d->L; ++l) {
         const uint64_t q = d->tables[l]->q;
         for (uint64_t i = 0; i < n; ++i) {
             uint64_t v = host_values[l * n + i];
             if (v >= q) return fail(PH_ERR_INVALID, "operand values must be canonical (< q)");
             h[l * n + i] = mk(v, q);
-            if (d->f64) hf[l * n + i] = mkf(h[l * n + i]);
+
         }
     }
     ph_operand *op = new (std::nothrow) ph_operand();
@@ -515,7 +532,7 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
     op->L = d->L;
     op->n = n;
     int rc = upload((void **)&op->d, h.data(), h.size() * sizeof(Shoup));
-    if (rc == PH_OK && d->f64) rc = upload((void **)&op->d_f, hf.data(), hf.size() * sizeof(ShoupF));
+    if (rc == PH_OK && d->f64) rc = upload((void **)&op->d_v, host_values, (size_t)d->L * n * sizeof(uint64_t));
     if (rc != PH_OK) {
         ph_operand_free(op);
         return rc;
@@ -527,7 +544,7 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
 void ph_operand_free(ph_operand *op) {
     if (!op) return;
     if (op->d) (void)hipFree(op->d);
-    if (op->d_f) (void)hipFree(op->d_f);
+    if (op->d_v) (void)hipFree(op->d_v);
     delete op;
 }
 
@@ -584,6 +601,128 @@ int ph_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes) {
 int ph_stream_synchronize(void *stream) {
     PH_HIP(hipStreamSynchronize((hipStream_t)stream));
     return PH_OK;
+}
+
+int ph_tuning_set(uint32_t log_n, int op, int arith, int loge) {
+    if (log_n > kMaxLogN || op < 0 || op > 4 || arith < 0 || arith > 1 || (loge != 0 && loge != 3 && loge != 4))
+        return fail(PH_ERR_INVALID, "bad tuning key or value");
+    g_tuning[log_n][op][arith].store(loge, std::memory_order_relaxed);
+    return PH_OK;
+}
+
+int ph_tuning_get(uint32_t log_n, int op, int arith) {
+    if (log_n > kMaxLogN || op < 0 || op > 4 || arith < 0 || arith > 1) return -1;
+    return g_tuning[log_n][op][arith].load(std::memory_order_relaxed);
+}
+
+int ph_tuning_save(const char *path) {
+    if (!path) return fail(PH_ERR_INVALID, "null path");
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(PH_ERR_INVALID, "cannot open tuning file for writing");
+    fprintf(f, "# primus_hip tuning: log_n op arith loge\n");
+    for (uint32_t n = 0; n <= kMaxLogN; ++n)
+        for (int op = 0; op < 5; ++op)
+            for (int ar = 0; ar < 2; ++ar) {
+                int v = g_tuning[n][op][ar].load();
+                if (v) fprintf(f, "%u %d %d %d\n", n, op, ar, v);
+            }
+    fclose(f);
+    return PH_OK;
+}
+
+int ph_tuning_load(const char *path) {
+    if (!path) return fail(PH_ERR_INVALID, "null path");
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(PH_ERR_INVALID, "cannot open tuning file");
+    char line[256];
+    int rc = PH_OK;
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        unsigned n;
+        int op, ar, v;
+        if (sscanf(line, "%u %d %d %d", &n, &op, &ar, &v) == 4) {
+            if (ph_tuning_set(n, op, ar, v) != PH_OK) rc = PH_ERR_INVALID;
+        }
+    }
+    fclose(f);
+    return rc;
+}
+
+int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *chosen, float *best_ms) {
+    if (!d || op < 0 || op > 4 || count == 0 || reps < 1) return fail(PH_ERR_INVALID, "bad autotune request");
+    const Op o = (Op)op;
+    const bool f64 = d->f64 && o != Op::FwdLazy;
+    const uint64_t n = 1ull << d->log_n;
+    const size_t words = count * n;
+    uint64_t *buf = nullptr, *buf2 = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PH_OK;
+    std::vector<Shoup> ones;
+    std::vector<uint64_t> onesv;
+    ph_operand op_dev{};
+    float best = 1e30f;
+    int best_loge = 0;
+    const int candidates[2] = {4, 3};
+    if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMalloc(&buf, words * 8) != hipSuccess ||
+        hipMalloc(&buf2, words * 8) != hipSuccess || hipMemset(buf, 0, words * 8) != hipSuccess ||
+        hipMemset(buf2, 0, words * 8) != hipSuccess) {
+        rc = fail(PH_ERR_DEVICE, "autotune set-up failed");
+        goto done;
+    }
+    if (o == Op::FwdMulInv) {
+        ones.assign(d->L * n, Shoup{1, 0});
+        op_dev.L = d->L;
+        op_dev.n = n;
+        for (uint32_t l = 0; l < d->L; ++l)
+            for (uint64_t i = 0; i < n; ++i) ones[l * n + i] = mk(1, d->tables[l]->q);
+        if (upload((void **)&op_dev.d, ones.data(), ones.size() * sizeof(Shoup)) != PH_OK) {
+            rc = PH_ERR_DEVICE;
+            goto done;
+        }
+        if (d->f64) {
+            onesv.assign(ones.size(), 1);
+            if (upload((void **)&op_dev.d_v, onesv.data(), onesv.size() * sizeof(uint64_t)) != PH_OK) {
+                rc = PH_ERR_DEVICE;
+                goto done;
+            }
+        }
+    }
+    for (int loge : candidates) {
+        if (loge == 3 && (d->log_n < 10 || d->log_n > kMaxSinglePassLogN)) continue;
+        g_tuning[d->log_n][op][f64 ? 1 : 0].store(loge);
+        // warm-up, then timed repetitions
+        for (int r = 0; r <= reps; ++r) {
+            if (r == 1) (void)hipEventRecord(e0, s);
+            rc = run_op(o, d->d_tabs, d->L, d->log_n, f64, buf, buf2, o == Op::FwdMulInv ? &op_dev : nullptr, count, s);
+            if (rc != PH_OK) goto done;
+        }
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(PH_ERR_DEVICE, "autotune timing failed");
+            goto done;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= reps;
+        if (ms < best) {
+            best = ms;
+            best_loge = loge;
+        }
+    }
+    g_tuning[d->log_n][op][f64 ? 1 : 0].store(best_loge);
+    if (chosen) *chosen = best_loge;
+    if (best_ms) *best_ms = best;
+done:
+    if (op_dev.d) (void)hipFree(op_dev.d);
+    if (op_dev.d_v) (void)hipFree(op_dev.d_v);
+    if (buf) (void)hipFree(buf);
+    if (buf2) (void)hipFree(buf2);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
 }
 
 }  // extern "C"

@@ -51,6 +51,21 @@ struct alignas(16) ShoupF {
     double wp;  // w / q rounded to binary64
 };
 
+// Pointer into the constant address space (read-only global memory).  Table
+// pointers read out of a DevTable are generic otherwise, and generic loads are
+// FLAT instructions that count against both memory counters (every LDS wait
+// then also waits for them) and never become scalar loads for wave-uniform
+// addresses.  Twiddle tables and operands are immutable while a kernel runs.
+template <class T>
+using gptr = const __attribute__((address_space(4))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T *p) {
+    return (gptr<T>)p;
+}
+
+__device__ __forceinline__ Shoup ld(gptr<Shoup> p) { return Shoup{p->w, p->wq}; }
+__device__ __forceinline__ ShoupF ld(gptr<ShoupF> p) { return ShoupF{p->w, p->wp}; }
+
 // Per-modulus device view of one UintNttTable.
 struct DevTable {
     uint64_t q, q2;
@@ -77,8 +92,8 @@ struct IntArith {
     using T = uint64_t;
     using Tw = Shoup;
 
-    __device__ __forceinline__ static const Tw *fwd(const DevTable &t) { return t.fwd; }
-    __device__ __forceinline__ static const Tw *inv(const DevTable &t) { return t.inv; }
+    __device__ __forceinline__ static gptr<Tw> fwd(const DevTable &t) { return as_global(t.fwd); }
+    __device__ __forceinline__ static gptr<Tw> inv(const DevTable &t) { return as_global(t.inv); }
 
     // Harvey forward butterfly, primitive.rs:225-228.
     __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
@@ -87,8 +102,10 @@ struct IntArith {
         a = u + v;
         b = u + t.q2 - v;
     }
+    static constexpr bool kReduceAtRoundStart = false;
+    __device__ __forceinline__ static void round_start(T &, const DevTable &) {}
     // Gentleman-Sande inverse butterfly, primitive.rs:262-265.
-    __device__ __forceinline__ static void gs(T &a, T &b, const Tw &w, const DevTable &t) {
+    __device__ __forceinline__ static void gs(T &a, T &b, const Tw &w, uint32_t, const DevTable &t) {
         uint64_t u = a, v = b;
         a = sub_if_ge(u + v, t.q2);
         b = shoup_lazy(w.w, w.wq, u + t.q2 - v, t.q);
@@ -126,10 +143,14 @@ struct F64Arith {
     using Tw = ShoupF;
     static constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
 
-    __device__ __forceinline__ static const Tw *fwd(const DevTable &t) { return t.fwd_f; }
-    __device__ __forceinline__ static const Tw *inv(const DevTable &t) { return t.inv_f; }
+    __device__ __forceinline__ static gptr<Tw> fwd(const DevTable &t) { return as_global(t.fwd_f); }
+    __device__ __forceinline__ static gptr<Tw> inv(const DevTable &t) { return as_global(t.inv_f); }
 
-    // x*w mod q in the centred range |r| <= 0.6251 q, for |x| <= 2^51.
+    // x*w mod q, centred.  Twiddles are stored centred (|w| <= q/2, |wp| < 1/2
+    // with |wp - w/q| <= d = 2^-54 + 2^-64), so |x*wp| < 2^51 for |x| <= 3.9 q
+    // and the C-rounding is exact; |x*w/q - k| <= 1/2 + 3.9 q d < 0.744, so
+    // |r| <= 0.744 q.  |h - k q| <= |r| + ulp(h)/2 < 2^53 keeps the last two
+    // steps exact.
     __device__ __forceinline__ static double mulmod(double x, double w, double wp, double q) {
         const double h = x * w;
         const double l = __fma_rn(x, w, -h);
@@ -142,17 +163,28 @@ struct F64Arith {
         const double k = __fma_rn(a, t.qinv, kRound) - kRound;
         return __fma_rn(-k, t.qd, a);
     }
+    // Forward register round: every coefficient is reduced once at the start
+    // (|x| <= q/2 + 1); each of the <= 4 stages then adds at most 0.744 q, so
+    // stage inputs stay below 2.74 q and round outputs below 3.48 q < 3.9 q.
+    static constexpr bool kReduceAtRoundStart = true;
+    __device__ __forceinline__ static void round_start(T &a, const DevTable &t) { a = reduce(a, t); }
     __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
-        const double u = reduce(a, t);                // |u| <= q/2 + 1
-        const double v = mulmod(b, w.w, w.wp, t.qd);  // |b| <= 1.126 q < 2^51
-        a = u + v;                                    // |a| <= 1.126 q
+        const double v = mulmod(b, w.w, w.wp, t.qd);
+        const double u = a;
+        a = u + v;
         b = u - v;
     }
-    __device__ __forceinline__ static void gs(T &a, T &b, const Tw &w, const DevTable &t) {
-        const double u = a, v = b;                        // |u|, |v| <= 0.6251 q
-        a = reduce(u + v, t);                             // <= q/2 + 1
-        b = mulmod(u - v, w.w, w.wp, t.qd);               // <= 0.6251 q
+    // Inverse stage p: sums are reduced on odd p only.  Entering an even stage
+    // every value is <= 0.744 q; its sums are <= 1.49 q and its differences,
+    // multiplied, <= 0.744 q.  The odd stage after it sees differences <= 2.98 q
+    // (multiplied to <= 0.744 q) and reduces its sums (<= q/2 + 1).
+    __device__ __forceinline__ static void gs(T &a, T &b, const Tw &w, uint32_t p, const DevTable &t) {
+        const double u = a, v = b;
+        const double s = u + v;
+        a = (p & 1u) ? reduce(s, t) : s;
+        b = mulmod(u - v, w.w, w.wp, t.qd);
     }
+    // Last stage, N^-1 folded in: inputs <= 2.98 q, outputs <= 0.744 q.
     __device__ __forceinline__ static void gs_last(T &a, T &b, const DevTable &t) {
         const double u = a, v = b;
         a = mulmod(u + v, t.inv_n_f, t.inv_n_p, t.qd);
@@ -162,7 +194,7 @@ struct F64Arith {
     __device__ __forceinline__ static T load(uint64_t v, const DevTable &t) {
         return reduce((double)v, t);
     }
-    // centred |x| <= q -> [0, q)
+    // centred |x| < q -> [0, q)
     __device__ __forceinline__ static uint64_t to_canon(double x, const DevTable &t) {
         const double r = x < 0.0 ? x + t.qd : x;
         return (uint64_t)r;
@@ -171,12 +203,16 @@ struct F64Arith {
         return to_canon(reduce(v, t), t);
     }
     __device__ __forceinline__ static uint64_t out_inv(T v, const DevTable &t) {
-        return to_canon(v, t);  // |v| <= 0.6251 q
+        return to_canon(v, t);  // |v| <= 0.744 q
     }
-    __device__ __forceinline__ static T mul_operand(T x, const ShoupF &w, const DevTable &t) {
-        return mulmod(x, w.w, w.wp, t.qd);
+    // Forward output (|x| <= 3.48 q, reduced here to q/2 + 1) times a canonical
+    // operand w < q with w/q formed on the fly (|wp - w/q| <= 2^-52):
+    // |x*w/q - k| <= 1/2 + 2^-3, so the product is <= 0.63 q.
+    __device__ __forceinline__ static T mul_operand(T x, uint64_t w, const DevTable &t) {
+        const double wd = (double)w;
+        return mulmod(reduce(x, t), wd, wd * t.qinv, t.qd);
     }
-    // x*y mod q for |x|, |y| <= 1.126 q: y/q is formed on the fly (two more roundings).
+    // x*y mod q: both reduced to q/2 + 1, y/q formed on the fly (<= 0.57 q out).
     __device__ __forceinline__ static T mul_pair(T x, T y, const DevTable &t) {
         const double yr = reduce(y, t);
         return mulmod(reduce(x, t), yr, yr * t.qinv, t.qd);
@@ -191,14 +227,18 @@ struct F64Arith {
 // chunk c at stage S.
 template <class A, int R>
 __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t c,
-                                         const typename A::Tw *__restrict__ tw, const DevTable &t) {
+                                         gptr<typename A::Tw> tw, const DevTable &t) {
+    if constexpr (A::kReduceAtRoundStart) {
+#pragma unroll
+        for (int j = 0; j < (1 << R); ++j) A::round_start(x[j], t);
+    }
 #pragma unroll
     for (int u = 0; u < R; ++u) {
         const int half = 1 << (R - u - 1);
         const uint32_t base = (1u << (S + u)) + (c << u);
 #pragma unroll
         for (int sub = 0; sub < (1 << u); ++sub) {
-            const typename A::Tw w = tw[base + sub];
+            const typename A::Tw w = ld(tw + base + sub);
 #pragma unroll
             for (int jj = 0; jj < half; ++jj) {
                 const int j0 = sub * 2 * half + jj;
@@ -213,7 +253,7 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
 // transform's final stage (gap N/2).
 template <class A, int R, bool LAST>
 __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t c, uint32_t n,
-                                         const typename A::Tw *__restrict__ itw, const DevTable &t) {
+                                         gptr<typename A::Tw> itw, const DevTable &t) {
 #pragma unroll
     for (int u = 0; u < R; ++u) {
         const int half = 1 << u;
@@ -224,11 +264,11 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
             const uint32_t base = 1u + n - (n >> (P + u)) + (c << (R - u - 1));
 #pragma unroll
             for (int sub = 0; sub < (1 << (R - u - 1)); ++sub) {
-                const typename A::Tw w = itw[base + sub];
+                const typename A::Tw w = ld(itw + base + sub);
 #pragma unroll
                 for (int jj = 0; jj < half; ++jj) {
                     const int j0 = sub * 2 * half + jj;
-                    A::gs(x[j0], x[j0 + half], w, t);
+                    A::gs(x[j0], x[j0 + half], w, P + u, t);
                 }
             }
         }

@@ -108,9 +108,11 @@ struct FwdRounds {
         }
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m) {
-            const uint32_t gi = t + ((uint32_t)m << LOGT);
-            const uint32_t c = (b << S) + (gi >> (LOGC - S - R));
-            ct_round<A, R>(x + (m << R), st0 + S, c, A::fwd(tab), tab);
+            constexpr int OB = LOGC - S - R;
+            uint32_t cl;
+            if constexpr (OB >= LOGT) cl = (uint32_t)m >> (OB - LOGT);  // wave-uniform
+            else cl = (t + ((uint32_t)m << LOGT)) >> OB;
+            ct_round<A, R>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
         }
         if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1>::run(x, lds, t, st0, b, tab);
     }
@@ -154,19 +156,36 @@ struct InvRounds {
         constexpr bool last_here = LAST && (I + 1 == G::NR);
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m) {
-            const uint32_t gi = t + ((uint32_t)m << LOGT);
-            const uint32_t c = (b << (LOGC - P - R)) + (gi >> P);
-            gs_round<A, R, last_here>(x + (m << R), P, c, n, A::inv(tab), tab);
+            uint32_t cl;
+            if constexpr (P >= LOGT) cl = (uint32_t)m >> (P - LOGT);  // wave-uniform
+            else cl = (t + ((uint32_t)m << LOGT)) >> P;
+            gs_round<A, R, last_here>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n, A::inv(tab), tab);
         }
         if constexpr (I + 1 < G::NR) InvRounds<A, LOGC, LOGT, I + 1, LAST>::run(x, lds, t, n, b, tab);
     }
 };
 
 __device__ __forceinline__ const Shoup *operand_of(const KArgs &a, const IntArith *) { return a.operand; }
-__device__ __forceinline__ const ShoupF *operand_of(const KArgs &a, const F64Arith *) { return a.operand_f; }
+__device__ __forceinline__ const uint64_t *operand_of(const KArgs &a, const F64Arith *) { return a.operand_v; }
+__device__ __forceinline__ Shoup ld_operand(gptr<Shoup> p, const IntArith *) { return ld(p); }
+__device__ __forceinline__ uint64_t ld_operand(gptr<uint64_t> p, const F64Arith *) { return *p; }
+
+// Waves per SIMD the launch bounds ask the register allocator for: as many as
+// the LDS footprint lets resident (at most 8), halved when a second operand's
+// transform is held in registers too.
+template <int LOGC, int LOGT, int MODE>
+struct Occupancy {
+    static constexpr int lds_bytes = (Geo<LOGC, LOGT>::LDS) * 8;
+    static constexpr int wgs_lds = (160 * 1024) / lds_bytes;
+    static constexpr int wgs = wgs_lds < 1 ? 1 : (wgs_lds > 32 ? 32 : wgs_lds);
+    static constexpr int waves_wg = (1 << LOGT) < 64 ? 1 : (1 << LOGT) / 64;
+    static constexpr int per_simd_raw = (wgs * waves_wg) / 4;
+    static constexpr int per_simd = per_simd_raw < 1 ? 1 : (per_simd_raw > 8 ? 8 : per_simd_raw);
+    static constexpr int value = (MODE & M_MUL_POLY) ? (per_simd / 2 < 1 ? 1 : per_simd / 2) : per_simd;
+};
 
 template <class A, int LOGC, int LOGT, int MODE>
-__global__ __launch_bounds__(1 << LOGT) void chunk_kernel(KArgs a) {
+__global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) void chunk_kernel(KArgs a) {
     using G = Geo<LOGC, LOGT>;
     using T = typename A::T;
     constexpr int LE = G::LOGE;
@@ -198,11 +217,15 @@ __global__ __launch_bounds__(1 << LOGT) void chunk_kernel(KArgs a) {
             FwdRounds<A, LOGC, LOGT, 0>::load_global(data, x, t, tab);
             FwdRounds<A, LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab);
         }
-        // after the last forward round thread t holds local slots t*E .. t*E+E-1
+        // after the last forward round thread t holds local slots t*E .. t*E+E-1.
+        // Keep the scheduler from hoisting the next phase's loads into this one:
+        // their registers would be live across the whole round and spill.
+        __builtin_amdgcn_sched_barrier(0);
         if constexpr ((MODE & M_MUL_SHOUP) != 0) {
-            const auto *op = operand_of(a, (const A *)nullptr) + (uint64_t)limb * n + (uint64_t)b * G::C + ((uint64_t)t << LE);
+            const auto op = as_global(operand_of(a, (const A *)nullptr)) + (uint64_t)limb * n + (uint64_t)b * G::C +
+                            ((uint64_t)t << LE);
 #pragma unroll
-            for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], op[j], tab);
+            for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], ld_operand(op + j, (const A *)nullptr), tab);
         }
         if constexpr ((MODE & M_INV) == 0) {
             uint64_t *dst = data + ((uint64_t)t << LE);
@@ -221,6 +244,7 @@ __global__ __launch_bounds__(1 << LOGT) void chunk_kernel(KArgs a) {
 
     if constexpr ((MODE & M_INV) != 0) {
         constexpr bool LAST = (MODE & M_INV_LAST) != 0;
+        __builtin_amdgcn_sched_barrier(0);
         InvRounds<A, LOGC, LOGT, 0, LAST>::run(x, lds, t, n, b, tab);
         InvRounds<A, LOGC, LOGT, G::NR - 1, LAST>::store_global(data, x, t, tab);
     }
@@ -244,9 +268,9 @@ __global__ __launch_bounds__(256) void column_kernel(KArgs a) {
 #pragma unroll
     for (int i = 0; i < (1 << LOGR); ++i) x[i] = src[(uint64_t)i << logc];
     if constexpr ((MODE & M_FWD) != 0) {
-        ct_round<IntArith, LOGR>(x, 0, 0, tab.fwd, tab);
+        ct_round<IntArith, LOGR>(x, 0, 0, IntArith::fwd(tab), tab);
     } else {
-        gs_round<IntArith, LOGR, true>(x, logc, 0, n, tab.inv, tab);
+        gs_round<IntArith, LOGR, true>(x, logc, 0, n, IntArith::inv(tab), tab);
     }
 #pragma unroll
     for (int i = 0; i < (1 << LOGR); ++i) dst[(uint64_t)i << logc] = x[i];
@@ -315,10 +339,11 @@ static hipError_t launch_column(const KArgs &a, uint32_t log_n, uint64_t count, 
     return hipGetLastError();
 }
 
-// Single-pass chunk launch for log_n = LOGN <= 13 with the mode chosen at run time.
-template <int LOGN>
-static hipError_t single_pass(Op op, KArgs a, uint64_t count, bool f64, hipStream_t s) {
-    constexpr int LOGT = LOGN > 4 ? LOGN - 4 : 0;
+// Single-pass chunk launch for log_n = LOGN <= 13, E = 2^LOGE coefficients per
+// thread, the mode chosen at run time.
+template <int LOGN, int LOGE>
+static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, bool f64, hipStream_t s) {
+    constexpr int LOGT = LOGN > LOGE ? LOGN - LOGE : 0;
     a.blocks_per_poly = 1;
     if constexpr (LOGN >= 6) {
         if (f64) {
@@ -343,6 +368,16 @@ static hipError_t single_pass(Op op, KArgs a, uint64_t count, bool f64, hipStrea
             return launch_chunk<IntArith, LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
     }
     return hipErrorInvalidValue;
+}
+
+// Coefficients per thread: 16 by default; 8 (twice the waves, one more LDS
+// round) for 2^10 <= N <= 2^13 when the tuning asks for it.
+template <int LOGN>
+static hipError_t single_pass(Op op, KArgs a, uint64_t count, bool f64, int loge, hipStream_t s) {
+    if constexpr (LOGN >= 10) {
+        if (loge == 3) return single_pass_e<LOGN, 3>(op, a, count, f64, s);
+    }
+    return single_pass_e<LOGN, 4>(op, a, count, f64, s);
 }
 
 // Two-pass schedule for log_n = 12 + LOGR: columns of 2^LOGR, chunks of 4096.
@@ -397,21 +432,21 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
 bool needs_scratch(Op op, uint32_t log_n) { return op == Op::PolyMul && log_n > kMaxSinglePassLogN; }
 
 hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, bool f64,
-                            hipStream_t s) {
+                            int loge, hipStream_t s) {
     switch (log_n) {
-        case 1: return single_pass<1>(op, a, count, f64, s);
-        case 2: return single_pass<2>(op, a, count, f64, s);
-        case 3: return single_pass<3>(op, a, count, f64, s);
-        case 4: return single_pass<4>(op, a, count, f64, s);
-        case 5: return single_pass<5>(op, a, count, f64, s);
-        case 6: return single_pass<6>(op, a, count, f64, s);
-        case 7: return single_pass<7>(op, a, count, f64, s);
-        case 8: return single_pass<8>(op, a, count, f64, s);
-        case 9: return single_pass<9>(op, a, count, f64, s);
-        case 10: return single_pass<10>(op, a, count, f64, s);
-        case 11: return single_pass<11>(op, a, count, f64, s);
-        case 12: return single_pass<12>(op, a, count, f64, s);
-        case 13: return single_pass<13>(op, a, count, f64, s);
+        case 1: return single_pass<1>(op, a, count, f64, loge, s);
+        case 2: return single_pass<2>(op, a, count, f64, loge, s);
+        case 3: return single_pass<3>(op, a, count, f64, loge, s);
+        case 4: return single_pass<4>(op, a, count, f64, loge, s);
+        case 5: return single_pass<5>(op, a, count, f64, loge, s);
+        case 6: return single_pass<6>(op, a, count, f64, loge, s);
+        case 7: return single_pass<7>(op, a, count, f64, loge, s);
+        case 8: return single_pass<8>(op, a, count, f64, loge, s);
+        case 9: return single_pass<9>(op, a, count, f64, loge, s);
+        case 10: return single_pass<10>(op, a, count, f64, loge, s);
+        case 11: return single_pass<11>(op, a, count, f64, loge, s);
+        case 12: return single_pass<12>(op, a, count, f64, loge, s);
+        case 13: return single_pass<13>(op, a, count, f64, loge, s);
         case 14: return two_pass<2>(op, a, count, scratch, s);
         case 15: return two_pass<3>(op, a, count, scratch, s);
         case 16: return two_pass<4>(op, a, count, scratch, s);

@@ -249,7 +249,7 @@ def test_pointwise_device():
 @pytest.mark.parametrize("q", [1125899906826241, 1125899906260993, 2147473409])
 def test_f64_extreme_operands(q):
     """Largest magnitudes the binary64 path can meet: every input and operand q-1 (or 4q-1)."""
-    log_n = 12
+    log_n = 12 if q > 1 << 40 else 10
     n = 1 << log_n
     tab = pa.UintCrtNttTable.new(log_n, [q])
     assert tab.arith() == "f64"
@@ -271,3 +271,61 @@ def test_f64_extreme_operands(q):
         pa.synchronize()
         want = ot.inverse_transform(O.pointwise_mul(ot.transform(b), ot.transform(np.full(n, q - 1, dtype=np.uint64)), q))
         np.testing.assert_array_equal(da.download(), want)
+
+
+@pytest.mark.parametrize("log_n", [10, 11, 12, 13])
+@pytest.mark.parametrize("bits", [50, 59])
+def test_launch_shapes_agree(log_n, bits):
+    """Every autotuner launch shape gives the oracle's outputs."""
+    from primus_amd import tuning
+
+    n = 1 << log_n
+    L, count = 2, 6
+    moduli = pa.ntt_primes_below(bits, 2 * n, L)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    arith = tab.arith()
+    tables = [O.Table(log_n, q) for q in moduli]
+    rng = np.random.default_rng(log_n + bits)
+    a = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    b = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    opv = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in moduli])
+    op = pa.Operand(tab, opv)
+    try:
+        for loge in (3, 4):
+            for name in ("forward", "inverse", "fwd_mul_inv", "negacyclic_mul", "forward_lazy"):
+                tuning.set_shape(log_n, name, arith, loge)
+                tuning.set_shape(log_n, name, "int", loge)
+            d = pa.DeviceBuffer.from_array(a)
+            pa.fwd_mul_inv_device(tab, d.ptr, count, op)
+            pa.synchronize()
+            got = d.download((count, n))
+            for i in range(count):
+                t = tables[i % L]
+                np.testing.assert_array_equal(got[i], t.inverse_transform(O.pointwise_mul(t.transform(a[i]),
+                                                                                        opv[i % L], t.q)))
+            d.upload(a)
+            db = pa.DeviceBuffer.from_array(b)
+            pa.negacyclic_mul_device(tab, d.ptr, db.ptr, count)
+            pa.synchronize()
+            got = d.download((count, n))
+            for i in range(count):
+                t = tables[i % L]
+                np.testing.assert_array_equal(got[i], t.inverse_transform(O.pointwise_mul(
+                    t.transform(a[i]), t.transform(b[i]), t.q)))
+            d.upload(a)
+            pa.transform_device(tab, d.ptr, count, lazy=True)
+            pa.synchronize()
+            got = d.download((count, n))
+            for i in range(count):
+                np.testing.assert_array_equal(got[i], tables[i % L].lazy_transform(a[i]))
+            d.upload(a)
+            pa.transform_device(tab, d.ptr, count)
+            pa.inverse_transform_device(tab, d.ptr, count)
+            pa.synchronize()
+            np.testing.assert_array_equal(d.download((count, n)), a)
+        loge, ms = tuning.autotune(tab, "fwd_mul_inv", 64, reps=3)
+        assert loge in (3, 4) and ms > 0
+    finally:
+        for name in ("forward", "inverse", "fwd_mul_inv", "negacyclic_mul", "forward_lazy"):
+            tuning.set_shape(log_n, name, "f64", 0)
+            tuning.set_shape(log_n, name, "int", 0)
