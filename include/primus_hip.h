@@ -173,6 +173,32 @@ int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *table, const uint64_t *d_a
 int ph_dcrt_pointwise_mul_add_device(const ph_dcrt_table *table, uint64_t *d_acc, const uint64_t *d_a,
                                      const uint64_t *d_b, size_t count, void *stream);
 
+/* ---- external product (RLWE x RGSW, k = 1, one modulus) ---- */
+
+typedef struct ph_ext_key ph_ext_key; /* device-resident RGSW/GGSW key in NTT form */
+
+/* The key of CrtGlwe::mul_dcrt_ggsw_inplace (primus_lattice/src/glwe/crt.rs:202-229)
+ * with the gadget basis BigUintApproxSignedBasis::new(q, log_basis,
+ * Some(decompose_length), rns) for a one-limb q
+ * (primus_decompose/src/big_integer/basis.rs:34-195).  key_ntt holds the GGSW
+ * (primus_lattice/src/ggsw/dcrt.rs:10-21) as [2 GLev][l GLWE][2 polys][N], NTT
+ * form, canonical.  The table must outlive the key; 2^6 <= N <= 2^12. */
+int ph_ext_key_new(const ph_ntt_table *table, uint32_t log_basis, uint32_t decompose_length,
+                   const uint64_t *key_ntt, ph_ext_key **out);
+void ph_ext_key_free(ph_ext_key *key);
+/* `count` independent external products: d_in [count][2][N] RLWE ciphertexts
+ * (a, b) in coefficient form, d_out [count][2][N].  Per product and input
+ * polynomial: approximate signed decomposition into l digits
+ * (primus_decompose/src/big_integer/common.rs:150-201, primus_rns/src/base.rs:201-236),
+ * forward transform of each digit, multiply-accumulate against the key
+ * (DcrtGlwe::add_dcrt_glev_mul_crt_poly_assign, primus_lattice/src/glwe/dcrt.rs:170-247);
+ * then, unless ntt_output, the inverse transform of both accumulators.  One
+ * fused launch. */
+int ph_ext_product_device(const ph_ext_key *key, const uint64_t *d_in, uint64_t *d_out, size_t count,
+                          int ntt_output, void *stream);
+/* The same on host memory, synchronous. */
+int ph_ext_product(const ph_ext_key *key, const uint64_t *in, uint64_t *out, size_t count, int ntt_output);
+
 /* ---- autotuning (no reference counterpart: launch-shape selection for the GPU) ---- */
 
 /* Operation codes for the tuning table. */

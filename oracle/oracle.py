@@ -68,6 +68,9 @@ def lib():
             ("po_schoolbook_negacyclic", None, [_p64, _p64, _p64, ctypes.c_size_t, _u64]),
             ("po_fwd_mul_inv_batch", None,
              [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32, _p64, ctypes.c_size_t, _p64, _p64, ctypes.c_int]),
+            ("po_basis_new", ctypes.c_int, [_u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+            ("po_decompose", None, [ctypes.c_void_p, _p64, ctypes.c_size_t, _p64]),
+            ("po_ext_product", None, [ctypes.c_void_p, ctypes.c_void_p, _p64, _p64, _p64, ctypes.c_int]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -221,3 +224,36 @@ def fwd_mul_inv_batch(tables, polys, operand, operand_q, threads=1):
     count = polys.size // n
     lib().po_fwd_mul_inv_batch(arr, L, ptr(polys), count, ptr(operand), ptr(operand_q), threads)
     return polys
+
+
+class Basis(ctypes.Structure):
+    """po_basis: BigUintApproxSignedBasis for one u64 limb
+    (primus_decompose/src/big_integer/basis.rs:34-195)."""
+
+    _fields_ = [("q", ctypes.c_uint64), ("log_basis", ctypes.c_uint32), ("l", ctypes.c_uint32),
+                ("drop_bits", ctypes.c_uint32), ("modulus_bits", ctypes.c_uint32), ("basis", ctypes.c_uint64),
+                ("basis_minus_one", ctypes.c_uint64), ("carry_mask", ctypes.c_uint64), ("split", ctypes.c_uint64),
+                ("has_split", ctypes.c_int), ("pow2_sub_modulus", ctypes.c_uint64)]
+
+    @classmethod
+    def new(cls, q, log_basis, l):
+        b = cls()
+        st = lib().po_basis_new(q, log_basis, l, ctypes.byref(b))
+        if st:
+            raise OracleError(st)
+        return b
+
+    def decompose(self, values):
+        v = np.ascontiguousarray(values, dtype=np.uint64)
+        out = np.zeros((self.l, v.size), dtype=np.uint64)
+        lib().po_decompose(ctypes.byref(self), ptr(v), v.size, ptr(out))
+        return out
+
+
+def ext_product(table, basis, key, rlwe, ntt_output=False):
+    """External product of one RLWE ciphertext [2][N] with a GGSW key [2][l][2][N] (NTT form)."""
+    key = np.ascontiguousarray(key, dtype=np.uint64)
+    x = np.ascontiguousarray(rlwe, dtype=np.uint64)
+    out = np.zeros(2 * table.n, dtype=np.uint64)
+    lib().po_ext_product(table.handle, ctypes.byref(basis), ptr(key), ptr(x), ptr(out), int(bool(ntt_output)))
+    return out.reshape(2, table.n)

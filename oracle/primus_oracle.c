@@ -481,3 +481,91 @@ void po_fwd_mul_inv_batch(po_table *const *tables, uint32_t L, uint64_t *polys, 
     }
     for (int k = 0; k < started; ++k) pthread_join(tid[k], NULL);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Gadget decomposition and the external product                             */
+/* ------------------------------------------------------------------------- */
+
+/* BigUintApproxSignedBasis::new for a one-limb modulus.
+ * primus_decompose/src/big_integer/basis.rs:34-195 */
+int po_basis_new(uint64_t q, uint32_t log_basis, uint32_t l, po_basis *out) {
+    if (q < 2 || log_basis == 0 || log_basis >= 64 || l == 0) return PO_ERR_INVALID;
+    uint32_t bits = 64 - (uint32_t)__builtin_clzll(q);
+    uint32_t full = bits / log_basis;
+    if (full < l) return PO_ERR_INVALID; /* basis.rs:57: decompose_length >= reverse_len */
+    po_basis b;
+    memset(&b, 0, sizeof b);
+    b.q = q;
+    b.log_basis = log_basis;
+    b.l = l;
+    b.modulus_bits = bits;
+    b.drop_bits = bits - l * log_basis;
+    b.basis = 1ull << log_basis;
+    b.basis_minus_one = b.basis - 1;
+    b.carry_mask = log_basis == 1 ? 2ull : ((1ull << log_basis) | (1ull << (log_basis - 1))); /* :75-79 */
+    /* split_value (:81-128) */
+    u128 value = 0;
+    if (log_basis == 1) {
+        if (b.drop_bits == 0) {
+            b.has_split = 0;
+        } else {
+            for (uint32_t i = 0; i < l; ++i) value = (value << 1) | 1;
+            value = (value << 1) | 1;
+            value <<= (b.drop_bits - 1);
+            b.has_split = value < q;
+        }
+    } else {
+        for (uint32_t i = 0; i < l; ++i) value = (value << log_basis) | (b.basis_minus_one >> 1);
+        if (b.drop_bits > 0) {
+            value = (value << 1) | 1;
+            value <<= (b.drop_bits - 1);
+        } else {
+            value += 1;
+        }
+        b.has_split = value < q;
+    }
+    b.split = (uint64_t)value;
+    /* next_pow_of_2_sub_modulus = (2^bits - 1) - (q - 1) (:134-144) */
+    b.pow2_sub_modulus = (uint64_t)((((u128)1) << bits) - q);
+    *out = b;
+    return PO_OK;
+}
+
+/* basis.rs:307-331, common.rs:141-152,184-201, primus_rns/src/base.rs:201-236 */
+void po_decompose(const po_basis *b, const uint64_t *values, size_t n, uint64_t *residues) {
+    const uint64_t half = (b->basis + 1) / 2;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t adj = values[i];
+        if (b->has_split && adj >= b->split) adj += b->pow2_sub_modulus;
+        int carry = b->drop_bits > 0 ? (int)((adj >> (b->drop_bits - 1)) & 1) : 0;
+        for (uint32_t d = 0; d < b->l; ++d) {
+            uint32_t shift = b->drop_bits + d * b->log_basis;
+            uint64_t temp = ((adj >> shift) & b->basis_minus_one) + (uint64_t)carry;
+            carry = (temp & b->carry_mask) != 0;
+            uint64_t digit = temp & b->basis_minus_one;
+            uint64_t r;
+            if (b->basis != 2) r = digit < half ? digit : (b->q - b->basis) + digit;
+            else r = digit;
+            residues[(size_t)d * n + i] = r;
+        }
+    }
+}
+
+void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, const uint64_t *in,
+                    uint64_t *out, int ntt_output) {
+    const size_t n = t->n;
+    uint64_t *res = (uint64_t *)malloc(b->l * n * 8);
+    memset(out, 0, 2 * n * 8);
+    for (int p = 0; p < 2; ++p) {
+        po_decompose(b, in + p * n, n, res);
+        for (uint32_t d = 0; d < b->l; ++d) {
+            uint64_t *x = res + (size_t)d * n;
+            po_transform(t, x); /* table.transform_slice(multi_residues), dcrt.rs:238 */
+            for (int j = 0; j < 2; ++j) /* add_dcrt_glwe_mul_dcrt_polynomial_assign, dcrt.rs:101-119 */
+                po_pointwise_mul_add(out + j * n, x, key + (((size_t)p * b->l + d) * 2 + j) * n, n, t->q);
+        }
+    }
+    if (!ntt_output)
+        for (int j = 0; j < 2; ++j) po_inverse_transform(t, out + j * n);
+    free(res);
+}

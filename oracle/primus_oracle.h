@@ -88,6 +88,31 @@ void po_schoolbook_negacyclic(const uint64_t *a, const uint64_t *b, uint64_t *c,
 void po_fwd_mul_inv_batch(po_table *const *tables, uint32_t L, uint64_t *polys, size_t count,
                           const uint64_t *operand, const uint64_t *operand_q, int threads);
 
+
+/* ---- Approximate signed gadget basis for one u64 limb ----
+ * BigUintApproxSignedBasis::new(q, log_basis, Some(l), rns) with a single
+ * modulus (primus_decompose/src/big_integer/basis.rs:34-195). */
+typedef struct {
+    uint64_t q;
+    uint32_t log_basis, l, drop_bits, modulus_bits;
+    uint64_t basis, basis_minus_one, carry_mask;
+    uint64_t split;            /* valid when has_split */
+    int has_split;
+    uint64_t pow2_sub_modulus; /* 2^modulus_bits - q */
+} po_basis;
+int po_basis_new(uint64_t q, uint32_t log_basis, uint32_t l, po_basis *out);
+/* init_value_carry_slice_inplace + l x (unsigned_decompose_slice_inplace ->
+ * wrapping_decompose_small_values_inplace): residues[l][n] in [0, q).
+ * basis.rs:307-331, common.rs:184-201, primus_rns/src/base.rs:201-236 */
+void po_decompose(const po_basis *b, const uint64_t *values, size_t n, uint64_t *residues);
+/* External product RLWE x RGSW (k = 1), single modulus:
+ * CrtGlwe::mul_dcrt_ggsw_inplace (primus_lattice/src/glwe/crt.rs:202-229) ->
+ * DcrtGlwe::add_dcrt_glev_mul_crt_poly_assign (primus_lattice/src/glwe/dcrt.rs:170-247),
+ * then the caller's inverse transform when ntt_output == 0.
+ * in: [2][N] coefficients < q; key: [2][l][2][N] NTT form < q; out: [2][N]. */
+void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, const uint64_t *in,
+                    uint64_t *out, int ntt_output);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@
 
 #include "../../include/primus_hip.h"
 #include "arith.hpp"
+#include "fhe_launch.hpp"
 #include "ntt_launch.hpp"
 
 using namespace ph;
@@ -600,6 +601,128 @@ int ph_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes) {
 
 int ph_stream_synchronize(void *stream) {
     PH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return PH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// external product
+// ---------------------------------------------------------------------------
+
+struct ph_ext_key {
+    const ph_ntt_table *table;  // not owned
+    ExtArgs proto;              // gadget constants, table and key pointers
+    Shoup *d_key = nullptr;
+    ShoupF *d_key_f = nullptr;
+    bool f64 = false;
+};
+
+int ph_ext_key_new(const ph_ntt_table *t, uint32_t log_basis, uint32_t decompose_length, const uint64_t *key_ntt,
+                   ph_ext_key **out) {
+    if (!t || !key_ntt || !out) return fail(PH_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (t->log_n < kExtMinLogN || t->log_n > kExtMaxLogN)
+        return fail(PH_ERR_UNSUPPORTED, "external product supports 2^6 <= N <= 2^12");
+    const uint64_t q = t->q, n = t->n;
+    // BigUintApproxSignedBasis::new(q, log_basis, Some(l)) for one limb
+    // (primus_decompose/src/big_integer/basis.rs:34-195)
+    const uint32_t bits = 64 - __builtin_clzll(q);
+    if (log_basis == 0 || log_basis >= 32 || decompose_length == 0 || bits / log_basis < decompose_length)
+        return fail(PH_ERR_INVALID, "gadget basis does not fit the modulus");
+    ExtArgs g{};
+    g.l = decompose_length;
+    g.log_basis = log_basis;
+    g.drop_bits = bits - decompose_length * log_basis;
+    const uint64_t basis = 1ull << log_basis;
+    g.basis_minus_one = basis - 1;
+    g.carry_mask = log_basis == 1 ? 2ull : (basis | (basis >> 1));
+    g.half = (basis + 1) / 2;
+    g.basis_is_two = basis == 2;
+    g.q_minus_basis = q - basis;
+    u128 split = 0;
+    bool has_split;
+    if (log_basis == 1) {
+        if (g.drop_bits == 0) {
+            has_split = false;
+        } else {
+            for (uint32_t i = 0; i <= decompose_length; ++i) split = (split << 1) | 1;
+            split <<= (g.drop_bits - 1);
+            has_split = split < q;
+        }
+    } else {
+        for (uint32_t i = 0; i < decompose_length; ++i) split = (split << log_basis) | (g.basis_minus_one >> 1);
+        if (g.drop_bits > 0) split = ((split << 1) | 1) << (g.drop_bits - 1);
+        else split += 1;
+        has_split = split < q;
+    }
+    g.has_split = has_split;
+    g.split = (uint64_t)split;
+    g.pow2_sub_q = (uint64_t)((((u128)1) << bits) - q);
+    g.tab = t->d_tab;
+
+    // key rows [2][l][2][N] in NTT form, permuted so that slot t*E + j of a row
+    // sits at j*(N/E) + t (the register layout of the kernel's last round)
+    const uint64_t rows = 4ull * decompose_length, E = 1ull << kExtLogE, T = n / E;
+    std::vector<Shoup> hk(rows * n);
+    std::vector<ShoupF> hf;
+    const bool f64 = t->f64;
+    if (f64) hf.resize(rows * n);
+    for (uint64_t r = 0; r < rows; ++r)
+        for (uint64_t k = 0; k < n; ++k) {
+            const uint64_t v = key_ntt[r * n + k];
+            if (v >= q) return fail(PH_ERR_INVALID, "key values must be canonical (< q)");
+            const uint64_t pos = r * n + (k % E) * T + k / E;
+            hk[pos] = mk(v, q);
+            if (f64) hf[pos] = mkf(hk[pos], q);
+        }
+    ph_ext_key *key = new (std::nothrow) ph_ext_key();
+    if (!key) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    key->table = t;
+    key->f64 = f64;
+    int rc = upload((void **)&key->d_key, hk.data(), hk.size() * sizeof(Shoup));
+    if (rc == PH_OK && f64) rc = upload((void **)&key->d_key_f, hf.data(), hf.size() * sizeof(ShoupF));
+    if (rc != PH_OK) {
+        ph_ext_key_free(key);
+        return rc;
+    }
+    g.key = key->d_key;
+    g.key_f = key->d_key_f;
+    key->proto = g;
+    *out = key;
+    return PH_OK;
+}
+
+void ph_ext_key_free(ph_ext_key *key) {
+    if (!key) return;
+    if (key->d_key) (void)hipFree(key->d_key);
+    if (key->d_key_f) (void)hipFree(key->d_key_f);
+    delete key;
+}
+
+int ph_ext_product_device(const ph_ext_key *key, const uint64_t *d_in, uint64_t *d_out, size_t count, int ntt_output,
+                          void *stream) {
+    if (!key || ((!d_in || !d_out) && count)) return fail(PH_ERR_INVALID, "null argument");
+    ExtArgs g = key->proto;
+    g.in = d_in;
+    g.out = d_out;
+    g.ntt_output = ntt_output ? 1u : 0u;
+    hipError_t e = launch_ext_product(g, key->table->log_n, key->f64, count, (hipStream_t)stream);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "external product kernel");
+}
+
+int ph_ext_product(const ph_ext_key *key, const uint64_t *in, uint64_t *out, size_t count, int ntt_output) {
+    if (!key || ((!in || !out) && count)) return fail(PH_ERR_INVALID, "null argument");
+    if (count == 0) return PH_OK;
+    const size_t words = count * 2 * key->table->n;
+    void *buf;
+    hipStream_t s;
+    int rc = staging(2 * words * 8, &buf, &s);
+    if (rc != PH_OK) return rc;
+    uint64_t *d_in = (uint64_t *)buf, *d_out = d_in + words;
+    PH_HIP(hipMemcpyAsync(d_in, in, words * 8, hipMemcpyHostToDevice, s));
+    rc = ph_ext_product_device(key, d_in, d_out, count, ntt_output, s);
+    if (rc != PH_OK) return rc;
+    PH_HIP(hipMemcpyAsync(out, d_out, words * 8, hipMemcpyDeviceToHost, s));
+    PH_HIP(hipStreamSynchronize(s));
     return PH_OK;
 }
 

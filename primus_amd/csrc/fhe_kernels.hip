@@ -1,0 +1,159 @@
+// fhe_kernels.hip -- fused gfx950 kernels for the two batched callers of the
+// NTT on the reference's hot path:
+//
+// * ext_product_kernel: RLWE x RGSW external product (k = 1) with the
+//   approximate signed gadget decomposition, one workgroup per product:
+//   decompose each input polynomial into l digit polynomials, forward-transform
+//   each digit in LDS, multiply-accumulate it against the resident key rows,
+//   inverse-transform the two accumulators.  Follows
+//   CrtGlwe::mul_dcrt_ggsw_inplace (primus_lattice/src/glwe/crt.rs:202-229) ->
+//   DcrtGlwe::add_dcrt_glev_mul_crt_poly_assign (primus_lattice/src/glwe/dcrt.rs:170-247).
+// * rlwe_encrypt_kernel: persistent, uneven batch of RLWE encryptions
+//   b = NTT(e + Delta*m) + a (.) NTT(s) over mixed ring degrees and limb
+//   counts in one launch.  Follows NttRlweSecretKey::encrypt_inplace_with_embedding
+//   (primus_fhe_core/src/secret_key/rlwe.rs:227-268).
+#include "fhe_launch.hpp"
+#include "ntt_rounds.hpp"
+
+namespace ph {
+
+// ---------------------------------------------------------------------------
+// external product
+// ---------------------------------------------------------------------------
+
+// Digit `d` of a coefficient, as the residue the reference transforms:
+// unsigned_decompose (primus_decompose/src/big_integer/common.rs:141-152) then
+// wrapping_decompose_small_values (primus_rns/src/base.rs:201-236).  `carry`
+// holds the carry into this digit and receives the carry out of it.
+__device__ __forceinline__ uint64_t gadget_digit(uint64_t adj, uint32_t d, uint32_t &carry, const ExtArgs &g) {
+    const uint32_t shift = g.drop_bits + d * g.log_basis;
+    const uint64_t temp = ((adj >> shift) & g.basis_minus_one) + carry;
+    carry = (temp & g.carry_mask) != 0;
+    const uint64_t digit = temp & g.basis_minus_one;
+    if (g.basis_is_two) return digit;
+    return digit < g.half ? digit : g.q_minus_basis + digit;
+}
+
+// acc += x * key (mod q), one term of DcrtPolynomial::add_mul_assign
+// (primus_poly/src/dcrt/mod.rs:94-114); canonical for IntArith, centred and
+// reduced every third term for F64Arith (|acc| <= q/2 + 1 + 3 * 0.744 q).
+__device__ __forceinline__ void ext_mac(uint64_t &acc, uint64_t x, gptr<Shoup> key, const DevTable &t, bool) {
+    acc = add_mod(acc, shoup_full(key->w, key->wq, x, t.q), t.q);
+}
+__device__ __forceinline__ void ext_mac(double &acc, double x, gptr<ShoupF> key, const DevTable &t, bool reduce) {
+    acc += F64Arith::mulmod(x, key->w, key->wp, t.qd);
+    if (reduce) acc = F64Arith::reduce(acc, t);
+}
+__device__ __forceinline__ gptr<Shoup> ext_key(const ExtArgs &g, const IntArith *) { return as_global(g.key); }
+__device__ __forceinline__ gptr<ShoupF> ext_key(const ExtArgs &g, const F64Arith *) { return as_global(g.key_f); }
+// Hides the table's twiddle pointers from loop-invariant code motion.
+__device__ __forceinline__ void opaque(DevTable &t) {
+    asm volatile("" : "+s"(t.fwd), "+s"(t.inv), "+s"(t.fwd_f), "+s"(t.inv_f));
+}
+__device__ __forceinline__ void ext_finish(uint64_t &, const DevTable &) {}
+__device__ __forceinline__ void ext_finish(double &acc, const DevTable &t) { acc = F64Arith::reduce(acc, t); }
+
+template <class A, int LOGN>
+__global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(ExtArgs g) {
+    constexpr int LOGT = LOGN - kExtLogE;
+    using G = Geo<LOGN, LOGT>;
+    using T = typename A::T;
+    static_assert(G::E == (1 << kExtLogE), "key permutation assumes this register layout");
+    constexpr int R0 = G::fr(0);
+    __shared__ T lds[G::LDS];
+    const uint32_t t = threadIdx.x;
+    const uint64_t prod = blockIdx.x;
+    const DevTable tab = *g.tab;
+    const uint32_t n = 1u << LOGN;
+    const auto key = ext_key(g, (const A *)nullptr);
+    T acc[2][G::E];
+#pragma unroll
+    for (int j = 0; j < G::E; ++j) acc[0][j] = acc[1][j] = (T)0;
+
+    uint32_t term = 0;
+    for (uint32_t p = 0; p < 2; ++p) {
+        const uint64_t *src = g.in + (prod * 2 + p) * n;
+        // init_value_carry_slice_inplace (primus_decompose/src/big_integer/basis.rs:307-331)
+        uint64_t adj[G::E];
+        uint32_t carry[G::E];
+#pragma unroll
+        for (int m = 0; m < (G::E >> R0); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R0); ++j) {
+                uint64_t v = src[fwd_slot<LOGN, LOGT, 0, R0>(t, m, j)];
+                if (g.has_split && v >= g.split) v += g.pow2_sub_q;
+                adj[(m << R0) + j] = v;
+                carry[(m << R0) + j] = g.drop_bits ? (uint32_t)((v >> (g.drop_bits - 1)) & 1u) : 0u;
+            }
+        for (uint32_t d = 0; d < g.l; ++d) {
+            T x[G::E];
+#pragma unroll
+            for (int i = 0; i < G::E; ++i) x[i] = A::load(gadget_digit(adj[i], d, carry[i], g), tab);
+            // The twiddles are the same for every digit; without this the
+            // compiler hoists all their loads out of the digit loop and keeps
+            // them live in registers for the whole kernel.
+            // The same holds for the per-thread LDS slot addresses.
+            DevTable td = tab;
+            opaque(td);
+            uint32_t tl = t;
+            asm volatile("" : "+v"(tl));
+            FwdRounds<A, LOGN, LOGT, 0>::run(x, lds, tl, 0, 0, td);
+            // thread t now holds slots t*E + j; key rows are stored permuted so
+            // that (row, t*E + j) sits at row*N + j*T + t: coalesced loads.
+            // Loads are issued a quarter at a time to bound live registers.
+            const bool red = (++term % 3) == 0;
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < G::E; ++j) {
+                const uint64_t base = ((uint64_t)(p * g.l + d) * 2) * n + (uint64_t)j * G::T + t;
+                ext_mac(acc[0][j], x[j], key + base, tab, red);
+                ext_mac(acc[1][j], x[j], key + base + n, tab, red);
+                if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        uint64_t *dst = g.out + (prod * 2 + o) * n;
+        T y[G::E];
+#pragma unroll
+        for (int j = 0; j < G::E; ++j) {
+            y[j] = acc[o][j];
+            ext_finish(y[j], tab);
+        }
+        if (g.ntt_output) {
+            // NTT-form result (bit-reversed slots), canonical
+#pragma unroll
+            for (int j = 0; j < G::E; ++j) dst[((uint64_t)t << G::LOGE) + j] = A::out_inv(y[j], tab);
+        } else {
+            InvRounds<A, LOGN, LOGT, 0, true>::run(y, lds, t, n, 0, tab);
+            InvRounds<A, LOGN, LOGT, G::NR - 1, true>::store_global(dst, y, t, tab);
+        }
+    }
+}
+
+template <class A, int LOGN>
+static hipError_t launch_ext_n(const ExtArgs &g, uint64_t count, hipStream_t s) {
+    ext_product_kernel<A, LOGN><<<dim3((uint32_t)count), dim3(1u << (LOGN - kExtLogE)), 0, s>>>(g);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    switch (log_n) {
+#define PH_EXT_CASE(LN)                                                                        \
+    case LN:                                                                                   \
+        return f64 ? launch_ext_n<F64Arith, LN>(g, count, s) : launch_ext_n<IntArith, LN>(g, count, s);
+        PH_EXT_CASE(6)
+        PH_EXT_CASE(7)
+        PH_EXT_CASE(8)
+        PH_EXT_CASE(9)
+        PH_EXT_CASE(10)
+        PH_EXT_CASE(11)
+        PH_EXT_CASE(12)
+#undef PH_EXT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ph

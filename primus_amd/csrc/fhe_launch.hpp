@@ -1,0 +1,36 @@
+// fhe_launch.hpp -- host-side view of the fused external-product and
+// RLWE-encryption kernels (fhe_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ntt_device.hpp"
+
+namespace ph {
+
+constexpr uint32_t kExtMinLogN = 6;
+// Coefficients per thread of the external-product kernel (2^kExtLogE); the
+// key rows are stored permuted for this register layout.
+constexpr uint32_t kExtLogE = 3;
+constexpr uint32_t kExtMaxLogN = 12;
+
+// External product arguments: gadget constants of
+// BigUintApproxSignedBasis::new(q, log_basis, Some(l)) for a one-limb q, the
+// resident key and the batch.
+struct ExtArgs {
+    const uint64_t *in;     // [count][2][N] coefficients < q
+    uint64_t *out;          // [count][2][N]
+    const Shoup *key;       // [2][l][2][N] NTT form, permuted for coalescing (integer path)
+    const ShoupF *key_f;    // the same, centred binary64 form (q < 2^50)
+    const DevTable *tab;
+    uint64_t split;         // values >= split are shifted by pow2_sub_q (has_split)
+    uint64_t pow2_sub_q;    // 2^bits(q) - q
+    uint64_t basis_minus_one, carry_mask, half, q_minus_basis;
+    uint32_t drop_bits, log_basis, l;
+    uint32_t has_split, basis_is_two, ntt_output;
+};
+
+hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64_t count, hipStream_t s);
+
+}  // namespace ph

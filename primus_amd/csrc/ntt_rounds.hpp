@@ -1,0 +1,143 @@
+// ntt_rounds.hpp -- register-round schedule of the chunk kernels: geometry,
+// slot numbering and the forward/inverse round sequences with their LDS
+// exchanges (see ntt_device.hpp for the butterflies and the arithmetic).
+#pragma once
+
+#include "ntt_device.hpp"
+
+namespace ph {
+
+template <int LOGC, int LOGT>
+struct Geo {
+    static constexpr int C = 1 << LOGC;
+    static constexpr int T = 1 << LOGT;
+    static constexpr int LOGE = LOGC - LOGT;
+    static constexpr int E = 1 << LOGE;
+    static constexpr int REM = LOGC % LOGE;
+    static constexpr int NR = LOGC / LOGE + (REM ? 1 : 0);
+    static constexpr int LDS = C + (C >> 4);
+    // forward rounds: a short round first (if any), then full rounds
+    static constexpr int fr(int i) { return (REM != 0 && i == 0) ? REM : LOGE; }
+    static constexpr int fs(int i) { return i == 0 ? 0 : (REM != 0 ? REM + (i - 1) * LOGE : i * LOGE); }
+    // inverse rounds: full rounds, then the short one
+    static constexpr int ir(int i) { return (REM != 0 && i == NR - 1) ? REM : LOGE; }
+    static constexpr int is(int i) { return i * LOGE; }
+    static_assert(LOGE >= 1, "a thread needs at least two coefficients");
+};
+
+// Local slot of register (m, j) in the forward round covering local stages [S, S+R).
+template <int LOGC, int LOGT, int S, int R>
+__device__ __forceinline__ uint32_t fwd_slot(uint32_t t, int m, int j) {
+    constexpr int OB = LOGC - S - R;
+    const uint32_t gi = t + ((uint32_t)m << LOGT);
+    const uint32_t c = gi >> OB;
+    const uint32_t o = gi & ((1u << OB) - 1u);
+    return (c << (LOGC - S)) + ((uint32_t)j << OB) + o;
+}
+
+// Local slot of register (m, j) in the inverse round covering local stages [P, P+R).
+template <int LOGC, int LOGT, int P, int R>
+__device__ __forceinline__ uint32_t inv_slot(uint32_t t, int m, int j) {
+    const uint32_t gi = t + ((uint32_t)m << LOGT);
+    const uint32_t c = gi >> P;
+    const uint32_t o = gi & ((1u << P) - 1u);
+    return (c << (P + R)) + ((uint32_t)j << P) + o;
+}
+
+template <class A, int LOGC, int LOGT, int I>
+struct FwdRounds {
+    using G = Geo<LOGC, LOGT>;
+    using T = typename A::T;
+    static constexpr int S = G::fs(I), R = G::fr(I);
+
+    __device__ __forceinline__ static void store(T *lds, const T *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))] = x[(m << R) + j];
+    }
+    __device__ __forceinline__ static void load(const T *lds, T *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))];
+    }
+    __device__ __forceinline__ static void load_global(const uint64_t *__restrict__ g, T *x, uint32_t t,
+                                                       const DevTable &tab) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j)
+                x[(m << R) + j] = A::load(g[fwd_slot<LOGC, LOGT, S, R>(t, m, j)], tab);
+    }
+
+    // Runs rounds I .. NR-1.  st0: global stage of local stage 0; b: chunk index
+    // of this workgroup inside the polynomial.
+    __device__ __forceinline__ static void run(T *x, T *lds, uint32_t t, uint32_t st0, uint32_t b,
+                                               const DevTable &tab) {
+        if constexpr (I > 0) {
+            __syncthreads();
+            FwdRounds<A, LOGC, LOGT, I - 1>::store(lds, x, t);
+            __syncthreads();
+            load(lds, x, t);
+        }
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m) {
+            constexpr int OB = LOGC - S - R;
+            uint32_t cl;
+            if constexpr (OB >= LOGT) cl = (uint32_t)m >> (OB - LOGT);  // wave-uniform
+            else cl = (t + ((uint32_t)m << LOGT)) >> OB;
+            ct_round<A, R>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
+        }
+        if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1>::run(x, lds, t, st0, b, tab);
+    }
+};
+
+template <class A, int LOGC, int LOGT, int I, bool LAST>
+struct InvRounds {
+    using G = Geo<LOGC, LOGT>;
+    using T = typename A::T;
+    static constexpr int P = G::is(I), R = G::ir(I);
+
+    __device__ __forceinline__ static void store(T *lds, const T *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))] = x[(m << R) + j];
+    }
+    __device__ __forceinline__ static void load(const T *lds, T *x, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))];
+    }
+    __device__ __forceinline__ static void store_global(uint64_t *__restrict__ g, const T *x, uint32_t t,
+                                                        const DevTable &tab) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j)
+                g[inv_slot<LOGC, LOGT, P, R>(t, m, j)] = A::out_inv(x[(m << R) + j], tab);
+    }
+
+    __device__ __forceinline__ static void run(T *x, T *lds, uint32_t t, uint32_t n, uint32_t b,
+                                               const DevTable &tab) {
+        if constexpr (I > 0) {
+            __syncthreads();
+            InvRounds<A, LOGC, LOGT, I - 1, LAST>::store(lds, x, t);
+            __syncthreads();
+            load(lds, x, t);
+        }
+        constexpr bool last_here = LAST && (I + 1 == G::NR);
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m) {
+            uint32_t cl;
+            if constexpr (P >= LOGT) cl = (uint32_t)m >> (P - LOGT);  // wave-uniform
+            else cl = (t + ((uint32_t)m << LOGT)) >> P;
+            gs_round<A, R, last_here>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n, A::inv(tab), tab);
+        }
+        if constexpr (I + 1 < G::NR) InvRounds<A, LOGC, LOGT, I + 1, LAST>::run(x, lds, t, n, b, tab);
+    }
+};
+
+}  // namespace ph
