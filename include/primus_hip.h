@@ -199,6 +199,37 @@ int ph_ext_product_device(const ph_ext_key *key, const uint64_t *d_in, uint64_t 
 /* The same on host memory, synchronous. */
 int ph_ext_product(const ph_ext_key *key, const uint64_t *in, uint64_t *out, size_t count, int ntt_output);
 
+/* ---- RLWE encryption, uneven batches ---- */
+
+typedef struct ph_rlwe_ctx ph_rlwe_ctx; /* device-resident tables for a grid of (ring degree, limb) */
+
+/* One request of an uneven batch: ring degree 2^log_n, `limbs` RNS limbs.
+ * a_offset: word offset of its [limbs][N] a_hat block in the a buffer (the
+ * b_hat block goes to the same offset of the output); small_offset: element
+ * offset of its N secret / error / message coefficients. */
+typedef struct {
+    uint32_t log_n;
+    uint32_t limbs;
+    uint64_t a_offset;
+    uint64_t small_offset;
+} ph_rlwe_request;
+
+/* Tables for ring degrees 2^min_log_n .. 2^max_log_n (within 2^10 .. 2^15),
+ * limb i of degree 2^l using moduli[(l - min_log_n) * max_limbs + i]. */
+int ph_rlwe_ctx_new(uint32_t min_log_n, uint32_t max_log_n, uint32_t max_limbs, const uint64_t *moduli,
+                    ph_rlwe_ctx **out);
+void ph_rlwe_ctx_free(ph_rlwe_ctx *ctx);
+uint64_t ph_rlwe_ctx_modulus(const ph_rlwe_ctx *ctx, uint32_t log_n, uint32_t limb);
+/* NttRlweSecretKey::encrypt_inplace_with_embedding, unsigned embedding, with the
+ * samples supplied (primus_fhe_core/src/secret_key/rlwe.rs:227-268): per request
+ * and limb, b_hat = NTT(e + Delta*m) + a_hat (.) NTT(s) mod q_i, with
+ * Delta = floor(Q/256) and Q the product of the request's limb moduli.  s, e
+ * are int8 (ternary secret, rounded Gaussian error), m uint8; a_hat is
+ * uniform in NTT form.  All requests run in one persistent launch. */
+int ph_rlwe_encrypt_device(const ph_rlwe_ctx *ctx, const ph_rlwe_request *requests, uint32_t n_requests,
+                           const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m,
+                           uint64_t *d_out, void *stream);
+
 /* ---- autotuning (no reference counterpart: launch-shape selection for the GPU) ---- */
 
 /* Operation codes for the tuning table. */

@@ -569,3 +569,33 @@ void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, c
         for (int j = 0; j < 2; ++j) po_inverse_transform(t, out + j * n);
     free(res);
 }
+
+/* ------------------------------------------------------------------------- */
+/* RLWE encryption with supplied samples                                     */
+/* ------------------------------------------------------------------------- */
+
+/* NttRlweSecretKey::encrypt_inplace_with_embedding, unsigned embedding, with
+ * the samples given instead of drawn (primus_fhe_core/src/secret_key/rlwe.rs:227-268):
+ *   b = e mod q                          (sample_gaussian_values_inplace, :245)
+ *   b = b + (delta * m mod q) mod q      (add_mul_factor_assign, Shoup, :247-260)
+ *   b = NTT(b)                           (transform_slice, :263)
+ *   b = b + a (.) s_hat (mod q)          (add_mul_assign, Barrett, :267)
+ * s_hat = NTT(s mod q) is the NTT-form secret of the key.  a_hat is the uniform
+ * polynomial already in NTT form (:265). */
+void po_rlwe_encrypt(const po_table *t, const int8_t *s, const int8_t *e, const uint8_t *m, uint64_t delta,
+                     const uint64_t *a_hat, uint64_t *b_hat) {
+    const size_t n = t->n;
+    const uint64_t q = t->q;
+    uint64_t *sh = (uint64_t *)malloc(n * 8);
+    for (size_t i = 0; i < n; ++i) sh[i] = s[i] >= 0 ? (uint64_t)s[i] : q - (uint64_t)(-(int)s[i]);
+    po_transform(t, sh);
+    uint64_t dq = po_shoup_quotient(delta, q);
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t b = e[i] >= 0 ? (uint64_t)e[i] : q - (uint64_t)(-(int)e[i]);
+        b = reduce_once(b + po_shoup_mul(delta, dq, m[i], q), q);
+        b_hat[i] = b;
+    }
+    po_transform(t, b_hat);
+    po_pointwise_mul_add(b_hat, a_hat, sh, n, q);
+    free(sh);
+}

@@ -117,3 +117,16 @@ void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, c
 }
 #endif
 #endif
+#ifndef PRIMUS_ORACLE_RLWE_H
+#define PRIMUS_ORACLE_RLWE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* RLWE encryption with supplied samples: b_hat = NTT(e + delta*m) + a_hat (.) NTT(s).
+ * primus_fhe_core/src/secret_key/rlwe.rs:227-268 */
+void po_rlwe_encrypt(const po_table *t, const int8_t *s, const int8_t *e, const uint8_t *m, uint64_t delta,
+                     const uint64_t *a_hat, uint64_t *b_hat);
+#ifdef __cplusplus
+}
+#endif
+#endif

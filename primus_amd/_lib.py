@@ -133,6 +133,10 @@ def _declare(L):
         "ph_ext_key_free": (None, [vp]),
         "ph_ext_product_device": (i, [vp, vp, vp, sz, i, vp]),
         "ph_ext_product": (i, [vp, p64, p64, sz, i]),
+        "ph_rlwe_ctx_new": (i, [u32, u32, u32, p64, pvp]),
+        "ph_rlwe_ctx_free": (None, [vp]),
+        "ph_rlwe_ctx_modulus": (u64, [vp, u32, u32]),
+        "ph_rlwe_encrypt_device": (i, [vp, vp, u32, vp, vp, vp, vp, vp, vp]),
         "ph_tuning_set": (i, [u32, i, i, i]),
         "ph_tuning_get": (i, [u32, i, i]),
         "ph_tuning_save": (i, [ctypes.c_char_p]),

@@ -6,9 +6,10 @@
 // fallback, and a missing GPU is reported as PH_ERR_DEVICE.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
-#include <atomic>
 #include <chrono>
 #include <mutex>
 #include <new>
@@ -724,6 +725,123 @@ int ph_ext_product(const ph_ext_key *key, const uint64_t *in, uint64_t *out, siz
     PH_HIP(hipMemcpyAsync(out, d_out, words * 8, hipMemcpyDeviceToHost, s));
     PH_HIP(hipStreamSynchronize(s));
     return PH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// RLWE encryption
+// ---------------------------------------------------------------------------
+
+struct ph_rlwe_ctx {
+    uint32_t min_log_n, max_log_n, max_limbs;
+    std::vector<ph_ntt_table *> tables;  // [(log_n - min_log_n) * max_limbs + i]
+    DevTable *d_tabs = nullptr;
+    int cus = 0;
+};
+
+void rlwe_ctx_free(ph_rlwe_ctx *c) {
+    if (!c) return;
+    for (ph_ntt_table *t : c->tables) free_table(t);
+    if (c->d_tabs) (void)hipFree(c->d_tabs);
+    delete c;
+}
+
+int ph_rlwe_ctx_new(uint32_t min_log_n, uint32_t max_log_n, uint32_t max_limbs, const uint64_t *moduli,
+                    ph_rlwe_ctx **out) {
+    if (!out || !moduli || max_limbs == 0 || min_log_n > max_log_n) return fail(PH_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    if (min_log_n < kRlweMinLogN || max_log_n > kRlweMaxLogN)
+        return fail(PH_ERR_UNSUPPORTED, "RLWE encryption supports 2^10 <= N <= 2^15");
+    ph_rlwe_ctx *c = new (std::nothrow) ph_rlwe_ctx();
+    if (!c) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    c->min_log_n = min_log_n;
+    c->max_log_n = max_log_n;
+    c->max_limbs = max_limbs;
+    std::vector<DevTable> views;
+    for (uint32_t ln = min_log_n; ln <= max_log_n; ++ln)
+        for (uint32_t i = 0; i < max_limbs; ++i) {
+            ph_ntt_table *t = nullptr;
+            int rc = build_table(ln, moduli[(ln - min_log_n) * max_limbs + i], PH_TABLE_INTEGER_ONLY, &t);
+            if (rc != PH_OK) {
+                rlwe_ctx_free(c);
+                return rc;
+            }
+            c->tables.push_back(t);
+            views.push_back(t->dev);
+        }
+    int rc = upload((void **)&c->d_tabs, views.data(), views.size() * sizeof(DevTable));
+    if (rc == PH_OK && hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess)
+        rc = fail(PH_ERR_DEVICE, "cannot query the compute-unit count");
+    if (rc != PH_OK) {
+        rlwe_ctx_free(c);
+        return rc;
+    }
+    *out = c;
+    return PH_OK;
+}
+
+void ph_rlwe_ctx_free(ph_rlwe_ctx *c) { rlwe_ctx_free(c); }
+
+uint64_t ph_rlwe_ctx_modulus(const ph_rlwe_ctx *c, uint32_t log_n, uint32_t limb) {
+    if (!c || log_n < c->min_log_n || log_n > c->max_log_n || limb >= c->max_limbs) return 0;
+    return c->tables[(log_n - c->min_log_n) * c->max_limbs + limb]->q;
+}
+
+int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs, const uint64_t *d_a,
+                           const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m, uint64_t *d_out, void *stream) {
+    if (!c || (!reqs && n_reqs)) return fail(PH_ERR_INVALID, "null argument");
+    if (n_reqs == 0) return PH_OK;
+    if (!d_a || !d_s || !d_e || !d_m || !d_out) return fail(PH_ERR_INVALID, "null device buffer");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<RlweItem> items;
+    for (uint32_t r = 0; r < n_reqs; ++r) {
+        const ph_rlwe_request &q = reqs[r];
+        if (q.log_n < c->min_log_n || q.log_n > c->max_log_n || q.limbs == 0 || q.limbs > c->max_limbs)
+            return fail(PH_ERR_INVALID, "request outside the context's ring degrees or limb count");
+        const uint64_t n = 1ull << q.log_n;
+        const uint32_t base = (q.log_n - c->min_log_n) * c->max_limbs;
+        // Delta = floor(Q / 256), Q the product of the request's moduli, kept
+        // per limb as Delta mod q_i = (-(Q mod 256)) * 256^-1 mod q_i.
+        uint64_t q_mod_256 = 1;
+        for (uint32_t i = 0; i < q.limbs; ++i) q_mod_256 = (q_mod_256 * (c->tables[base + i]->q & 255)) & 255;
+        for (uint32_t i = 0; i < q.limbs; ++i) {
+            const uint64_t qi = c->tables[base + i]->q;
+            const uint64_t inv256 = inv_mod(256 % qi, qi);
+            const uint64_t delta = h_mulmod((qi - q_mod_256 % qi) % qi, inv256, qi);
+            RlweItem it{};
+            it.row = q.a_offset + (uint64_t)i * n;
+            it.small = q.small_offset;
+            it.delta = delta;
+            it.delta_q = h_shoup_quotient(delta, qi);
+            it.log_n = q.log_n;
+            it.tab = base + i;
+            items.push_back(it);
+        }
+    }
+    // largest transforms first: the workgroups' strided walk then balances
+    std::stable_sort(items.begin(), items.end(),
+                     [](const RlweItem &x, const RlweItem &y) { return x.log_n > y.log_n; });
+    // persistent: two resident workgroups per compute unit (the kernel's occupancy)
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(items.size(), (uint64_t)std::max(c->cus, 1) * 2);
+    RlweItem *d_items = nullptr;
+    uint64_t *scratch = nullptr;
+    PH_HIP(hipMallocAsync((void **)&d_items, items.size() * sizeof(RlweItem), s));
+    PH_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(RlweItem), hipMemcpyHostToDevice, s));
+    PH_HIP(hipMallocAsync((void **)&scratch, (size_t)grid * (1ull << kRlweMaxLogN) * 8, s));
+    PH_HIP(hipStreamSynchronize(s));  // the host item list may go away after this
+    RlweArgs a{};
+    a.items = d_items;
+    a.n_items = (uint32_t)items.size();
+    a.tabs = c->d_tabs;
+    a.a = d_a;
+    a.out = d_out;
+    a.s = d_s;
+    a.e = d_e;
+    a.m = d_m;
+    a.scratch = scratch;
+    hipError_t e = launch_rlwe_encrypt(a, grid, s);
+    (void)hipFreeAsync(scratch, s);
+    (void)hipFreeAsync(d_items, s);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
 }
 
 int ph_tuning_set(uint32_t log_n, int op, int arith, int loge) {

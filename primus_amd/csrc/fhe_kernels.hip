@@ -156,4 +156,117 @@ hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------
+// RLWE encryption
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t lift_small(int v, uint64_t q) { return v >= 0 ? (uint64_t)v : q - (uint64_t)(-v); }
+
+// e + Delta*m (mod q): sample_gaussian_values_inplace then add_mul_factor_assign
+// (primus_fhe_core/src/secret_key/rlwe.rs:245-260).
+__device__ __forceinline__ uint64_t rlwe_x(const RlweArgs &a, const RlweItem &it, uint64_t k, const DevTable &t) {
+    const uint64_t e = lift_small(a.e[it.small + k], t.q);
+    return add_mod(e, shoup_full(it.delta, it.delta_q, a.m[it.small + k], t.q), t.q);
+}
+
+// b_hat = x_hat + a_hat (.) s_hat, NttPolynomial::add_mul_assign (rlwe.rs:267):
+// Barrett reduce_mul_add on canonical operands.
+__device__ __forceinline__ void rlwe_mac(const RlweArgs &a, const RlweItem &it, uint64_t k, uint64_t x, uint64_t s,
+                                         const DevTable &t) {
+    const uint64_t xs = IntArith::canon_fwd(x, t), ss = IntArith::canon_fwd(s, t);
+    a.out[it.row + k] = barrett_mul_add(a.a[it.row + k], ss, xs, t.q, t.br0, t.br1);
+}
+
+// N <= 2^12: both transforms in LDS, 256 threads, E = N / 256.
+template <int LOGN>
+__device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &it, const DevTable &t, uint64_t *lds,
+                                            uint32_t tid) {
+    constexpr int LOGT = 8;
+    using G = Geo<LOGN, LOGT>;
+    constexpr int R0 = G::fr(0);
+    uint64_t sx[G::E], x[G::E];
+#pragma unroll
+    for (int m = 0; m < (G::E >> R0); ++m)
+#pragma unroll
+        for (int j = 0; j < (1 << R0); ++j) {
+            const uint32_t k = fwd_slot<LOGN, LOGT, 0, R0>(tid, m, j);
+            sx[(m << R0) + j] = lift_small(a.s[it.small + k], t.q);
+            x[(m << R0) + j] = rlwe_x(a, it, k, t);
+        }
+    FwdRounds<IntArith, LOGN, LOGT, 0>::run(sx, lds, tid, 0, 0, t);
+    FwdRounds<IntArith, LOGN, LOGT, 0>::run(x, lds, tid, 0, 0, t);
+#pragma unroll
+    for (int j = 0; j < G::E; ++j) rlwe_mac(a, it, ((uint64_t)tid << G::LOGE) + j, x[j], sx[j], t);
+}
+
+// N = 2^(12+R): column pass (stages 0..R-1) of s into scratch and of x into
+// the output row, then 2^R chunk passes of 4096 in LDS with the MAC fused.
+template <int R>
+__device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem &it, const DevTable &t, uint64_t *lds,
+                                              uint32_t tid, uint64_t *scr) {
+    constexpr int LOGC = 12, LOGT = 8;
+    constexpr uint32_t C = 1u << LOGC;
+    using G = Geo<LOGC, LOGT>;
+    for (uint32_t cc = 0; cc < C / kRlweThreads; ++cc) {
+        const uint32_t col = tid + cc * kRlweThreads;
+        uint64_t v[1 << R], w[1 << R];
+#pragma unroll
+        for (int i = 0; i < (1 << R); ++i) {
+            const uint64_t k = (uint64_t)i * C + col;
+            v[i] = lift_small(a.s[it.small + k], t.q);
+            w[i] = rlwe_x(a, it, k, t);
+        }
+        ct_round<IntArith, R>(v, 0, 0, IntArith::fwd(t), t);
+        ct_round<IntArith, R>(w, 0, 0, IntArith::fwd(t), t);
+#pragma unroll
+        for (int i = 0; i < (1 << R); ++i) {
+            scr[(uint64_t)i * C + col] = v[i];
+            a.out[it.row + (uint64_t)i * C + col] = w[i];
+        }
+    }
+    __syncthreads();  // the column results are read by other threads of the workgroup
+    for (uint32_t b = 0; b < (1u << R); ++b) {
+        asm volatile("" : "+v"(tid));  // keep slot addresses out of loop-invariant code motion
+        uint64_t sx[G::E], x[G::E];
+        FwdRounds<IntArith, LOGC, LOGT, 0>::load_global(scr + (uint64_t)b * C, sx, tid, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0>::run(sx, lds, tid, R, b, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0>::load_global(a.out + it.row + (uint64_t)b * C, x, tid, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0>::run(x, lds, tid, R, b, t);
+#pragma unroll
+        for (int j = 0; j < G::E; ++j) rlwe_mac(a, it, (uint64_t)b * C + ((uint64_t)tid << G::LOGE) + j, x[j], sx[j], t);
+    }
+}
+
+// Two waves per SIMD: the 2^12 path holds both transforms (s and e + Delta m) in
+// registers at once, and capping it lower spills.
+__global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs a) {
+    __shared__ uint64_t lds[Geo<12, 8>::LDS];
+    uint64_t *scr = a.scratch + (uint64_t)blockIdx.x * (1ull << kRlweMaxLogN);
+    for (uint32_t i = blockIdx.x; i < a.n_items; i += gridDim.x) {
+        const RlweItem it = a.items[i];
+        const DevTable t = a.tabs[it.tab];
+        // Slot addresses depend on the thread alone; hidden from loop-invariant
+        // code motion they are recomputed per item instead of being held in
+        // registers (for every ring degree at once) across the whole loop.
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        switch (it.log_n) {
+            case 10: rlwe_single<10>(a, it, t, lds, tid); break;
+            case 11: rlwe_single<11>(a, it, t, lds, tid); break;
+            case 12: rlwe_single<12>(a, it, t, lds, tid); break;
+            case 13: rlwe_two_pass<1>(a, it, t, lds, tid, scr); break;
+            case 14: rlwe_two_pass<2>(a, it, t, lds, tid, scr); break;
+            case 15: rlwe_two_pass<3>(a, it, t, lds, tid, scr); break;
+            default: break;
+        }
+        __syncthreads();  // LDS and scratch are reused by the next item
+    }
+}
+
+hipError_t launch_rlwe_encrypt(const RlweArgs &a, uint32_t grid, hipStream_t s) {
+    if (a.n_items == 0 || grid == 0) return hipSuccess;
+    rlwe_encrypt_kernel<<<dim3(grid), dim3(kRlweThreads), 0, s>>>(a);
+    return hipGetLastError();
+}
+
 }  // namespace ph

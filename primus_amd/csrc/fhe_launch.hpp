@@ -33,4 +33,34 @@ struct ExtArgs {
 
 hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64_t count, hipStream_t s);
 
+// RLWE encryption: ring degrees 2^10 .. 2^15 in one persistent launch.
+constexpr uint32_t kRlweMinLogN = 10;
+constexpr uint32_t kRlweMaxLogN = 15;
+constexpr uint32_t kRlweThreads = 256;
+
+// One (request, limb) work item.
+struct RlweItem {
+    uint64_t row;        // word offset of this limb's row in a_hat / b_hat
+    uint64_t small;      // element offset of the request's s, e, m
+    uint64_t delta;      // Delta mod q_i
+    uint64_t delta_q;    // its Shoup quotient
+    uint32_t log_n;
+    uint32_t tab;        // index into the table array
+};
+
+struct RlweArgs {
+    const RlweItem *items;  // sorted by decreasing ring degree
+    uint32_t n_items;
+    uint32_t pad_;
+    const DevTable *tabs;
+    const uint64_t *a;      // a_hat rows (NTT form)
+    uint64_t *out;          // b_hat rows
+    const int8_t *s;        // ternary secret
+    const int8_t *e;        // error
+    const uint8_t *m;       // message
+    uint64_t *scratch;      // 2^15 words per workgroup (ring degrees above 2^12)
+};
+
+hipError_t launch_rlwe_encrypt(const RlweArgs &a, uint32_t grid, hipStream_t s);
+
 }  // namespace ph

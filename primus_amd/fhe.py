@@ -50,3 +50,53 @@ class ExtKey:
 
     def product_device(self, d_in, d_out, count, ntt_output=False, stream=None):
         _check(lib().ph_ext_product_device(self._h, d_in, d_out, count, int(bool(ntt_output)), _stream(stream)))
+
+
+class RlweRequest(ctypes.Structure):
+    """ph_rlwe_request: one request of an uneven encryption batch."""
+
+    _fields_ = [("log_n", ctypes.c_uint32), ("limbs", ctypes.c_uint32), ("a_offset", ctypes.c_uint64),
+                ("small_offset", ctypes.c_uint64)]
+
+
+class RlweContext:
+    """Device-resident tables for RLWE encryption over a grid of (ring degree, limb).
+
+    moduli[log_n] lists the limb moduli of ring degree 2^log_n (at least max_limbs each).
+    """
+
+    def __init__(self, min_log_n, max_log_n, max_limbs, moduli):
+        grid = np.zeros(((max_log_n - min_log_n + 1), max_limbs), dtype=np.uint64)
+        for ln in range(min_log_n, max_log_n + 1):
+            grid[ln - min_log_n] = moduli[ln][:max_limbs]
+        h = ctypes.c_void_p()
+        _check(lib().ph_rlwe_ctx_new(min_log_n, max_log_n, max_limbs, grid.ctypes.data_as(_p64), ctypes.byref(h)))
+        self._h = h
+        self.min_log_n, self.max_log_n, self.max_limbs = min_log_n, max_log_n, max_limbs
+
+    def modulus(self, log_n, limb):
+        return lib().ph_rlwe_ctx_modulus(self._h, log_n, limb)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().ph_rlwe_ctx_free(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def encrypt_device(self, requests, d_a, d_s, d_e, d_m, d_out, stream=None):
+        """requests: sequence of (log_n, limbs, a_offset, small_offset)."""
+        arr = (RlweRequest * len(requests))(*[RlweRequest(*r) for r in requests])
+        _check(lib().ph_rlwe_encrypt_device(self._h, ctypes.cast(arr, ctypes.c_void_p), len(requests), d_a, d_s,
+                                            d_e, d_m, d_out, _stream(stream)))
+
+
+def rlwe_layout(shapes):
+    """Offsets for a batch of (log_n, limbs) requests: (requests, a_words, small_elems)."""
+    reqs, a_off, s_off = [], 0, 0
+    for log_n, limbs in shapes:
+        reqs.append((log_n, limbs, a_off, s_off))
+        a_off += limbs << log_n
+        s_off += 1 << log_n
+    return reqs, a_off, s_off

@@ -1,0 +1,144 @@
+"""RLWE encryption over uneven batches: oracle pinning on the CPU, GPU parity.
+
+The oracle restates NttRlweSecretKey::encrypt_inplace_with_embedding with the
+samples supplied (primus_fhe_core/src/secret_key/rlwe.rs:227-268); it is pinned
+by the algebraic identity b_hat - a_hat (.) NTT(s) = NTT(e + Delta m) and by
+decryption: INTT(b_hat - a_hat (.) s_hat) = e + Delta m.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def oracle_encrypt(table, s, e, m, delta, a_hat):
+    L = O.lib()
+    L.po_rlwe_encrypt.restype = None
+    L.po_rlwe_encrypt.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_uint64, O._p64, O._p64]
+    out = np.zeros(table.n, dtype=np.uint64)
+    a_hat = np.ascontiguousarray(a_hat, dtype=np.uint64)
+    L.po_rlwe_encrypt(table.handle, _p(np.ascontiguousarray(s, dtype=np.int8)),
+                      _p(np.ascontiguousarray(e, dtype=np.int8)), _p(np.ascontiguousarray(m, dtype=np.uint8)),
+                      delta, O.ptr(a_hat), O.ptr(out))
+    return out
+
+
+def delta_of(moduli, qi):
+    """Delta = floor(Q/256) mod q_i, Q the product of the request's moduli."""
+    Q = 1
+    for q in moduli:
+        Q *= q
+    return (Q // 256) % qi
+
+
+def samples(n, rng):
+    s = rng.integers(-1, 2, size=n).astype(np.int8)  # ternary, uniform over {-1, 0, 1}
+    e = np.clip(np.rint(rng.normal(0, 3.2, size=n)), -19, 19).astype(np.int8)  # rounded Gaussian, 6 sigma cut
+    m = rng.integers(0, 256, size=n).astype(np.uint8)
+    return s, e, m
+
+
+def test_oracle_encrypt_decrypts():
+    import primus_amd as pa
+
+    log_n = 10
+    n = 1 << log_n
+    moduli = pa.ntt_primes_below(59, 2 * n, 3)
+    rng = np.random.default_rng(3)
+    s, e, m = samples(n, rng)
+    for q in moduli:
+        t = O.Table(log_n, q)
+        a_hat = rng.integers(0, q, size=n, dtype=np.uint64)
+        d = delta_of(moduli, q)
+        b = oracle_encrypt(t, s, e, m, d, a_hat)
+        s_hat = t.transform(np.array([int(x) % q for x in s], dtype=np.uint64))
+        # b - a*s in the NTT domain, back to coefficients: e + Delta*m (mod q)
+        diff = (b.astype(object) - O.pointwise_mul(a_hat, s_hat, q).astype(object)) % q
+        dec = t.inverse_transform(np.array(diff, dtype=np.uint64))
+        want = np.array([(int(ei) + d * int(mi)) % q for ei, mi in zip(e, m)], dtype=np.uint64)
+        np.testing.assert_array_equal(dec, want)
+
+
+def test_delta_formula_matches_bigint():
+    """The library's Delta mod q_i uses (-(Q mod 256)) * 256^-1 mod q_i; check it against big integers."""
+    import primus_amd as pa
+
+    for L in (1, 2, 5, 12):
+        moduli = pa.ntt_primes_below(59, 2048, L)
+        r = 1
+        for q in moduli:
+            r = r * (q & 255) & 255
+        for q in moduli:
+            assert (-r) * pow(256, -1, q) % q == delta_of(moduli, q)
+
+
+# ---------------------------------------------------------------- GPU parity
+
+def _moduli_grid(min_ln, max_ln, limbs):
+    import primus_amd as pa
+
+    return {ln: pa.ntt_primes_below(59, 2 << ln, limbs) for ln in range(min_ln, max_ln + 1)}
+
+
+def _run_batch(shapes, seed, max_limbs=12, check=None):
+    import primus_amd as pa
+    from primus_amd.fhe import RlweContext, rlwe_layout
+
+    grid = _moduli_grid(10, 15, max_limbs)
+    ctx = RlweContext(10, 15, max_limbs, grid)
+    reqs, a_words, small = rlwe_layout(shapes)
+    rng = np.random.default_rng(seed)
+    a = np.zeros(a_words, dtype=np.uint64)
+    for (log_n, limbs, ao, so) in reqs:
+        n = 1 << log_n
+        for i in range(limbs):
+            a[ao + i * n: ao + (i + 1) * n] = rng.integers(0, grid[log_n][i], size=n, dtype=np.uint64)
+    s, e, m = samples(small, rng)
+    da = pa.DeviceBuffer.from_array(a)
+    dout = pa.DeviceBuffer(a_words)
+    ds, de, dm = (pa.DeviceBuffer.from_array(np.frombuffer(np.ascontiguousarray(v).tobytes().ljust(
+        ((small + 7) // 8) * 8, b"\0"), dtype=np.uint64)) for v in (s, e, m))
+    ctx.encrypt_device(reqs, da.ptr, ds.ptr, de.ptr, dm.ptr, dout.ptr)
+    pa.synchronize()
+    out = dout.download()
+    idx = range(len(reqs)) if check is None else check
+    for r in idx:
+        log_n, limbs, ao, so = reqs[r]
+        n = 1 << log_n
+        mods = grid[log_n][:limbs]
+        for i in range(limbs):
+            q = mods[i]
+            t = O.Table(log_n, q)
+            want = oracle_encrypt(t, s[so:so + n], e[so:so + n], m[so:so + n], delta_of(mods, q),
+                                  a[ao + i * n: ao + (i + 1) * n])
+            np.testing.assert_array_equal(out[ao + i * n: ao + (i + 1) * n], want, err_msg="req %d limb %d" % (r, i))
+    return reqs
+
+
+@pytest.mark.gpu
+def test_gpu_rlwe_every_degree():
+    shapes = [(ln, L) for ln in range(10, 16) for L in (1, 3)]
+    _run_batch(shapes, seed=1)
+
+
+@pytest.mark.gpu
+def test_gpu_rlwe_uneven_batch():
+    rng = np.random.default_rng(7)
+    shapes = [(int(rng.integers(10, 16)), int(rng.integers(1, 13))) for _ in range(48)]
+    _run_batch(shapes, seed=2)
+
+
+@pytest.mark.gpu
+def test_gpu_rlwe_full_configuration_sampled():
+    """1024 requests, N ~ U{2^10..2^15}, L ~ U{1..12}: every limb of a sample of requests checked."""
+    rng = np.random.default_rng(0x5052494D5553 + 3)
+    shapes = [(int(rng.integers(10, 16)), int(rng.integers(1, 13))) for _ in range(1024)]
+    sample = sorted(set(int(x) for x in rng.integers(0, 1024, size=24)) | {0, 1023})
+    _run_batch(shapes, seed=3, check=sample)
