@@ -119,6 +119,11 @@ int ph_dcrt_table_new(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_co
 int ph_dcrt_table_new_ex(uint32_t log_n, const uint64_t *moduli, uint32_t moduli_count, uint32_t flags,
                          ph_dcrt_table **out);
 void ph_dcrt_table_free(ph_dcrt_table *table);
+/* A DcrtTable over existing NttTables of one ring degree (the
+ * UintCrtNttTable { ntt_tables } of primus_ntt/src/dcrt/primitive.rs:9-14
+ * without rebuilding them).  The view does not own the tables, which must
+ * outlive it. */
+int ph_dcrt_table_view(const ph_ntt_table *const *tables, uint32_t moduli_count, ph_dcrt_table **out);
 /* As ph_ntt_table_arith, for all limbs together. */
 int ph_dcrt_table_arith(const ph_dcrt_table *table);
 /* DcrtTable::{poly_length, moduli_count, crt_poly_length} (dcrt/primitive.rs:57-70) */
@@ -186,6 +191,8 @@ typedef struct ph_ext_key ph_ext_key; /* device-resident RGSW/GGSW key in NTT fo
 int ph_ext_key_new(const ph_ntt_table *table, uint32_t log_basis, uint32_t decompose_length,
                    const uint64_t *key_ntt, ph_ext_key **out);
 void ph_ext_key_free(ph_ext_key *key);
+/* log2 of the key's ring degree (0 for a null key). */
+uint32_t ph_ext_key_log_n(const ph_ext_key *key);
 /* `count` independent external products: d_in [count][2][N] RLWE ciphertexts
  * (a, b) in coefficient form, d_out [count][2][N].  Per product and input
  * polynomial: approximate signed decomposition into l digits
@@ -229,6 +236,59 @@ uint64_t ph_rlwe_ctx_modulus(const ph_rlwe_ctx *ctx, uint32_t log_n, uint32_t li
 int ph_rlwe_encrypt_device(const ph_rlwe_ctx *ctx, const ph_rlwe_request *requests, uint32_t n_requests,
                            const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m,
                            uint64_t *d_out, void *stream);
+
+/* ---- serving: continuous batching (no reference counterpart; the reference is a
+ * single-threaded library, this is the request loop around the batched forms) ---- */
+
+#define PH_REQ_POLY_MUL 0     /* negacyclic product, [limbs][N] in a and b */
+#define PH_REQ_RLWE_ENCRYPT 1 /* as ph_rlwe_encrypt_device, one request */
+#define PH_REQ_EXT_PRODUCT 2  /* as ph_ext_product_device with the server's key */
+
+typedef struct {
+    int kind;
+    uint32_t log_n;
+    uint32_t limbs;      /* poly-multiply / RLWE limbs; ignored for external products */
+    const uint64_t *in0; /* poly-mul: a [limbs][N]; RLWE: a_hat [limbs][N]; ext: (a, b) [2][N] */
+    const uint64_t *in1; /* poly-mul: b [limbs][N] */
+    const int8_t *s;     /* RLWE: ternary secret [N] */
+    const int8_t *e;     /* RLWE: error [N] */
+    const uint8_t *m;    /* RLWE: message [N] */
+    uint64_t *out;       /* poly-mul, RLWE: [limbs][N]; ext: [2][N] */
+} ph_serve_request;
+
+typedef struct {
+    uint64_t hbm_budget_bytes; /* admission limit on the device memory of in-flight requests */
+    uint32_t max_batch;        /* requests per batch; 1 = the plain one-launch-per-request path */
+    uint32_t streams;          /* batches in flight at once */
+    uint32_t min_log_n, max_log_n, max_limbs;
+    const uint64_t *moduli;    /* [(max_log_n - min_log_n + 1)][max_limbs] limb moduli per ring degree */
+    const ph_ext_key *ext_key; /* key of the external-product requests (NULL: none accepted) */
+} ph_server_config;
+
+typedef struct {
+    uint64_t completed, batches, launches, peak_inflight_bytes;
+} ph_server_stats;
+
+typedef struct ph_server ph_server;
+
+/* Builds the resident tables and starts the dispatcher and completion threads. */
+int ph_server_new(const ph_server_config *config, ph_server **out);
+/* Drains outstanding work and stops. */
+void ph_server_free(ph_server *server);
+/* Queues a request; host buffers must stay valid until it completes.  Rejects
+ * malformed requests (PH_ERR_INVALID) and ones larger than the budget
+ * (PH_ERR_NO_MEMORY). */
+int ph_server_submit(ph_server *server, const ph_serve_request *request, uint64_t *ticket);
+/* Blocks until the ticket completes; *status receives its PH_* result. */
+int ph_server_wait(ph_server *server, uint64_t ticket, int *status);
+/* Seconds from submission to completion of a finished ticket. */
+int ph_server_latency(ph_server *server, uint64_t ticket, double *seconds);
+int ph_server_drain(ph_server *server);
+int ph_server_stats_get(ph_server *server, ph_server_stats *out);
+/* Open-loop load: submits requests[i] at Poisson arrival times of the given
+ * rate, waits for all, reports per-request latency and the elapsed time. */
+int ph_server_run_poisson(ph_server *server, const ph_serve_request *requests, size_t n, double rate_per_s,
+                          uint64_t seed, double *latency_s, double *elapsed_s);
 
 /* ---- autotuning (no reference counterpart: launch-shape selection for the GPU) ---- */
 

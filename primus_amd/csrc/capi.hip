@@ -37,6 +37,7 @@ struct ph_ntt_table {
 struct ph_dcrt_table {
     uint32_t log_n, L;
     bool f64 = false;
+    bool owns = true;  // false for a view over tables owned elsewhere
     std::vector<ph_ntt_table *> tables;
     DevTable *d_tabs = nullptr;
 };
@@ -457,9 +458,37 @@ int ph_dcrt_table_new_ex(uint32_t log_n, const uint64_t *moduli, uint32_t moduli
     return PH_OK;
 }
 
+int ph_dcrt_table_view(const ph_ntt_table *const *tables, uint32_t moduli_count, ph_dcrt_table **out) {
+    if (!out || !tables || moduli_count == 0) return fail(PH_ERR_INVALID, "null tables or output");
+    *out = nullptr;
+    const uint32_t log_n = tables[0] ? tables[0]->log_n : 0;
+    for (uint32_t i = 0; i < moduli_count; ++i)
+        if (!tables[i] || tables[i]->log_n != log_n) return fail(PH_ERR_INVALID, "tables must share the ring degree");
+    ph_dcrt_table *d = new (std::nothrow) ph_dcrt_table();
+    if (!d) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    d->log_n = log_n;
+    d->L = moduli_count;
+    d->owns = false;
+    d->f64 = true;
+    std::vector<DevTable> views;
+    for (uint32_t i = 0; i < moduli_count; ++i) {
+        d->tables.push_back(const_cast<ph_ntt_table *>(tables[i]));
+        views.push_back(tables[i]->dev);
+        d->f64 = d->f64 && tables[i]->f64;
+    }
+    int rc = upload((void **)&d->d_tabs, views.data(), views.size() * sizeof(DevTable));
+    if (rc != PH_OK) {
+        ph_dcrt_table_free(d);
+        return rc;
+    }
+    *out = d;
+    return PH_OK;
+}
+
 void ph_dcrt_table_free(ph_dcrt_table *d) {
     if (!d) return;
-    for (ph_ntt_table *t : d->tables) free_table(t);
+    if (d->owns)
+        for (ph_ntt_table *t : d->tables) free_table(t);
     if (d->d_tabs) (void)hipFree(d->d_tabs);
     delete d;
 }
@@ -692,6 +721,8 @@ int ph_ext_key_new(const ph_ntt_table *t, uint32_t log_basis, uint32_t decompose
     return PH_OK;
 }
 
+uint32_t ph_ext_key_log_n(const ph_ext_key *key) { return key ? key->table->log_n : 0; }
+
 void ph_ext_key_free(ph_ext_key *key) {
     if (!key) return;
     if (key->d_key) (void)hipFree(key->d_key);
@@ -826,7 +857,8 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     uint64_t *scratch = nullptr;
     PH_HIP(hipMallocAsync((void **)&d_items, items.size() * sizeof(RlweItem), s));
     PH_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(RlweItem), hipMemcpyHostToDevice, s));
-    PH_HIP(hipMallocAsync((void **)&scratch, (size_t)grid * (1ull << kRlweMaxLogN) * 8, s));
+    if (items.front().log_n > 12)  // only the two-pass ring degrees use scratch
+        PH_HIP(hipMallocAsync((void **)&scratch, (size_t)grid * (1ull << kRlweMaxLogN) * 8, s));
     PH_HIP(hipStreamSynchronize(s));  // the host item list may go away after this
     RlweArgs a{};
     a.items = d_items;
@@ -839,7 +871,7 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     a.m = d_m;
     a.scratch = scratch;
     hipError_t e = launch_rlwe_encrypt(a, grid, s);
-    (void)hipFreeAsync(scratch, s);
+    if (scratch) (void)hipFreeAsync(scratch, s);
     (void)hipFreeAsync(d_items, s);
     return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
 }

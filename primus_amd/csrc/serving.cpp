@@ -1,0 +1,523 @@
+// serving.cpp -- continuous-batching request loop over the batched transforms.
+//
+// A dispatcher thread turns the pending queue into batches: it admits
+// requests in arrival order while their device footprint fits the HBM budget
+// and a stream slot is free, groups them by kind and shape (one launch per
+// poly-multiply shape, one persistent launch for all RLWE encryptions, one for
+// all external products), stages inputs through pinned memory and enqueues
+// copies and kernels on the slot's stream.  A completion thread retires
+// batches in order, copies results to the callers' buffers and frees the
+// budget.  Requests that arrive while batches are on the GPU join the next
+// batch as soon as a slot and memory free up; batches on different slots
+// overlap on the device.  Tables and keys are built once and stay resident.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <random>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "../../include/primus_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint64_t kAlign = 256;
+
+uint64_t round_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+struct Ticket {
+    ph_serve_request req;
+    Clock::time_point arrival, done_at;
+    int status = -1;  // -1 pending, else a PH_* code
+    uint64_t bytes = 0;
+};
+
+struct Part {          // one request's place in a batch
+    uint64_t ticket;
+    uint64_t in0, in1, out, small;  // arena offsets (bytes)
+    uint64_t out_bytes;
+};
+
+struct Batch {
+    std::vector<Part> parts;
+    uint64_t arena_bytes = 0, budget_bytes = 0;
+    int slot = -1;
+    hipEvent_t done = nullptr;
+    int status = PH_OK;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    void *pinned = nullptr;
+    uint64_t pinned_cap = 0;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct ph_server {
+    ph_server_config cfg{};
+    std::vector<uint64_t> moduli;
+    std::vector<ph_ntt_table *> tables;                 // [(log_n - min) * max_limbs + i]
+    std::map<std::pair<uint32_t, uint32_t>, ph_dcrt_table *> dcrt;  // (log_n, limbs) -> view
+    ph_rlwe_ctx *rlwe = nullptr;
+    uint32_t rlwe_min = 0, rlwe_max = 0, rlwe_limbs = 0;
+    std::vector<Slot> slots;
+
+    std::mutex mu;
+    std::condition_variable cv_pending, cv_done, cv_slot;
+    std::deque<uint64_t> pending;
+    // a deque: submissions append while the dispatcher and completer hold
+    // references to earlier tickets outside the lock
+    std::deque<Ticket> tickets;
+    std::deque<Batch> inflight;
+    uint64_t inflight_bytes = 0;
+    bool stopping = false;
+    std::thread dispatcher, completer;
+
+    ph_server_stats stats{};
+    Clock::time_point started;
+};
+
+namespace {
+
+// Device bytes a request holds while in flight (inputs, outputs, scratch).
+uint64_t request_bytes(const ph_serve_request &r) {
+    const uint64_t n = 1ull << r.log_n;
+    switch (r.kind) {
+        case PH_REQ_POLY_MUL: return 3 * round_up(r.limbs * n * 8);
+        case PH_REQ_RLWE_ENCRYPT: return 2 * round_up(r.limbs * n * 8) + 3 * round_up(n);
+        case PH_REQ_EXT_PRODUCT: return 2 * round_up(2 * n * 8);
+    }
+    return 0;
+}
+
+int validate(const ph_server *s, const ph_serve_request &r) {
+    switch (r.kind) {
+        case PH_REQ_POLY_MUL:
+            if (r.log_n < s->cfg.min_log_n || r.log_n > s->cfg.max_log_n || r.limbs == 0 ||
+                r.limbs > s->cfg.max_limbs || !r.in0 || !r.in1 || !r.out)
+                return PH_ERR_INVALID;
+            return PH_OK;
+        case PH_REQ_RLWE_ENCRYPT:
+            if (!s->rlwe || r.log_n < s->rlwe_min || r.log_n > s->rlwe_max || r.limbs == 0 || r.limbs > s->rlwe_limbs ||
+                !r.in0 || !r.s || !r.e || !r.m || !r.out)
+                return PH_ERR_INVALID;
+            return PH_OK;
+        case PH_REQ_EXT_PRODUCT:
+            if (!s->cfg.ext_key || !r.in0 || !r.out || r.log_n != ph_ext_key_log_n(s->cfg.ext_key))
+                return PH_ERR_INVALID;
+            return PH_OK;
+    }
+    return PH_ERR_INVALID;
+}
+
+// RLWE scratch for ring degrees above 2^12: 2^15 words for each of the
+// kernel's resident workgroups (two per compute unit).
+uint64_t rlwe_scratch_bytes() {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) cus = 256;
+    return (uint64_t)std::max(cus, 1) * 2 * (1ull << 15) * 8;
+}
+
+int ensure_pinned(Slot &sl, uint64_t bytes) {
+    if (sl.pinned_cap >= bytes) return PH_OK;
+    if (sl.pinned) (void)hipHostFree(sl.pinned);
+    sl.pinned = nullptr;
+    sl.pinned_cap = 0;
+    uint64_t cap = std::max<uint64_t>(bytes, 64ull << 20);
+    if (hipHostMalloc(&sl.pinned, cap, hipHostMallocDefault) != hipSuccess) return PH_ERR_NO_MEMORY;
+    sl.pinned_cap = cap;
+    return PH_OK;
+}
+
+// Enqueues one batch on its slot's stream.  Called by the dispatcher without
+// the lock held; the batch's tickets are not touched by anyone else meanwhile.
+//
+// Arena layout (staging mirrors it byte for byte): one [count][limbs][N] block
+// pair per poly-multiply shape (a, then b; the product replaces a), the RLWE
+// requests' a_hat / b_hat / s / e / m regions with shared per-request offsets,
+// and the external products' input and output blocks.  Each group is one
+// launch.
+int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
+    Slot &sl = s->slots[b.slot];
+    hipStream_t st = sl.stream;
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        uint64_t o = off;
+        off += round_up(bytes);
+        return o;
+    };
+    // poly-multiply groups
+    std::map<std::pair<uint32_t, uint32_t>, std::vector<size_t>> groups;
+    std::vector<size_t> rlwe, ext;
+    for (size_t i = 0; i < tk.size(); ++i) {
+        const ph_serve_request &r = tk[i]->req;
+        if (r.kind == PH_REQ_POLY_MUL) groups[{r.log_n, r.limbs}].push_back(i);
+        else if (r.kind == PH_REQ_RLWE_ENCRYPT) rlwe.push_back(i);
+        else ext.push_back(i);
+    }
+    struct GroupAt { uint64_t a, b, count; ph_dcrt_table *t; };
+    std::vector<GroupAt> gat;
+    for (auto &g : groups) {
+        const uint64_t row = (uint64_t)g.first.second << g.first.first;  // words per request
+        GroupAt x{take(row * 8 * g.second.size()), take(row * 8 * g.second.size()), g.second.size(), s->dcrt[g.first]};
+        for (size_t k = 0; k < g.second.size(); ++k) {
+            Part &p = b.parts[g.second[k]];
+            p.in0 = x.a + k * row * 8;
+            p.in1 = x.b + k * row * 8;
+            p.out = p.in0;
+            p.out_bytes = row * 8;
+        }
+        gat.push_back(x);
+    }
+    // RLWE regions
+    uint64_t a_words = 0, small = 0;
+    std::vector<ph_rlwe_request> rq;
+    for (size_t i : rlwe) {
+        const ph_serve_request &r = tk[i]->req;
+        ph_rlwe_request x;
+        x.log_n = r.log_n;
+        x.limbs = r.limbs;
+        x.a_offset = a_words;
+        x.small_offset = small;
+        rq.push_back(x);
+        a_words += (uint64_t)r.limbs << r.log_n;
+        small += 1ull << r.log_n;
+    }
+    const uint64_t ra = take(a_words * 8), rout = take(a_words * 8), rs = take(small), re = take(small), rm = take(small);
+    for (size_t k = 0; k < rlwe.size(); ++k) {
+        Part &p = b.parts[rlwe[k]];
+        p.in0 = ra + rq[k].a_offset * 8;
+        p.out = rout + rq[k].a_offset * 8;
+        p.small = rq[k].small_offset;
+        p.out_bytes = ((uint64_t)rq[k].limbs << rq[k].log_n) * 8;
+    }
+    // external products (all with the server key's ring degree)
+    uint64_t ext_row = 0;
+    if (!ext.empty()) ext_row = 2ull * (1ull << tk[ext[0]]->req.log_n) * 8;
+    const uint64_t xin = take(ext_row * ext.size()), xout = take(ext_row * ext.size());
+    for (size_t k = 0; k < ext.size(); ++k) {
+        Part &p = b.parts[ext[k]];
+        p.in0 = xin + k * ext_row;
+        p.out = xout + k * ext_row;
+        p.out_bytes = ext_row;
+    }
+    b.arena_bytes = off;
+
+    if (ensure_pinned(sl, off) != PH_OK) return PH_ERR_NO_MEMORY;
+    char *host = (char *)sl.pinned;
+    for (size_t i = 0; i < tk.size(); ++i) {
+        const ph_serve_request &r = tk[i]->req;
+        const Part &p = b.parts[i];
+        const uint64_t n = 1ull << r.log_n;
+        std::memcpy(host + p.in0, r.in0, p.out_bytes);
+        if (r.kind == PH_REQ_POLY_MUL) std::memcpy(host + p.in1, r.in1, p.out_bytes);
+        if (r.kind == PH_REQ_RLWE_ENCRYPT) {
+            std::memcpy(host + rs + p.small, r.s, n);
+            std::memcpy(host + re + p.small, r.e, n);
+            std::memcpy(host + rm + p.small, r.m, n);
+        }
+    }
+    void *arena = nullptr;
+    if (hipMallocAsync(&arena, std::max<uint64_t>(off, kAlign), st) != hipSuccess) return PH_ERR_NO_MEMORY;
+    char *dev = (char *)arena;
+    int rc = PH_OK;
+    if (hipMemcpyAsync(dev, host, off, hipMemcpyHostToDevice, st) != hipSuccess) rc = PH_ERR_DEVICE;
+    for (const GroupAt &g : gat) {
+        if (rc != PH_OK) break;
+        const uint32_t L = (uint32_t)ph_dcrt_moduli_count(g.t);
+        rc = ph_dcrt_negacyclic_mul_device(g.t, (uint64_t *)(dev + g.a), (const uint64_t *)(dev + g.b), g.count * L, st);
+        ++s->stats.launches;
+    }
+    if (rc == PH_OK && !rlwe.empty()) {
+        rc = ph_rlwe_encrypt_device(s->rlwe, rq.data(), (uint32_t)rq.size(), (const uint64_t *)(dev + ra),
+                                    (const int8_t *)(dev + rs), (const int8_t *)(dev + re), (const uint8_t *)(dev + rm),
+                                    (uint64_t *)(dev + rout), st);
+        ++s->stats.launches;
+    }
+    if (rc == PH_OK && !ext.empty()) {
+        rc = ph_ext_product_device(s->cfg.ext_key, (const uint64_t *)(dev + xin), (uint64_t *)(dev + xout), ext.size(),
+                                   0, st);
+        ++s->stats.launches;
+    }
+    // results back to the staging area, then the event the completer waits on
+    for (size_t i = 0; i < tk.size() && rc == PH_OK; ++i) {
+        const Part &p = b.parts[i];
+        if (hipMemcpyAsync(host + p.out, dev + p.out, p.out_bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = PH_ERR_DEVICE;
+    }
+    (void)hipFreeAsync(arena, st);
+    if (hipEventRecord(b.done, st) != hipSuccess && rc == PH_OK) rc = PH_ERR_DEVICE;
+    return rc;
+}
+
+void dispatcher_loop(ph_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        s->cv_pending.wait(lk, [&] { return s->stopping || !s->pending.empty(); });
+        if (s->pending.empty() && s->stopping) return;
+        // a free slot
+        int slot = -1;
+        s->cv_slot.wait(lk, [&] {
+            for (size_t i = 0; i < s->slots.size(); ++i)
+                if (!s->slots[i].busy) return true;
+            return false;
+        });
+        for (size_t i = 0; i < s->slots.size(); ++i)
+            if (!s->slots[i].busy) {
+                slot = (int)i;
+                break;
+            }
+        // admit in arrival order while the budget allows
+        Batch b;
+        std::vector<Ticket *> tk;
+        bool has_rlwe_big = false;
+        while (!s->pending.empty() && tk.size() < s->cfg.max_batch) {
+            Ticket &t = s->tickets[s->pending.front()];
+            uint64_t need = t.bytes;
+            if (t.req.kind == PH_REQ_RLWE_ENCRYPT && t.req.log_n > 12 && !has_rlwe_big) need += rlwe_scratch_bytes();
+            if (s->inflight_bytes + b.budget_bytes + need > s->cfg.hbm_budget_bytes && !tk.empty()) break;
+            if (s->inflight_bytes + b.budget_bytes + need > s->cfg.hbm_budget_bytes && tk.empty() &&
+                !s->inflight.empty())
+                break;  // wait for memory to free up
+            if (t.req.kind == PH_REQ_RLWE_ENCRYPT && t.req.log_n > 12) has_rlwe_big = true;
+            b.budget_bytes += need;
+            Part p{};
+            p.ticket = s->pending.front();
+            b.parts.push_back(p);
+            tk.push_back(&t);
+            s->pending.pop_front();
+        }
+        if (tk.empty()) {
+            s->cv_done.wait(lk);  // memory is held by in-flight batches
+            continue;
+        }
+        b.slot = slot;
+        s->slots[slot].busy = true;
+        s->inflight_bytes += b.budget_bytes;
+        s->stats.peak_inflight_bytes = std::max(s->stats.peak_inflight_bytes, s->inflight_bytes);
+        (void)hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+        lk.unlock();
+        b.status = launch_batch(s, b, tk);
+        lk.lock();
+        s->inflight.push_back(std::move(b));
+        ++s->stats.batches;
+        s->cv_done.notify_all();  // wakes the completer
+    }
+}
+
+void completer_loop(ph_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        s->cv_done.wait(lk, [&] { return !s->inflight.empty() || (s->stopping && s->pending.empty()); });
+        if (s->inflight.empty()) {
+            if (s->stopping && s->pending.empty()) return;
+            continue;
+        }
+        Batch &b = s->inflight.front();
+        hipEvent_t ev = b.done;
+        lk.unlock();
+        int st = b.status;
+        if (hipEventSynchronize(ev) != hipSuccess && st == PH_OK) st = PH_ERR_DEVICE;
+        // results from the pinned staging area to the callers' buffers
+        Slot &sl = s->slots[b.slot];
+        for (const Part &p : b.parts) {
+            Ticket &t = s->tickets[p.ticket];
+            if (st == PH_OK) std::memcpy(t.req.out, (char *)sl.pinned + p.out, p.out_bytes);
+        }
+        const auto now = Clock::now();
+        lk.lock();
+        for (const Part &p : b.parts) {
+            Ticket &t = s->tickets[p.ticket];
+            t.done_at = now;
+            t.status = st;
+            ++s->stats.completed;
+        }
+        s->inflight_bytes -= b.budget_bytes;
+        sl.busy = false;
+        (void)hipEventDestroy(ev);
+        s->inflight.pop_front();
+        s->cv_done.notify_all();
+        s->cv_slot.notify_all();
+    }
+}
+
+void destroy(ph_server *s) {
+    if (!s) return;
+    for (auto &kv : s->dcrt) ph_dcrt_table_free(kv.second);
+    for (ph_ntt_table *t : s->tables) ph_ntt_table_free(t);
+    if (s->rlwe) ph_rlwe_ctx_free(s->rlwe);
+    for (Slot &sl : s->slots) {
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.pinned) (void)hipHostFree(sl.pinned);
+    }
+    delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ph_server_new(const ph_server_config *cfg, ph_server **out) {
+    if (!cfg || !out || !cfg->moduli || cfg->max_limbs == 0 || cfg->min_log_n > cfg->max_log_n ||
+        cfg->streams == 0 || cfg->max_batch == 0)
+        return PH_ERR_INVALID;
+    *out = nullptr;
+    ph_server *s = new (std::nothrow) ph_server();
+    if (!s) return PH_ERR_NO_MEMORY;
+    s->cfg = *cfg;
+    const uint32_t degrees = cfg->max_log_n - cfg->min_log_n + 1;
+    s->moduli.assign(cfg->moduli, cfg->moduli + (size_t)degrees * cfg->max_limbs);
+    s->cfg.moduli = s->moduli.data();
+    int rc = PH_OK;
+    for (uint32_t d = 0; d < degrees && rc == PH_OK; ++d)
+        for (uint32_t i = 0; i < cfg->max_limbs && rc == PH_OK; ++i) {
+            ph_ntt_table *t = nullptr;
+            rc = ph_ntt_table_new(cfg->min_log_n + d, s->moduli[(size_t)d * cfg->max_limbs + i], &t);
+            if (rc == PH_OK) s->tables.push_back(t);
+        }
+    for (uint32_t d = 0; d < degrees && rc == PH_OK; ++d)
+        for (uint32_t L = 1; L <= cfg->max_limbs && rc == PH_OK; ++L) {
+            ph_dcrt_table *v = nullptr;
+            rc = ph_dcrt_table_view(&s->tables[(size_t)d * cfg->max_limbs], L, &v);
+            if (rc == PH_OK) s->dcrt[{cfg->min_log_n + d, L}] = v;
+        }
+    // RLWE tables over the part of the grid its kernel covers (2^10 .. 2^15, <= 12 limbs)
+    s->rlwe_min = std::max(cfg->min_log_n, 10u);
+    s->rlwe_max = std::min(cfg->max_log_n, 15u);
+    s->rlwe_limbs = std::min(cfg->max_limbs, 12u);
+    if (rc == PH_OK && s->rlwe_min <= s->rlwe_max) {
+        std::vector<uint64_t> sub;
+        for (uint32_t ln = s->rlwe_min; ln <= s->rlwe_max; ++ln)
+            for (uint32_t i = 0; i < s->rlwe_limbs; ++i)
+                sub.push_back(s->moduli[(size_t)(ln - cfg->min_log_n) * cfg->max_limbs + i]);
+        rc = ph_rlwe_ctx_new(s->rlwe_min, s->rlwe_max, s->rlwe_limbs, sub.data(), &s->rlwe);
+    }
+    s->slots.resize(cfg->streams);
+    for (Slot &sl : s->slots)
+        if (rc == PH_OK && hipStreamCreate(&sl.stream) != hipSuccess) rc = PH_ERR_DEVICE;
+    if (rc != PH_OK) {
+        destroy(s);
+        return rc;
+    }
+    s->started = Clock::now();
+    s->dispatcher = std::thread(dispatcher_loop, s);
+    s->completer = std::thread(completer_loop, s);
+    *out = s;
+    return PH_OK;
+}
+
+int ph_server_submit(ph_server *s, const ph_serve_request *req, uint64_t *ticket) {
+    if (!s || !req) return PH_ERR_INVALID;
+    int rc = validate(s, *req);
+    if (rc != PH_OK) return rc;
+    const uint64_t bytes = request_bytes(*req);
+    if (bytes > s->cfg.hbm_budget_bytes) return PH_ERR_NO_MEMORY;  // can never be admitted
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return PH_ERR_INVALID;
+    Ticket t;
+    t.req = *req;
+    t.arrival = Clock::now();
+    t.bytes = bytes;
+    s->tickets.push_back(t);
+    const uint64_t id = s->tickets.size() - 1;
+    s->pending.push_back(id);
+    if (ticket) *ticket = id;
+    s->cv_pending.notify_one();
+    return PH_OK;
+}
+
+int ph_server_wait(ph_server *s, uint64_t ticket, int *status) {
+    if (!s) return PH_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (ticket >= s->tickets.size()) return PH_ERR_INVALID;
+    s->cv_done.wait(lk, [&] { return s->tickets[ticket].status >= 0; });
+    if (status) *status = s->tickets[ticket].status;
+    return PH_OK;
+}
+
+int ph_server_latency(ph_server *s, uint64_t ticket, double *seconds) {
+    if (!s || !seconds) return PH_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (ticket >= s->tickets.size() || s->tickets[ticket].status < 0) return PH_ERR_INVALID;
+    const Ticket &t = s->tickets[ticket];
+    *seconds = std::chrono::duration<double>(t.done_at - t.arrival).count();
+    return PH_OK;
+}
+
+int ph_server_drain(ph_server *s) {
+    if (!s) return PH_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->inflight.empty(); });
+    return PH_OK;
+}
+
+int ph_server_stats_get(ph_server *s, ph_server_stats *out) {
+    if (!s || !out) return PH_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    return PH_OK;
+}
+
+void ph_server_free(ph_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_pending.notify_all();
+    s->cv_done.notify_all();
+    s->cv_slot.notify_all();
+    if (s->dispatcher.joinable()) s->dispatcher.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+    }
+    s->cv_done.notify_all();
+    if (s->completer.joinable()) s->completer.join();
+    destroy(s);
+}
+
+int ph_server_run_poisson(ph_server *s, const ph_serve_request *reqs, size_t n, double rate_per_s, uint64_t seed,
+                          double *latency_s, double *elapsed_s) {
+    if (!s || (!reqs && n) || rate_per_s <= 0) return PH_ERR_INVALID;
+    std::mt19937_64 rng(seed);
+    std::exponential_distribution<double> gap(rate_per_s);
+    std::vector<uint64_t> ids(n);
+    const auto t0 = Clock::now();
+    double at = 0;
+    for (size_t i = 0; i < n; ++i) {
+        at += gap(rng);
+        const auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(at));
+        // sleep most of the gap, spin the last stretch for sub-100us arrival accuracy
+        auto now = Clock::now();
+        if (due - now > std::chrono::microseconds(200)) std::this_thread::sleep_until(due - std::chrono::microseconds(100));
+        while (Clock::now() < due) {
+        }
+        int rc = ph_server_submit(s, &reqs[i], &ids[i]);
+        if (rc != PH_OK) return rc;
+    }
+    int rc = ph_server_drain(s);
+    const auto t1 = Clock::now();
+    if (elapsed_s) *elapsed_s = std::chrono::duration<double>(t1 - t0).count();
+    for (size_t i = 0; i < n && rc == PH_OK; ++i) {
+        int st = 0;
+        ph_server_wait(s, ids[i], &st);
+        if (st != PH_OK) rc = st;
+        if (latency_s) ph_server_latency(s, ids[i], &latency_s[i]);
+    }
+    return rc;
+}
+
+}  // extern "C"
