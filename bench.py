@@ -10,7 +10,8 @@ batch (384 MiB) keep the working set larger than the 256 MiB Infinity Cache.
 
 Prints one JSON line: NTT/s (value), HBM GB/s, p50/p99 batch latency, the
 plain one-launch-per-limb-polynomial GPU path, the HBM roofline of the fused
-kernel and the CPU oracle timed on this host.
+kernel, the CPU oracle timed on this host, and secondary lines for the other
+batched forms (plain N=1024 products, uneven RLWE encryption, external product).
 """
 import argparse
 import ctypes
@@ -33,16 +34,18 @@ NTT_PER_STEP = 2 * COUNT
 BYTES_PER_STEP = COUNT * N * 8 * 2  # every coefficient read once and written once
 ROTATE = 12
 HBM_PEAK_GBPS = 8000.0
-SEED = 0x5052494D5553 + 1
+SEED = 0x5052494D5553
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fwd_mul_inv_f64_n4096_l4.json")
 
 
-def workload(rng):
-    import primus_amd as pa
-
-    moduli = pa.ntt_primes_below(50, 2 * N, LIMBS)
-    polys = np.stack([rng.integers(0, moduli[i % LIMBS], size=N, dtype=np.uint64) for i in range(COUNT)])
-    operand = np.stack([rng.integers(0, q, size=N, dtype=np.uint64) for q in moduli])
-    return moduli, polys, operand
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
 
 
 def cpu_baseline(moduli, polys, operand, budget_s=8.0):
@@ -58,62 +61,330 @@ def cpu_baseline(moduli, polys, operand, budget_s=8.0):
         O.fwd_mul_inv_batch(tables, sample, operand, opq, threads=threads)
         reps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 50:
+        if el >= budget_s or reps >= 400:
             break
-    # single thread on a quarter of the batch
     one = polys[:COUNT // 4].copy()
     t1 = time.perf_counter()
     O.fwd_mul_inv_batch(tables, one, operand, opq, threads=1)
     el1 = time.perf_counter() - t1
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return {
         "value": reps * NTT_PER_STEP / el,
         "unit": "NTT/s",
         "cores": threads,
         "kind": "port",
-        "sample": "%d full batches (1024 limb-polys, fwd+Shoup mul+inv) on %d threads in %.1f s; "
-                  "single thread: %.0f NTT/s" % (reps, threads, el, (COUNT // 4) * 2 / el1),
+        "sample": "%d full batches (1024 limb-polys of N=4096, fwd + Shoup mul + inv) on %d threads in %.1f s; "
+                  "single thread %.0f NTT/s" % (reps, threads, el, (COUNT // 4) * 2 / el1),
         "single_thread_value": (COUNT // 4) * 2 / el1,
-        "cpu_model": cpu_model,
+        "cpu_model": cpu_model(),
         "nproc": os.cpu_count(),
     }
+
+
+class Timer:
+    """HIP events on the stream the kernels are launched on (torch's view of it).
+
+    run() times K steps twice: once bracketed by two events only (throughput:
+    nothing is queued between the launches), then once with an event after
+    every step (per-step latency for p50/p99)."""
+
+    def __init__(self, torch, stream):
+        self.torch, self.stream = torch, stream
+
+    def _bracket(self, step, K):
+        e0, e1 = (self.torch.cuda.Event(enable_timing=True) for _ in range(2))
+        self.torch.cuda.synchronize()
+        e0.record(self.stream)
+        for i in range(K):
+            step(i)
+        e1.record(self.stream)
+        self.stream.synchronize()
+        self.torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    def run(self, step, K, W, latency=True):
+        for i in range(W):
+            step(i)
+        self.stream.synchronize()
+        total = self._bracket(step, K)
+        if not latency:
+            return None, total
+        evs = [self.torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+        self.torch.cuda.synchronize()
+        evs[0].record(self.stream)
+        for i in range(K):
+            step(i)
+            evs[i + 1].record(self.stream)
+        self.stream.synchronize()
+        self.torch.cuda.synchronize()
+        per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(K)])
+        return per, total
+
+    def run_graph(self, step, per_graph, K, W):
+        """Capture per_graph consecutive steps into one HIP graph and replay it
+        K // per_graph times; returns ms per step."""
+        torch = self.torch
+        g = torch.cuda.CUDAGraph()
+        self.stream.synchronize()
+        with torch.cuda.graph(g, stream=self.stream):
+            for i in range(per_graph):
+                step(i)
+        self.stream.synchronize()
+        reps = max(K // per_graph, 1)
+        with torch.cuda.stream(self.stream):
+            for _ in range(max(W // per_graph, 1)):
+                g.replay()
+            total = self._bracket(lambda i: g.replay(), reps)
+        return total / (reps * per_graph)
+
+
+def secondary_plain_products(pa, timer, sp):
+    """Plain N=1024 products: 64 pairs, q the largest prime < 2^31 with q = 1 mod 2048."""
+    from oracle import oracle as O
+
+    q, log_n, pairs = 2147473409, 10, 64
+    n = 1 << log_n
+    rng = np.random.default_rng(SEED)
+    a = rng.integers(0, q, size=(pairs, n), dtype=np.uint64)
+    b = rng.integers(0, q, size=(pairs, n), dtype=np.uint64)
+    tab = pa.UintCrtNttTable.new(log_n, [q])
+    da, db = pa.DeviceBuffer.from_array(a), pa.DeviceBuffer.from_array(b)
+    L = pa.lib()
+    pa.negacyclic_mul_device(tab, da.ptr, db.ptr, pairs, stream=sp)
+    pa.synchronize(sp)
+    got = da.download((pairs, n))
+    t = O.Table(log_n, q)
+    ok = all(np.array_equal(got[i], O.schoolbook(a[i], b[i], q)) for i in range(2))
+
+    def batched(i):
+        L.ph_dcrt_negacyclic_mul_device(tab.handle, da.ptr, db.ptr, pairs, sp)
+
+    def plain(i):
+        for k in range(pairs):
+            L.ph_dcrt_negacyclic_mul_device(tab.handle, ctypes.c_void_p(da.ptr.value + k * n * 8),
+                                            ctypes.c_void_p(db.ptr.value + k * n * 8), 1, sp)
+
+    per, tot = timer.run(batched, 1000, 50)
+    pper, ptot = timer.run(plain, 20, 2)
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < 1.0:
+        for i in range(pairs):
+            t.inverse_transform(O.pointwise_mul(t.transform(a[i]), t.transform(b[i]), q))
+        reps += 1
+    cpu_s = (time.perf_counter() - t0) / reps
+    return {"workload": "64 pairs N=1024, q=2147473409: fwd a, fwd b, pointwise, inv (one launch)",
+            "parity_vs_schoolbook": ok, "ms_per_batch": tot / 1000, "p50_ms": float(np.percentile(per, 50)),
+            "p99_ms": float(np.percentile(per, 99)), "ntt_per_s": 3 * pairs / (tot / 1000 * 1e-3),
+            "plain_ms_per_batch": ptot / 20, "plain_p99_ms": float(np.percentile(pper, 99)),
+            "batched_vs_plain": (ptot / 20) / (tot / 1000),
+            "cpu_single_thread_ms_per_batch": cpu_s * 1e3, "arith": tab.arith()}
+
+
+def secondary_rlwe(pa, timer, sp):
+    """Uneven RLWE encryption: 1024 requests, N ~ U{2^10..2^15}, L ~ U{1..12}, 59-bit primes."""
+    from primus_amd.fhe import RlweContext, rlwe_layout
+
+    rng = np.random.default_rng(SEED + 3)
+    shapes = [(int(rng.integers(10, 16)), int(rng.integers(1, 13))) for _ in range(1024)]
+    grid = {ln: pa.ntt_primes_below(59, 2 << ln, 12) for ln in range(10, 16)}
+    ctx = RlweContext(10, 15, 12, grid)
+    reqs, a_words, small = rlwe_layout(shapes)
+    a = np.empty(a_words, dtype=np.uint64)
+    for (ln, L, ao, so) in reqs:
+        for i in range(L):
+            a[ao + (i << ln): ao + ((i + 1) << ln)] = rng.integers(0, grid[ln][i], size=1 << ln, dtype=np.uint64)
+    s = rng.integers(-1, 2, size=small).astype(np.int8)
+    e = np.clip(np.rint(rng.normal(0, 3.2, size=small)), -19, 19).astype(np.int8)
+    m = rng.integers(0, 256, size=small).astype(np.uint8)
+
+    def as_words(v):
+        raw = np.ascontiguousarray(v).tobytes()
+        return np.frombuffer(raw.ljust((len(raw) + 7) // 8 * 8, b"\0"), dtype=np.uint64)
+
+    da, dout = pa.DeviceBuffer.from_array(a), pa.DeviceBuffer(a_words)
+    ds, de, dm = (pa.DeviceBuffer.from_array(as_words(v)) for v in (s, e, m))
+
+    def step(i):
+        ctx.encrypt_device(reqs, da.ptr, ds.ptr, de.ptr, dm.ptr, dout.ptr, stream=sp)
+
+    per, tot = timer.run(step, 100, 5)
+    ms = tot / 100
+    ntts = 2 * sum(L for _, L in shapes)
+    return {"workload": "1024 RLWE encryptions, N~U{2^10..2^15}, L~U{1..12}, one persistent launch",
+            "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+            "ntt_per_s": ntts / (ms * 1e-3), "coeff_limbs": int(a_words),
+            "algorithmic_gbps": (16 * a_words + 3 * small) / (ms * 1e-3) / 1e9}
+
+
+def secondary_ext_product(pa, timer, sp):
+    """External product: 2048 products, N=1024, q=4294957057, base 2^8, l=3."""
+    from primus_amd.fhe import ExtKey
+
+    q, log_n, count = 4294957057, 10, 2048
+    n = 1 << log_n
+    rng = np.random.default_rng(SEED + 4)
+    key = ExtKey(pa.UintNttTable.new(log_n, q), 8, 3, rng.integers(0, q, size=(2, 3, 2, n), dtype=np.uint64))
+    x = rng.integers(0, q, size=(count, 2, n), dtype=np.uint64)
+    din, dout = pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(count * 2 * n)
+
+    def step(i):
+        key.product_device(din.ptr, dout.ptr, count, stream=sp)
+
+    per, tot = timer.run(step, 1000, 20)
+    ms = tot / 1000
+    return {"workload": "2048 RLWE x RGSW external products, N=1024, q=4294957057, B=2^8, l=3, one launch",
+            "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+            "ntt_per_s": 8 * count / (ms * 1e-3), "algorithmic_gbps": 4 * n * 8 * count / (ms * 1e-3) / 1e9,
+            "arith": "f64"}
+
+
+def serving_pool(pa, grid, ext_q, pool, seed):
+    """Distinct requests of the serving mix: 50% negacyclic products, 30% RLWE
+    encryptions (N ~ U{2^10..2^16}, L ~ U{1..16}, 59-bit primes) and 20%
+    external products (N = 1024, q = 4294957057)."""
+    from primus_amd import serving as sv
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(pool):
+        u = rng.random()
+        if u < 0.8:
+            ln, L = int(rng.integers(10, 17)), int(rng.integers(1, 17))
+            n = 1 << ln
+            a = np.concatenate([rng.integers(0, grid[ln][j], size=n, dtype=np.uint64) for j in range(L)])
+            if u < 0.5:
+                b = np.concatenate([rng.integers(0, grid[ln][j], size=n, dtype=np.uint64) for j in range(L)])
+                reqs.append(sv.Request(sv.POLY_MUL, ln, L, in0=a, in1=b))
+            else:
+                s = rng.integers(-1, 2, size=n).astype(np.int8)
+                e = np.clip(np.rint(rng.normal(0, 3.2, size=n)), -19, 19).astype(np.int8)
+                m = rng.integers(0, 256, size=n).astype(np.uint8)
+                reqs.append(sv.Request(sv.RLWE_ENCRYPT, ln, L, in0=a, s=s, e=e, m=m))
+        else:
+            reqs.append(sv.Request(sv.EXT_PRODUCT, 10, 1, in0=rng.integers(0, ext_q, size=2048, dtype=np.uint64)))
+    return reqs
+
+
+def serving_main(args):
+    """Serving stream: Poisson arrivals at the offered rates, continuous batching
+    against the plain path (one request at a time, one launch per
+    limb-polynomial), both through the same request loop and HBM budget."""
+    import primus_amd as pa
+    from oracle import oracle as O
+    from primus_amd import serving as sv
+    from primus_amd.fhe import ExtKey
+
+    ext_q = 4294957057
+    grid = {ln: pa.ntt_primes_below(59, 2 << ln, 16) for ln in range(10, 17)}
+    rng = np.random.default_rng(SEED + 5)
+    key_np = rng.integers(0, ext_q, size=(2, 3, 2, 1024), dtype=np.uint64)
+    key = ExtKey(pa.UintNttTable.new(10, ext_q), 8, 3, key_np)
+    pool = serving_pool(pa, grid, ext_q, args.serving_pool, SEED + 6)
+    budget = 64 << 30
+
+    def run(srv, rate, count, seed):
+        reqs = [pool[i % len(pool)] for i in range(count)]
+        before = srv.stats()
+        lat, elapsed = srv.run_poisson(reqs, rate, seed=seed)
+        after = srv.stats()
+        return {"offered_per_s": rate, "requests": count, "elapsed_s": elapsed,
+                "throughput_per_s": count / elapsed, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "batches": after["batches"] - before["batches"],
+                "launches": after["launches"] - before["launches"],
+                "peak_inflight_bytes": after["peak_inflight_bytes"]}
+
+    batched = sv.Server(grid, 10, 16, 16, ext_key=key, hbm_budget_bytes=budget, max_batch=args.serving_max_batch,
+                        streams=4)
+    plain = sv.Server(grid, 10, 16, 16, ext_key=key, hbm_budget_bytes=budget, max_batch=1, streams=1,
+                      per_poly_launches=True)
+    # parity gate: the first requests of each kind against the oracle, through both servers
+    checked = 0
+    for r in pool:
+        if r.log_n != 10 or checked >= 3:
+            continue
+        for srv in (batched, plain):
+            r.out[:] = 0
+            srv.wait(srv.submit(r))
+            n = 1 << r.log_n
+            if r.kind == sv.POLY_MUL:
+                want = np.concatenate([O.schoolbook(r.in0[j * n:(j + 1) * n], r.in1[j * n:(j + 1) * n], grid[10][j])
+                                       for j in range(r.limbs)])
+            elif r.kind == sv.RLWE_ENCRYPT:
+                mods = grid[10][:r.limbs]
+                want = np.concatenate([O.rlwe_encrypt(O.Table(10, q), r.s, r.e, r.m, O.rlwe_delta(mods, q),
+                                                      r.in0[j * n:(j + 1) * n]) for j, q in enumerate(mods)])
+            else:
+                want = O.ext_product(O.Table(10, ext_q), O.Basis.new(ext_q, 8, 3), key_np,
+                                     r.in0.reshape(2, 1024)).ravel()
+            if not np.array_equal(r.out, want):
+                print("bench: serving output differs from the oracle; refusing to time", file=sys.stderr)
+                return 1
+        checked += 1
+
+    out = {"metric": "serving stream: p50/p99 latency and throughput, continuous batching vs plain per-request launches",
+           "workload": "Poisson arrivals; mix 50% negacyclic product / 30% RLWE encryption (N~U{2^10..2^16}, "
+                       "L~U{1..16}, 59-bit primes) / 20% external product (N=1024, q=4294957057, B=2^8, l=3); "
+                       "host buffers in and out (PCIe included); 64 GiB HBM admission budget",
+           "pool_distinct_requests": len(pool), "max_batch": args.serving_max_batch, "rates": []}
+    for k, rate in enumerate(args.serving_rates):
+        count = int(min(args.serving_requests, rate * args.serving_seconds))
+        b = run(batched, rate, count, seed=11 + k)
+        p = run(plain, rate, int(min(count, args.serving_plain_requests)), seed=11 + k)
+        out["rates"].append({"batched": b, "plain": p})
+        print(json.dumps({"progress": "rate done", "rate": rate}), file=sys.stderr, flush=True)
+    # saturation: everything offered at once, the highest throughput each path reaches
+    count = int(min(args.serving_requests, 20000))
+    b = run(batched, 1e7, count, seed=99)
+    p = run(plain, 1e7, int(min(count, args.serving_plain_requests)), seed=99)
+    out["saturation"] = {"batched": b, "plain": p,
+                         "throughput_ratio": b["throughput_per_s"] / p["throughput_per_s"]}
+    batched.close()
+    plain.close()
+    print(json.dumps(out))
+    return 0
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="counter runs: only the fused batch kernel (--steps launches) and one 256 MiB "
                          "device-to-device copy as the byte-count calibration")
+    ap.add_argument("--serving", action="store_true",
+                    help="the serving stream benchmark (Poisson arrivals, batched vs plain) instead of the headline")
+    ap.add_argument("--serving-rates", type=float, nargs="+", default=[1000.0, 10000.0, 50000.0])
+    ap.add_argument("--serving-requests", type=int, default=100000)
+    ap.add_argument("--serving-seconds", type=float, default=20.0,
+                    help="cap on offered time per rate: requests = min(--serving-requests, rate * this)")
+    ap.add_argument("--serving-plain-requests", type=int, default=3000)
+    ap.add_argument("--serving-pool", type=int, default=256)
+    ap.add_argument("--serving-max-batch", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         return 2
 
+    if args.serving:
+        return serving_main(args)
+
     import torch
 
     import primus_amd as pa
-    from primus_amd import ntt as pn
-
-    rng = np.random.default_rng(SEED)
-    moduli, polys, operand = workload(rng)
-    table = pa.UintCrtNttTable.new(LOG_N, moduli)
-    op = pa.Operand(table, operand)
+    from oracle import oracle as O
     from primus_amd import tuning
 
-    shape, shape_ms = tuning.autotune(table, "fwd_mul_inv", COUNT, reps=20)
+    rng = np.random.default_rng(SEED + 1)
+    moduli = pa.ntt_primes_below(50, 2 * N, LIMBS)
+    polys = np.stack([rng.integers(0, moduli[i % LIMBS], size=N, dtype=np.uint64) for i in range(COUNT)])
+    operand = np.stack([rng.integers(0, q, size=N, dtype=np.uint64) for q in moduli])
+    table = pa.UintCrtNttTable.new(LOG_N, moduli)
+    op = pa.Operand(table, operand)
 
     dev = torch.device("cuda:0")
     host = torch.from_numpy(polys.view(np.int64))
@@ -124,22 +395,23 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     sp = ctypes.c_void_p(stream.cuda_stream)
     ptrs = [ctypes.c_void_p(bufs[r].data_ptr()) for r in range(ROTATE)]
+    timer = Timer(torch, stream)
+    L = pa.lib()
 
-    def launcher(tab, operand):
+    def launcher(tab, operand_):
         def step(i):
-            pn._check(pa.lib().ph_dcrt_fwd_mul_inv_device(tab.handle, ptrs[i % ROTATE], COUNT, operand.handle, sp))
+            L.ph_dcrt_fwd_mul_inv_device(tab.handle, ptrs[i % ROTATE], COUNT, operand_.handle, sp)
         return step
 
+    shape, shape_ms = tuning.autotune(table, "fwd_mul_inv", COUNT, reps=20)
     step = launcher(table, op)
-
-    # correctness gate on the first limb-polynomials before any timing
-    from oracle import oracle as O
-
-    otabs = [O.Table(LOG_N, q) for q in moduli]
     int_table = pa.UintCrtNttTable.new(LOG_N, moduli, integer_only=True)
     int_op = pa.Operand(int_table, operand)
     int_shape, _ = tuning.autotune(int_table, "fwd_mul_inv", COUNT, reps=10)
     int_step = launcher(int_table, int_op)
+
+    # correctness gate on the first limb-polynomials before any timing
+    otabs = [O.Table(LOG_N, q) for q in moduli]
     for st in (step, int_step):
         st(0)
         stream.synchronize()
@@ -153,66 +425,55 @@ def main():
         bufs[0].copy_(host)
         torch.cuda.synchronize()
 
-    def timed(st, K, W):
-        for i in range(W):
-            st(i)
-        stream.synchronize()
-        evs = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        evs[0].record(stream)
-        for i in range(K):
-            st(i)
-            evs[i + 1].record(stream)
-        stream.synchronize()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(K)])  # ms per launch
-        return per, evs[0].elapsed_time(evs[K]), wall
-
     K = args.steps
     if args.profile:
-        per, total_ms, _ = timed(step, K, args.warmup)
+        _, total = timer.run(step, K, args.warmup, latency=False)
         src = torch.empty(32 << 20, dtype=torch.int64, device=dev)
         dst = torch.empty_like(src)
         src.fill_(1)
         torch.cuda.synchronize()
         dst.copy_(src)  # calibration: 256 MiB read + 256 MiB written
         torch.cuda.synchronize()
-        print(json.dumps({"profile": True, "kernel_ms": float(np.mean(per)), "launches": K + args.warmup,
-                          "calibration_copy_bytes": 2 * src.numel() * 8}))
+        print(json.dumps({"profile": True, "kernel_ms": total / K, "launches": K + args.warmup,
+                          "calibration_copy_bytes": src.numel() * 8}))
         return 0
-    per, total_ms, wall = timed(step, K, args.warmup)
-    ms_per_step = total_ms / K
+
+    t_wall = time.perf_counter()
+    per, total_ms = timer.run(step, K, args.warmup)
+    wall = time.perf_counter() - t_wall
+    stream_ms = total_ms / K
+    # the same K batches as replays of one captured HIP graph of ROTATE launches
+    try:
+        graph_ms, graph_err = timer.run_graph(step, ROTATE, K, args.warmup), None
+    except Exception as exc:  # capture unsupported: report it, time stream launches only
+        graph_ms, graph_err = None, str(exc)[:200]
+    launch = "hipGraph" if graph_ms is not None and graph_ms < stream_ms else "stream"
+    ms_per_step = graph_ms if launch == "hipGraph" else stream_ms
     value = NTT_PER_STEP / (ms_per_step * 1e-3)
-    kernel_ms = float(np.mean(per))
+    kernel_ms = ms_per_step  # timed region / launches: the fused kernel is the only one in it
     achieved = BYTES_PER_STEP / (kernel_ms * 1e-3) / 1e9
-    int_per, int_total, _ = timed(int_step, max(K // 4, 10), args.warmup)
+    int_per, int_total = timer.run(int_step, max(K // 4, 10), args.warmup, latency=False)
     int_ms = int_total / max(K // 4, 10)
 
     # plain path: one launch per limb-polynomial, per-limb single-modulus tables
     limb_tabs = [pa.UintCrtNttTable.new(LOG_N, [q]) for q in moduli]
     limb_ops = [pa.Operand(limb_tabs[l], operand[l]) for l in range(LIMBS)]
-    L = pa.lib()
     stride = N * 8
 
     def plain_step(r):
         base = bufs[r % ROTATE].data_ptr()
         for i in range(COUNT):
             l = i % LIMBS
-            pn._check(L.ph_dcrt_fwd_mul_inv_device(limb_tabs[l].handle, ctypes.c_void_p(base + i * stride), 1,
-                                                   limb_ops[l].handle, sp))
+            L.ph_dcrt_fwd_mul_inv_device(limb_tabs[l].handle, ctypes.c_void_p(base + i * stride), 1,
+                                         limb_ops[l].handle, sp)
 
-    plain_step(0)
-    stream.synchronize()
-    pe = [torch.cuda.Event(enable_timing=True) for _ in range(args.plain_steps + 1)]
-    pe[0].record(stream)
-    for r in range(args.plain_steps):
-        plain_step(r)
-        pe[r + 1].record(stream)
-    stream.synchronize()
-    plain_per = np.array([pe[i].elapsed_time(pe[i + 1]) for i in range(args.plain_steps)])
-    plain_ms = float(np.mean(plain_per))
+    plain_per, plain_total = timer.run(plain_step, args.plain_steps, 1)
+    plain_ms = plain_total / args.plain_steps
+
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("traffic_bytes")
 
     out = {
         "metric": "NTTs/sec + HBM GB/s at N=4096x4 limbs; p50/p99 latency, batched vs plain",
@@ -231,11 +492,15 @@ def main():
             "workload": "batched RNS NTT: 256 polys x 4 limbs, N=4096, fwd NTT + Shoup pointwise + inv NTT, "
                         "one launch per batch",
             "N": N, "limbs": LIMBS, "polys": POLYS, "moduli": moduli, "ntt_per_step": NTT_PER_STEP,
-            "rotating_buffers": ROTATE, "parallelism": "single GPU",
+            "rotating_buffers": ROTATE, "parallelism": "single GPU", "launch": launch,
         },
-        "hbm_gbps": BYTES_PER_STEP * K / (total_ms * 1e-3) / 1e9,
+        "hbm_gbps": BYTES_PER_STEP / (ms_per_step * 1e-3) / 1e9,
+        "stream_launch_ms": stream_ms,
+        "graph_replay_ms": graph_ms,
+        "graph_error": graph_err,
         "p50_ms": float(np.percentile(per, 50)),
         "p99_ms": float(np.percentile(per, 99)),
+        "latency_note": "p50/p99 of K single-batch spans between per-step HIP events on the launch stream",
         "wall_s": wall,
         "plain": {
             "desc": "one launch per limb-polynomial (1024 launches per batch)",
@@ -260,12 +525,18 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "chunk_kernel<F64Arith,12,8,FWD|MUL_SHOUP|INV|INV_LAST> (fused fwd -> mul -> inv)",
             "kernel_ms": kernel_ms,
             "bytes_per_launch": BYTES_PER_STEP,
         },
     }
+    if not args.no_secondary:
+        out["secondary"] = {
+            "plain_n1024": secondary_plain_products(pa, timer, sp),
+            "rlwe_uneven": secondary_rlwe(pa, timer, sp),
+            "ext_product": secondary_ext_product(pa, timer, sp),
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(moduli, polys, operand)
     print(json.dumps(out))

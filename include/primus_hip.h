@@ -221,7 +221,7 @@ typedef struct {
     uint64_t small_offset;
 } ph_rlwe_request;
 
-/* Tables for ring degrees 2^min_log_n .. 2^max_log_n (within 2^10 .. 2^15),
+/* Tables for ring degrees 2^min_log_n .. 2^max_log_n (within 2^10 .. 2^16),
  * limb i of degree 2^l using moduli[(l - min_log_n) * max_limbs + i]. */
 int ph_rlwe_ctx_new(uint32_t min_log_n, uint32_t max_log_n, uint32_t max_limbs, const uint64_t *moduli,
                     ph_rlwe_ctx **out);
@@ -263,6 +263,8 @@ typedef struct {
     uint32_t min_log_n, max_log_n, max_limbs;
     const uint64_t *moduli;    /* [(max_log_n - min_log_n + 1)][max_limbs] limb moduli per ring degree */
     const ph_ext_key *ext_key; /* key of the external-product requests (NULL: none accepted) */
+    uint32_t per_poly_launches; /* 1: one launch per limb-polynomial of a poly-multiply; with
+                                 * max_batch 1 and one stream this is the plain path */
 } ph_server_config;
 
 typedef struct {

@@ -257,3 +257,26 @@ def ext_product(table, basis, key, rlwe, ntt_output=False):
     out = np.zeros(2 * table.n, dtype=np.uint64)
     lib().po_ext_product(table.handle, ctypes.byref(basis), ptr(key), ptr(x), ptr(out), int(bool(ntt_output)))
     return out.reshape(2, table.n)
+
+
+def rlwe_encrypt(table, s, e, m, delta, a_hat):
+    """b_hat = NTT(e + delta*m) + a_hat (.) NTT(s) for one limb (po_rlwe_encrypt)."""
+    L = lib()
+    L.po_rlwe_encrypt.restype = None
+    L.po_rlwe_encrypt.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_uint64, _p64, _p64]
+    out = np.zeros(table.n, dtype=np.uint64)
+    a_hat = np.ascontiguousarray(a_hat, dtype=np.uint64)
+    vp = ctypes.c_void_p
+    L.po_rlwe_encrypt(table.handle, np.ascontiguousarray(s, dtype=np.int8).ctypes.data_as(vp),
+                      np.ascontiguousarray(e, dtype=np.int8).ctypes.data_as(vp),
+                      np.ascontiguousarray(m, dtype=np.uint8).ctypes.data_as(vp), delta, ptr(a_hat), ptr(out))
+    return out
+
+
+def rlwe_delta(moduli, qi):
+    """Delta = floor(Q/256) mod q_i, Q the product of the request's moduli (Python big integers)."""
+    Q = 1
+    for q in moduli:
+        Q *= q
+    return (Q // 256) % qi

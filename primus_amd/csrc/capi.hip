@@ -781,7 +781,7 @@ int ph_rlwe_ctx_new(uint32_t min_log_n, uint32_t max_log_n, uint32_t max_limbs, 
     if (!out || !moduli || max_limbs == 0 || min_log_n > max_log_n) return fail(PH_ERR_INVALID, "bad arguments");
     *out = nullptr;
     if (min_log_n < kRlweMinLogN || max_log_n > kRlweMaxLogN)
-        return fail(PH_ERR_UNSUPPORTED, "RLWE encryption supports 2^10 <= N <= 2^15");
+        return fail(PH_ERR_UNSUPPORTED, "RLWE encryption supports 2^10 <= N <= 2^16");
     ph_rlwe_ctx *c = new (std::nothrow) ph_rlwe_ctx();
     if (!c) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
     c->min_log_n = min_log_n;
@@ -857,8 +857,9 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     uint64_t *scratch = nullptr;
     PH_HIP(hipMallocAsync((void **)&d_items, items.size() * sizeof(RlweItem), s));
     PH_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(RlweItem), hipMemcpyHostToDevice, s));
-    if (items.front().log_n > 12)  // only the two-pass ring degrees use scratch
-        PH_HIP(hipMallocAsync((void **)&scratch, (size_t)grid * (1ull << kRlweMaxLogN) * 8, s));
+    // only the two-pass ring degrees use scratch: one largest ring degree per workgroup
+    const uint64_t scratch_words = items.front().log_n > 12 ? 1ull << items.front().log_n : 0;
+    if (scratch_words) PH_HIP(hipMallocAsync((void **)&scratch, (size_t)grid * scratch_words * 8, s));
     PH_HIP(hipStreamSynchronize(s));  // the host item list may go away after this
     RlweArgs a{};
     a.items = d_items;
@@ -870,6 +871,7 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     a.e = d_e;
     a.m = d_m;
     a.scratch = scratch;
+    a.scratch_words = scratch_words;
     hipError_t e = launch_rlwe_encrypt(a, grid, s);
     if (scratch) (void)hipFreeAsync(scratch, s);
     (void)hipFreeAsync(d_items, s);

@@ -241,7 +241,7 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
 // registers at once, and capping it lower spills.
 __global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs a) {
     __shared__ uint64_t lds[Geo<12, 8>::LDS];
-    uint64_t *scr = a.scratch + (uint64_t)blockIdx.x * (1ull << kRlweMaxLogN);
+    uint64_t *scr = a.scratch + (uint64_t)blockIdx.x * a.scratch_words;
     for (uint32_t i = blockIdx.x; i < a.n_items; i += gridDim.x) {
         const RlweItem it = a.items[i];
         const DevTable t = a.tabs[it.tab];
@@ -257,6 +257,7 @@ __global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs 
             case 13: rlwe_two_pass<1>(a, it, t, lds, tid, scr); break;
             case 14: rlwe_two_pass<2>(a, it, t, lds, tid, scr); break;
             case 15: rlwe_two_pass<3>(a, it, t, lds, tid, scr); break;
+            case 16: rlwe_two_pass<4>(a, it, t, lds, tid, scr); break;
             default: break;
         }
         __syncthreads();  // LDS and scratch are reused by the next item

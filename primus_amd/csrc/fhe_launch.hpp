@@ -33,9 +33,9 @@ struct ExtArgs {
 
 hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64_t count, hipStream_t s);
 
-// RLWE encryption: ring degrees 2^10 .. 2^15 in one persistent launch.
+// RLWE encryption: ring degrees 2^10 .. 2^16 in one persistent launch.
 constexpr uint32_t kRlweMinLogN = 10;
-constexpr uint32_t kRlweMaxLogN = 15;
+constexpr uint32_t kRlweMaxLogN = 16;
 constexpr uint32_t kRlweThreads = 256;
 
 // One (request, limb) work item.
@@ -58,7 +58,8 @@ struct RlweArgs {
     const int8_t *s;        // ternary secret
     const int8_t *e;        // error
     const uint8_t *m;       // message
-    uint64_t *scratch;      // 2^15 words per workgroup (ring degrees above 2^12)
+    uint64_t *scratch;      // scratch_words per workgroup (ring degrees above 2^12)
+    uint64_t scratch_words; // the largest ring degree of the launch
 };
 
 hipError_t launch_rlwe_encrypt(const RlweArgs &a, uint32_t grid, hipStream_t s);

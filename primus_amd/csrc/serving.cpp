@@ -71,6 +71,7 @@ struct ph_server {
     std::vector<uint64_t> moduli;
     std::vector<ph_ntt_table *> tables;                 // [(log_n - min) * max_limbs + i]
     std::map<std::pair<uint32_t, uint32_t>, ph_dcrt_table *> dcrt;  // (log_n, limbs) -> view
+    std::map<std::pair<uint32_t, uint32_t>, ph_dcrt_table *> single;  // (log_n, limb) -> one-limb view
     ph_rlwe_ctx *rlwe = nullptr;
     uint32_t rlwe_min = 0, rlwe_max = 0, rlwe_limbs = 0;
     std::vector<Slot> slots;
@@ -123,12 +124,14 @@ int validate(const ph_server *s, const ph_serve_request &r) {
     return PH_ERR_INVALID;
 }
 
-// RLWE scratch for ring degrees above 2^12: 2^15 words for each of the
-// kernel's resident workgroups (two per compute unit).
-uint64_t rlwe_scratch_bytes() {
+// Device scratch of an RLWE launch whose largest ring degree is 2^log_n: one
+// polynomial of that degree for each of the kernel's resident workgroups (two
+// per compute unit) when log_n > 12, none otherwise.
+uint64_t rlwe_scratch_bytes(uint32_t log_n) {
+    if (log_n <= 12) return 0;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) cus = 256;
-    return (uint64_t)std::max(cus, 1) * 2 * (1ull << 15) * 8;
+    return (uint64_t)std::max(cus, 1) * 2 * (1ull << log_n) * 8;
 }
 
 int ensure_pinned(Slot &sl, uint64_t bytes) {
@@ -168,11 +171,12 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
         else if (r.kind == PH_REQ_RLWE_ENCRYPT) rlwe.push_back(i);
         else ext.push_back(i);
     }
-    struct GroupAt { uint64_t a, b, count; ph_dcrt_table *t; };
+    struct GroupAt { uint64_t a, b, count; ph_dcrt_table *t; uint32_t log_n; };
     std::vector<GroupAt> gat;
     for (auto &g : groups) {
         const uint64_t row = (uint64_t)g.first.second << g.first.first;  // words per request
-        GroupAt x{take(row * 8 * g.second.size()), take(row * 8 * g.second.size()), g.second.size(), s->dcrt[g.first]};
+        GroupAt x{take(row * 8 * g.second.size()), take(row * 8 * g.second.size()), g.second.size(), s->dcrt[g.first],
+                  g.first.first};
         for (size_t k = 0; k < g.second.size(); ++k) {
             Part &p = b.parts[g.second[k]];
             p.in0 = x.a + k * row * 8;
@@ -238,6 +242,16 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
     for (const GroupAt &g : gat) {
         if (rc != PH_OK) break;
         const uint32_t L = (uint32_t)ph_dcrt_moduli_count(g.t);
+        if (s->cfg.per_poly_launches) {
+            const uint32_t log_n = g.log_n;
+            const uint64_t row = (1ull << log_n) * 8;
+            for (uint64_t k = 0; k < g.count * L && rc == PH_OK; ++k) {
+                rc = ph_dcrt_negacyclic_mul_device(s->single[{log_n, (uint32_t)(k % L)}], (uint64_t *)(dev + g.a + k * row),
+                                                   (const uint64_t *)(dev + g.b + k * row), 1, st);
+                ++s->stats.launches;
+            }
+            continue;
+        }
         rc = ph_dcrt_negacyclic_mul_device(g.t, (uint64_t *)(dev + g.a), (const uint64_t *)(dev + g.b), g.count * L, st);
         ++s->stats.launches;
     }
@@ -283,16 +297,21 @@ void dispatcher_loop(ph_server *s) {
         // admit in arrival order while the budget allows
         Batch b;
         std::vector<Ticket *> tk;
-        bool has_rlwe_big = false;
+        uint32_t rlwe_max_ln = 0;  // largest RLWE ring degree admitted to this batch
         while (!s->pending.empty() && tk.size() < s->cfg.max_batch) {
             Ticket &t = s->tickets[s->pending.front()];
             uint64_t need = t.bytes;
-            if (t.req.kind == PH_REQ_RLWE_ENCRYPT && t.req.log_n > 12 && !has_rlwe_big) need += rlwe_scratch_bytes();
-            if (s->inflight_bytes + b.budget_bytes + need > s->cfg.hbm_budget_bytes && !tk.empty()) break;
-            if (s->inflight_bytes + b.budget_bytes + need > s->cfg.hbm_budget_bytes && tk.empty() &&
-                !s->inflight.empty())
-                break;  // wait for memory to free up
-            if (t.req.kind == PH_REQ_RLWE_ENCRYPT && t.req.log_n > 12) has_rlwe_big = true;
+            uint32_t ln = rlwe_max_ln;
+            if (t.req.kind == PH_REQ_RLWE_ENCRYPT && t.req.log_n > ln) {
+                ln = t.req.log_n;
+                need += rlwe_scratch_bytes(ln) - rlwe_scratch_bytes(rlwe_max_ln);
+            }
+            // every request fits the budget alone (checked at submission), so
+            // an empty device always admits the first one
+            if (s->inflight_bytes + b.budget_bytes + need > s->cfg.hbm_budget_bytes) {
+                if (!tk.empty() || !s->inflight.empty()) break;  // wait for memory to free up
+            }
+            rlwe_max_ln = ln;
             b.budget_bytes += need;
             Part p{};
             p.ticket = s->pending.front();
@@ -357,6 +376,7 @@ void completer_loop(ph_server *s) {
 void destroy(ph_server *s) {
     if (!s) return;
     for (auto &kv : s->dcrt) ph_dcrt_table_free(kv.second);
+    for (auto &kv : s->single) ph_dcrt_table_free(kv.second);
     for (ph_ntt_table *t : s->tables) ph_ntt_table_free(t);
     if (s->rlwe) ph_rlwe_ctx_free(s->rlwe);
     for (Slot &sl : s->slots) {
@@ -394,10 +414,16 @@ int ph_server_new(const ph_server_config *cfg, ph_server **out) {
             rc = ph_dcrt_table_view(&s->tables[(size_t)d * cfg->max_limbs], L, &v);
             if (rc == PH_OK) s->dcrt[{cfg->min_log_n + d, L}] = v;
         }
-    // RLWE tables over the part of the grid its kernel covers (2^10 .. 2^15, <= 12 limbs)
+    for (uint32_t d = 0; d < degrees && rc == PH_OK && cfg->per_poly_launches; ++d)
+        for (uint32_t i = 0; i < cfg->max_limbs && rc == PH_OK; ++i) {
+            ph_dcrt_table *v = nullptr;
+            rc = ph_dcrt_table_view(&s->tables[(size_t)d * cfg->max_limbs + i], 1, &v);
+            if (rc == PH_OK) s->single[{cfg->min_log_n + d, i}] = v;
+        }
+    // RLWE tables over the part of the grid its kernel covers (2^10 .. 2^16)
     s->rlwe_min = std::max(cfg->min_log_n, 10u);
-    s->rlwe_max = std::min(cfg->max_log_n, 15u);
-    s->rlwe_limbs = std::min(cfg->max_limbs, 12u);
+    s->rlwe_max = std::min(cfg->max_log_n, 16u);
+    s->rlwe_limbs = cfg->max_limbs;
     if (rc == PH_OK && s->rlwe_min <= s->rlwe_max) {
         std::vector<uint64_t> sub;
         for (uint32_t ln = s->rlwe_min; ln <= s->rlwe_max; ++ln)
@@ -424,7 +450,8 @@ int ph_server_submit(ph_server *s, const ph_serve_request *req, uint64_t *ticket
     int rc = validate(s, *req);
     if (rc != PH_OK) return rc;
     const uint64_t bytes = request_bytes(*req);
-    if (bytes > s->cfg.hbm_budget_bytes) return PH_ERR_NO_MEMORY;  // can never be admitted
+    const uint64_t scratch = req->kind == PH_REQ_RLWE_ENCRYPT ? rlwe_scratch_bytes(req->log_n) : 0;
+    if (bytes + scratch > s->cfg.hbm_budget_bytes) return PH_ERR_NO_MEMORY;  // can never be admitted
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stopping) return PH_ERR_INVALID;
     Ticket t;

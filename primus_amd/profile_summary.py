@@ -1,0 +1,81 @@
+"""Summaries of rocprofv3 result databases (rocpd SQLite) for the profiles/ folder.
+
+    python -m primus_amd.profile_summary stats RESULTS_DB
+        per-kernel dispatch count, grid, average / min / max duration
+    python -m primus_amd.profile_summary pmc FETCH_DB WRITE_DB KERNEL_SUBSTRING COPY_BYTES
+        FETCH_SIZE and WRITE_SIZE per dispatch of the kernel, calibrated by the
+        byte counts the same counters give for the largest device-to-device
+        copy in the run (COPY_BYTES read and COPY_BYTES written); prints a
+        markdown table and one JSON line with the calibrated HBM traffic.
+"""
+import json
+import sqlite3
+import sys
+
+
+def _rows(db, sql):
+    con = sqlite3.connect(db)
+    try:
+        return list(con.execute(sql))
+    finally:
+        con.close()
+
+
+def stats(db, last=0):
+    """Per (kernel, grid): dispatches and durations; with last > 0 also the
+    average over the last `last` dispatches of each (the timed steps of a run
+    whose earlier dispatches are warm-up and tuning)."""
+    rows = _rows(db, "select name, grid_x, count(*), avg(duration), min(duration), max(duration) "
+                     "from kernels group by name, grid_x order by sum(duration) desc")
+    print("| kernel | grid | dispatches | avg us | min us | max us |%s" % (" avg of last %d us |" % last if last else ""))
+    print("|---|---|---|---|---|---|%s" % ("---|" if last else ""))
+    for name, grid, cnt, avg, mn, mx in rows:
+        tail = ""
+        if last:
+            con = sqlite3.connect(db)
+            try:
+                d = [r[0] for r in con.execute("select duration from kernels where name = ? and grid_x = ? "
+                                               "order by start", (name, grid))]
+            finally:
+                con.close()
+            d = d[-last:]
+            tail = " %.2f |" % (sum(d) / len(d) / 1e3)
+        print("| %s | %d | %d | %.2f | %.2f | %.2f |%s" % (name[:90], grid, cnt, avg / 1e3, mn / 1e3, mx / 1e3, tail))
+
+
+def _per_dispatch(db, counter):
+    return _rows(db, "select kernel_name, grid_size, value from counters_collection where counter_name = '%s' "
+                     "order by dispatch_id" % counter)
+
+
+def pmc(fetch_db, write_db, kernel, copy_bytes):
+    copy_bytes = int(copy_bytes)
+    fetch = _per_dispatch(fetch_db, "FETCH_SIZE")
+    write = _per_dispatch(write_db, "WRITE_SIZE")
+    kf = [v for n, g, v in fetch if kernel in n]
+    kw = [v for n, g, v in write if kernel in n]
+    # the calibration copy: the largest copy kernel dispatch of the run
+    cf = max(v for n, g, v in fetch if "copyBuffer" in n or "copy" in n.lower())
+    cw = max(v for n, g, v in write if "copyBuffer" in n or "copy" in n.lower())
+    fetch_scale = copy_bytes / (cf * 1024.0)
+    write_scale = copy_bytes / (cw * 1024.0)
+    f = sum(kf) / len(kf)
+    w = sum(kw) / len(kw)
+    traffic = f * 1024.0 * fetch_scale + w * 1024.0 * write_scale
+    print("| counter | kernel KiB/dispatch | copy KiB (%d B moved each way) | scale | kernel bytes |" % copy_bytes)
+    print("|---|---|---|---|---|")
+    print("| FETCH_SIZE | %.1f | %.1f | %.4f | %.0f |" % (f, cf, fetch_scale, f * 1024 * fetch_scale))
+    print("| WRITE_SIZE | %.1f | %.1f | %.4f | %.0f |" % (w, cw, write_scale, w * 1024 * write_scale))
+    print()
+    print(json.dumps({"kernel": kernel, "fetch_kib": f, "write_kib": w, "fetch_scale": fetch_scale,
+                      "write_scale": write_scale, "traffic_bytes": traffic, "dispatches": len(kf)}))
+
+
+if __name__ == "__main__":
+    if len(sys.argv) >= 3 and sys.argv[1] == "stats":
+        stats(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 0)
+    elif len(sys.argv) >= 6 and sys.argv[1] == "pmc":
+        pmc(*sys.argv[2:6])
+    else:
+        print(__doc__)
+        sys.exit(2)

@@ -24,7 +24,7 @@ class ServeRequest(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32), ("streams", ctypes.c_uint32),
                 ("min_log_n", ctypes.c_uint32), ("max_log_n", ctypes.c_uint32), ("max_limbs", ctypes.c_uint32),
-                ("moduli", ctypes.POINTER(ctypes.c_uint64)), ("ext_key", _vp)]
+                ("moduli", ctypes.POINTER(ctypes.c_uint64)), ("ext_key", _vp), ("per_poly_launches", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -80,7 +80,7 @@ class Request:
 
 class Server:
     def __init__(self, moduli, min_log_n, max_log_n, max_limbs, ext_key=None, hbm_budget_bytes=64 << 30,
-                 max_batch=256, streams=4):
+                 max_batch=256, streams=4, per_poly_launches=False):
         L = _declare()
         grid = np.zeros((max_log_n - min_log_n + 1, max_limbs), dtype=np.uint64)
         for ln in range(min_log_n, max_log_n + 1):
@@ -89,7 +89,7 @@ class Server:
         self._ext_key = ext_key
         cfg = ServerConfig(hbm_budget_bytes, max_batch, streams, min_log_n, max_log_n, max_limbs,
                            grid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
-                           ext_key.handle if ext_key is not None else None)
+                           ext_key.handle if ext_key is not None else None, int(per_poly_launches))
         h = _vp()
         _check(L.ph_server_new(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

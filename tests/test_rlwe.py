@@ -5,37 +5,14 @@ samples supplied (primus_fhe_core/src/secret_key/rlwe.rs:227-268); it is pinned
 by the algebraic identity b_hat - a_hat (.) NTT(s) = NTT(e + Delta m) and by
 decryption: INTT(b_hat - a_hat (.) s_hat) = e + Delta m.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 
 
-def _p(a):
-    return a.ctypes.data_as(ctypes.c_void_p)
-
-
-def oracle_encrypt(table, s, e, m, delta, a_hat):
-    L = O.lib()
-    L.po_rlwe_encrypt.restype = None
-    L.po_rlwe_encrypt.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                  ctypes.c_uint64, O._p64, O._p64]
-    out = np.zeros(table.n, dtype=np.uint64)
-    a_hat = np.ascontiguousarray(a_hat, dtype=np.uint64)
-    L.po_rlwe_encrypt(table.handle, _p(np.ascontiguousarray(s, dtype=np.int8)),
-                      _p(np.ascontiguousarray(e, dtype=np.int8)), _p(np.ascontiguousarray(m, dtype=np.uint8)),
-                      delta, O.ptr(a_hat), O.ptr(out))
-    return out
-
-
-def delta_of(moduli, qi):
-    """Delta = floor(Q/256) mod q_i, Q the product of the request's moduli."""
-    Q = 1
-    for q in moduli:
-        Q *= q
-    return (Q // 256) % qi
+oracle_encrypt = O.rlwe_encrypt
+delta_of = O.rlwe_delta
 
 
 def samples(n, rng):
@@ -87,12 +64,12 @@ def _moduli_grid(min_ln, max_ln, limbs):
     return {ln: pa.ntt_primes_below(59, 2 << ln, limbs) for ln in range(min_ln, max_ln + 1)}
 
 
-def _run_batch(shapes, seed, max_limbs=12, check=None):
+def _run_batch(shapes, seed, max_limbs=12, check=None, max_ln=15):
     import primus_amd as pa
     from primus_amd.fhe import RlweContext, rlwe_layout
 
-    grid = _moduli_grid(10, 15, max_limbs)
-    ctx = RlweContext(10, 15, max_limbs, grid)
+    grid = _moduli_grid(10, max_ln, max_limbs)
+    ctx = RlweContext(10, max_ln, max_limbs, grid)
     reqs, a_words, small = rlwe_layout(shapes)
     rng = np.random.default_rng(seed)
     a = np.zeros(a_words, dtype=np.uint64)
@@ -126,6 +103,13 @@ def _run_batch(shapes, seed, max_limbs=12, check=None):
 def test_gpu_rlwe_every_degree():
     shapes = [(ln, L) for ln in range(10, 16) for L in (1, 3)]
     _run_batch(shapes, seed=1)
+
+
+@pytest.mark.gpu
+def test_gpu_rlwe_degree_2_16_and_16_limbs():
+    """The serving grid's largest shapes: N = 2^16 beside smaller degrees, up to 16 limbs."""
+    shapes = [(16, 1), (16, 16), (13, 16), (10, 2), (16, 5)]
+    _run_batch(shapes, seed=4, max_limbs=16, max_ln=16)
 
 
 @pytest.mark.gpu

@@ -66,7 +66,7 @@ def _expected(req, grid, ext_key_np):
     return O.ext_product(t, O.Basis.new(EXT_Q, 8, 3), ext_key_np, req.in0.reshape(2, 1024)).ravel()
 
 
-def _server(max_batch, streams, budget=64 << 30, max_limbs=4):
+def _server(max_batch, streams, budget=64 << 30, max_limbs=4, per_poly=False):
     import primus_amd as pa
     from primus_amd import serving as sv
     from primus_amd.fhe import ExtKey
@@ -77,11 +77,13 @@ def _server(max_batch, streams, budget=64 << 30, max_limbs=4):
     ext_tab = pa.UintNttTable.new(10, EXT_Q)
     key = ExtKey(ext_tab, 8, 3, key_np)
     srv = sv.Server(grid, 10, 14, max_limbs, ext_key=key, hbm_budget_bytes=budget, max_batch=max_batch,
-                    streams=streams)
+                    streams=streams, per_poly_launches=per_poly)
     return srv, grid, key_np, key
 
 
 def test_server_mix_matches_oracle_and_alone():
+    from primus_amd import serving as sv
+
     srv, grid, key_np, key = _server(max_batch=64, streams=3)
     reqs = _mix(grid, key_np, 36, seed=5, max_limbs=4)
     tickets = [srv.submit(r) for r in reqs]
@@ -93,12 +95,15 @@ def test_server_mix_matches_oracle_and_alone():
     st = srv.stats()
     assert st["completed"] == len(reqs) and st["batches"] >= 1
     srv.close()
-    # the same requests one at a time (the plain path): identical results
-    alone, grid2, _, key2 = _server(max_batch=1, streams=1)
+    # the same requests one at a time, one launch per limb-polynomial (the
+    # plain path): identical results
+    alone, grid2, _, key2 = _server(max_batch=1, streams=1, per_poly=True)
     for r, got in zip(reqs, mixed):
         r.out[:] = 0
         assert alone.wait(alone.submit(r)) == 0
         np.testing.assert_array_equal(r.out, got)
+    n_poly_limbs = sum(r.limbs for r in reqs if r.kind == sv.POLY_MUL)
+    assert alone.stats()["launches"] == n_poly_limbs + sum(r.kind != sv.POLY_MUL for r in reqs)
     alone.close()
 
 
