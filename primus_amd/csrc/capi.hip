@@ -929,7 +929,11 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
     const bool f64 = d->f64 && o != Op::FwdLazy;
     const uint64_t n = 1ull << d->log_n;
     const size_t words = count * n;
-    uint64_t *buf = nullptr, *buf2 = nullptr;
+    // Rotate over enough copies of the batch (up to 16) that the working set
+    // exceeds 512 MiB: a cache-resident batch would favour other shapes than
+    // a batch streamed from HBM.
+    const size_t copies = std::min<size_t>(16, std::max<size_t>(1, ((512ull << 20) + words * 8 - 1) / (words * 8)));
+    std::vector<uint64_t *> bufs(copies, nullptr), bufs2(copies, nullptr);
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = PH_OK;
@@ -940,11 +944,18 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
     int best_loge = 0;
     const int candidates[2] = {4, 3};
     if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-        hipEventCreate(&e1) != hipSuccess || hipMalloc(&buf, words * 8) != hipSuccess ||
-        hipMalloc(&buf2, words * 8) != hipSuccess || hipMemset(buf, 0, words * 8) != hipSuccess ||
-        hipMemset(buf2, 0, words * 8) != hipSuccess) {
+        hipEventCreate(&e1) != hipSuccess) {
         rc = fail(PH_ERR_DEVICE, "autotune set-up failed");
         goto done;
+    }
+    for (size_t c = 0; c < copies; ++c) {
+        const bool second = o == Op::PolyMul;
+        if (hipMalloc(&bufs[c], words * 8) != hipSuccess || hipMemset(bufs[c], 0, words * 8) != hipSuccess ||
+            (second && (hipMalloc(&bufs2[c], words * 8) != hipSuccess ||
+                        hipMemset(bufs2[c], 0, words * 8) != hipSuccess))) {
+            rc = fail(PH_ERR_NO_MEMORY, "autotune buffers");
+            goto done;
+        }
     }
     if (o == Op::FwdMulInv) {
         ones.assign(d->L * n, Shoup{1, 0});
@@ -970,7 +981,8 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
         // warm-up, then timed repetitions
         for (int r = 0; r <= reps; ++r) {
             if (r == 1) (void)hipEventRecord(e0, s);
-            rc = run_op(o, d->d_tabs, d->L, d->log_n, f64, buf, buf2, o == Op::FwdMulInv ? &op_dev : nullptr, count, s);
+            rc = run_op(o, d->d_tabs, d->L, d->log_n, f64, bufs[r % copies], bufs2[r % copies],
+                        o == Op::FwdMulInv ? &op_dev : nullptr, count, s);
             if (rc != PH_OK) goto done;
         }
         (void)hipEventRecord(e1, s);
@@ -992,8 +1004,10 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
 done:
     if (op_dev.d) (void)hipFree(op_dev.d);
     if (op_dev.d_v) (void)hipFree(op_dev.d_v);
-    if (buf) (void)hipFree(buf);
-    if (buf2) (void)hipFree(buf2);
+    for (size_t c = 0; c < copies; ++c) {
+        if (bufs[c]) (void)hipFree(bufs[c]);
+        if (bufs2[c]) (void)hipFree(bufs2[c]);
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (s) (void)hipStreamDestroy(s);

@@ -193,8 +193,8 @@ __device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &i
             sx[(m << R0) + j] = lift_small(a.s[it.small + k], t.q);
             x[(m << R0) + j] = rlwe_x(a, it, k, t);
         }
-    FwdRounds<IntArith, LOGN, LOGT, 0>::run(sx, lds, tid, 0, 0, t);
-    FwdRounds<IntArith, LOGN, LOGT, 0>::run(x, lds, tid, 0, 0, t);
+    FwdRounds<IntArith, LOGN, LOGT, 0, false>::run(sx, lds, tid, 0, 0, t);
+    FwdRounds<IntArith, LOGN, LOGT, 0, false>::run(x, lds, tid, 0, 0, t);
 #pragma unroll
     for (int j = 0; j < G::E; ++j) rlwe_mac(a, it, ((uint64_t)tid << G::LOGE) + j, x[j], sx[j], t);
 }
@@ -216,8 +216,8 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
             v[i] = lift_small(a.s[it.small + k], t.q);
             w[i] = rlwe_x(a, it, k, t);
         }
-        ct_round<IntArith, R>(v, 0, 0, IntArith::fwd(t), t);
-        ct_round<IntArith, R>(w, 0, 0, IntArith::fwd(t), t);
+        ct_round<IntArith, R, false>(v, 0, 0, IntArith::fwd(t), t);
+        ct_round<IntArith, R, false>(w, 0, 0, IntArith::fwd(t), t);
 #pragma unroll
         for (int i = 0; i < (1 << R); ++i) {
             scr[(uint64_t)i * C + col] = v[i];
@@ -228,10 +228,10 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
     for (uint32_t b = 0; b < (1u << R); ++b) {
         asm volatile("" : "+v"(tid));  // keep slot addresses out of loop-invariant code motion
         uint64_t sx[G::E], x[G::E];
-        FwdRounds<IntArith, LOGC, LOGT, 0>::load_global(scr + (uint64_t)b * C, sx, tid, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0>::run(sx, lds, tid, R, b, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0>::load_global(a.out + it.row + (uint64_t)b * C, x, tid, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0>::run(x, lds, tid, R, b, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0, false>::load_global(scr + (uint64_t)b * C, sx, tid, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0, false>::run(sx, lds, tid, R, b, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0, false>::load_global(a.out + it.row + (uint64_t)b * C, x, tid, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0, false>::run(x, lds, tid, R, b, t);
 #pragma unroll
         for (int j = 0; j < G::E; ++j) rlwe_mac(a, it, (uint64_t)b * C + ((uint64_t)tid << G::LOGE) + j, x[j], sx[j], t);
     }

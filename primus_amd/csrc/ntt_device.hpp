@@ -224,8 +224,8 @@ struct F64Arith {
 // ---------------------------------------------------------------------------
 
 // R forward stages on one register group x[0 .. 2^R), global stages [S, S+R),
-// chunk c at stage S.
-template <class A, int R>
+// chunk c at stage S.  FENCE: schedule stage by stage (see below).
+template <class A, int R, bool FENCE = true>
 __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t c,
                                          gptr<typename A::Tw> tw, const DevTable &t) {
     if constexpr (A::kReduceAtRoundStart) {
@@ -245,13 +245,17 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
                 A::ct(x[j0], x[j0 + half], w, t);
             }
         }
+        // one stage at a time: hoisting every stage's twiddles to the top of
+        // the round keeps them all live at once and spills at 128 VGPRs.
+        // Callers that interleave two independent rounds leave it off.
+        if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // R inverse stages on one register group, global inverse stages [P, P+R)
 // (gaps 2^P .. 2^(P+R-1)), chunk c = k >> (P+R).  LAST: the round ends with the
 // transform's final stage (gap N/2).
-template <class A, int R, bool LAST>
+template <class A, int R, bool LAST, bool FENCE = true>
 __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t c, uint32_t n,
                                          gptr<typename A::Tw> itw, const DevTable &t) {
 #pragma unroll
@@ -272,6 +276,7 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
                 }
             }
         }
+        if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);  // as in ct_round
     }
 }
 

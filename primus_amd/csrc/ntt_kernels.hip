@@ -39,8 +39,9 @@ __device__ __forceinline__ Shoup ld_operand(gptr<Shoup> p, const IntArith *) { r
 __device__ __forceinline__ uint64_t ld_operand(gptr<uint64_t> p, const F64Arith *) { return *p; }
 
 // Waves per SIMD the launch bounds ask the register allocator for: as many as
-// the LDS footprint lets resident (at most 8), halved when a second operand's
-// transform is held in registers too.
+// the LDS footprint lets resident, at most 4 (128 VGPRs: 8 waves per SIMD
+// would leave 64, and the radix-8 rounds spill there), halved when a second
+// operand's transform is held in registers too.
 template <int LOGC, int LOGT, int MODE>
 struct Occupancy {
     static constexpr int lds_bytes = (Geo<LOGC, LOGT>::LDS) * 8;
@@ -48,7 +49,7 @@ struct Occupancy {
     static constexpr int wgs = wgs_lds < 1 ? 1 : (wgs_lds > 32 ? 32 : wgs_lds);
     static constexpr int waves_wg = (1 << LOGT) < 64 ? 1 : (1 << LOGT) / 64;
     static constexpr int per_simd_raw = (wgs * waves_wg) / 4;
-    static constexpr int per_simd = per_simd_raw < 1 ? 1 : (per_simd_raw > 8 ? 8 : per_simd_raw);
+    static constexpr int per_simd = per_simd_raw < 1 ? 1 : (per_simd_raw > 4 ? 4 : per_simd_raw);
     static constexpr int value = (MODE & M_MUL_POLY) ? (per_simd / 2 < 1 ? 1 : per_simd / 2) : per_simd;
 };
 

@@ -44,7 +44,9 @@ __device__ __forceinline__ uint32_t inv_slot(uint32_t t, int m, int j) {
     return (c << (P + R)) + ((uint32_t)j << P) + o;
 }
 
-template <class A, int LOGC, int LOGT, int I>
+// FENCE: schedule each round stage by stage (ct_round); off for kernels that
+// keep two transforms in flight and need the scheduler's freedom.
+template <class A, int LOGC, int LOGT, int I, bool FENCE = true>
 struct FwdRounds {
     using G = Geo<LOGC, LOGT>;
     using T = typename A::T;
@@ -77,7 +79,7 @@ struct FwdRounds {
                                                const DevTable &tab) {
         if constexpr (I > 0) {
             __syncthreads();
-            FwdRounds<A, LOGC, LOGT, I - 1>::store(lds, x, t);
+            FwdRounds<A, LOGC, LOGT, I - 1, FENCE>::store(lds, x, t);
             __syncthreads();
             load(lds, x, t);
         }
@@ -87,9 +89,9 @@ struct FwdRounds {
             uint32_t cl;
             if constexpr (OB >= LOGT) cl = (uint32_t)m >> (OB - LOGT);  // wave-uniform
             else cl = (t + ((uint32_t)m << LOGT)) >> OB;
-            ct_round<A, R>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
+            ct_round<A, R, FENCE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
         }
-        if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1>::run(x, lds, t, st0, b, tab);
+        if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1, FENCE>::run(x, lds, t, st0, b, tab);
     }
 };
 

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -64,6 +65,92 @@ struct Slot {
     bool busy = false;
 };
 
+struct CopyJob {
+    void *dst;
+    const void *src;
+    uint64_t bytes;
+};
+
+// Host-side staging copies of one thread (the dispatcher's inputs or the
+// completer's outputs) spread over a few helper threads: a single thread's
+// memcpy bandwidth, not PCIe, otherwise caps the request rate.  run() returns
+// when every job is done; small batches are copied by the caller alone.  One
+// caller at a time.
+class CopyPool {
+  public:
+    explicit CopyPool(unsigned helpers) {
+        for (unsigned i = 0; i < helpers; ++i) threads_.emplace_back([this] { worker(); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : threads_) t.join();
+    }
+    void run(const std::vector<CopyJob> &jobs) {
+        uint64_t total = 0;
+        for (const CopyJob &j : jobs) total += j.bytes;
+        if (threads_.empty() || total < (4ull << 20)) {
+            for (const CopyJob &j : jobs) std::memcpy(j.dst, j.src, j.bytes);
+            return;
+        }
+        // pieces of at most 2 MiB so that the helpers share the work evenly
+        std::vector<CopyJob> pieces;
+        for (const CopyJob &j : jobs)
+            for (uint64_t o = 0; o < j.bytes; o += 2ull << 20)
+                pieces.push_back({(char *)j.dst + o, (const char *)j.src + o, std::min<uint64_t>(2ull << 20, j.bytes - o)});
+        std::unique_lock<std::mutex> lk(mu_);
+        // a helper still leaving the previous run reads pieces_: let it go first
+        cv_done_.wait(lk, [&] { return active_ == 0; });
+        pieces_.swap(pieces);
+        next_.store(0);
+        done_ = 0;
+        ++generation_;
+        ++active_;  // the caller works too
+        lk.unlock();
+        cv_.notify_all();
+        const size_t finished = drain();
+        lk.lock();
+        done_ += finished;
+        --active_;
+        cv_done_.wait(lk, [&] { return done_ == pieces_.size() && active_ == 0; });
+    }
+
+  private:
+    size_t drain() {
+        size_t n = 0;
+        for (size_t i = next_.fetch_add(1); i < pieces_.size(); i = next_.fetch_add(1), ++n)
+            std::memcpy(pieces_[i].dst, pieces_[i].src, pieces_[i].bytes);
+        return n;
+    }
+    void worker() {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        while (true) {
+            cv_.wait(lk, [&] { return stop_ || generation_ != seen; });
+            if (stop_) return;
+            seen = generation_;
+            ++active_;
+            lk.unlock();
+            const size_t finished = drain();
+            lk.lock();
+            done_ += finished;
+            --active_;
+            cv_done_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::vector<CopyJob> pieces_;  // replaced only under mu_ with no helper active
+    std::atomic<size_t> next_{0};
+    size_t done_ = 0, active_ = 0;
+    uint64_t generation_ = 0;
+    bool stop_ = false;
+    std::mutex mu_;
+    std::condition_variable cv_, cv_done_;
+};
+
 }  // namespace
 
 struct ph_server {
@@ -89,6 +176,8 @@ struct ph_server {
 
     ph_server_stats stats{};
     Clock::time_point started;
+    // staging copies: inputs (dispatcher thread), outputs (completion thread)
+    CopyPool copy_in{4}, copy_out{4};
 };
 
 namespace {
@@ -222,18 +311,20 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
 
     if (ensure_pinned(sl, off) != PH_OK) return PH_ERR_NO_MEMORY;
     char *host = (char *)sl.pinned;
+    std::vector<CopyJob> jobs;
     for (size_t i = 0; i < tk.size(); ++i) {
         const ph_serve_request &r = tk[i]->req;
         const Part &p = b.parts[i];
         const uint64_t n = 1ull << r.log_n;
-        std::memcpy(host + p.in0, r.in0, p.out_bytes);
-        if (r.kind == PH_REQ_POLY_MUL) std::memcpy(host + p.in1, r.in1, p.out_bytes);
+        jobs.push_back({host + p.in0, r.in0, p.out_bytes});
+        if (r.kind == PH_REQ_POLY_MUL) jobs.push_back({host + p.in1, r.in1, p.out_bytes});
         if (r.kind == PH_REQ_RLWE_ENCRYPT) {
-            std::memcpy(host + rs + p.small, r.s, n);
-            std::memcpy(host + re + p.small, r.e, n);
-            std::memcpy(host + rm + p.small, r.m, n);
+            jobs.push_back({host + rs + p.small, r.s, n});
+            jobs.push_back({host + re + p.small, r.e, n});
+            jobs.push_back({host + rm + p.small, r.m, n});
         }
     }
+    s->copy_in.run(jobs);
     void *arena = nullptr;
     if (hipMallocAsync(&arena, std::max<uint64_t>(off, kAlign), st) != hipSuccess) return PH_ERR_NO_MEMORY;
     char *dev = (char *)arena;
@@ -266,10 +357,16 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
                                    0, st);
         ++s->stats.launches;
     }
-    // results back to the staging area, then the event the completer waits on
-    for (size_t i = 0; i < tk.size() && rc == PH_OK; ++i) {
-        const Part &p = b.parts[i];
-        if (hipMemcpyAsync(host + p.out, dev + p.out, p.out_bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
+    // results back to the staging area, one copy per contiguous result region
+    // (each product group's a blocks, the RLWE b_hat rows, the external
+    // products' outputs), then the event the completer waits on
+    std::vector<std::pair<uint64_t, uint64_t>> regions;  // (offset, bytes)
+    for (const GroupAt &g : gat) regions.push_back({g.a, g.b - g.a});
+    if (!rlwe.empty()) regions.push_back({rout, a_words * 8});
+    if (!ext.empty()) regions.push_back({xout, ext_row * ext.size()});
+    for (const auto &rg : regions) {
+        if (rc != PH_OK || rg.second == 0) continue;
+        if (hipMemcpyAsync(host + rg.first, dev + rg.first, rg.second, hipMemcpyDeviceToHost, st) != hipSuccess)
             rc = PH_ERR_DEVICE;
     }
     (void)hipFreeAsync(arena, st);
@@ -347,15 +444,16 @@ void completer_loop(ph_server *s) {
         }
         Batch &b = s->inflight.front();
         hipEvent_t ev = b.done;
+        Slot &sl = s->slots[b.slot];
+        // the callers' buffers, read while the ticket list cannot grow
+        std::vector<CopyJob> jobs;
+        for (const Part &p : b.parts)
+            jobs.push_back({s->tickets[p.ticket].req.out, (char *)sl.pinned + p.out, p.out_bytes});
         lk.unlock();
         int st = b.status;
         if (hipEventSynchronize(ev) != hipSuccess && st == PH_OK) st = PH_ERR_DEVICE;
         // results from the pinned staging area to the callers' buffers
-        Slot &sl = s->slots[b.slot];
-        for (const Part &p : b.parts) {
-            Ticket &t = s->tickets[p.ticket];
-            if (st == PH_OK) std::memcpy(t.req.out, (char *)sl.pinned + p.out, p.out_bytes);
-        }
+        if (st == PH_OK) s->copy_out.run(jobs);
         const auto now = Clock::now();
         lk.lock();
         for (const Part &p : b.parts) {

@@ -113,7 +113,9 @@ def test_server_poisson_load_and_budget():
     lat, elapsed = srv.run_poisson(reqs, rate_per_s=2000.0, seed=1)
     assert np.all(lat > 0) and elapsed > 0
     assert srv.stats()["peak_inflight_bytes"] <= 256 << 20
-    for r in reqs[:12]:
+    # batches of up to 32 requests exceed the size above which staging copies
+    # are split over helper threads: every result is checked
+    for r in reqs:
         np.testing.assert_array_equal(r.out, _expected(r, grid, key_np))
     srv.close()
 
