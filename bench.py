@@ -511,10 +511,12 @@ def main():
         },
         "batched_vs_plain": plain_ms / ms_per_step,
         "arith": table.arith(),
-        "autotune": {"coefficients_per_thread": 1 << shape, "best_ms": shape_ms},
+        "autotune": {"coefficients_per_thread": tuning.describe(shape)[0], "streaming": tuning.describe(shape)[1],
+                     "best_ms": shape_ms},
         "int_path": {
             "desc": "same fused kernel on 64-bit integer Shoup arithmetic (32-bit multiply chains)",
-            "coefficients_per_thread": 1 << int_shape,
+            "coefficients_per_thread": tuning.describe(int_shape)[0],
+            "streaming": tuning.describe(int_shape)[1],
             "ms_per_batch": int_ms,
             "ntt_per_s": NTT_PER_STEP / (int_ms * 1e-3),
             "hbm_frac": BYTES_PER_STEP / (int_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
@@ -526,7 +528,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": traffic,
-            "kernel": "chunk_kernel<F64Arith,12,8,FWD|MUL_SHOUP|INV|INV_LAST> (fused fwd -> mul -> inv)",
+            "kernel": "%s<F64Arith,12,%d,FWD|MUL_SHOUP|INV|INV_LAST> (fused fwd -> mul -> inv)" % (
+                "stream_kernel" if tuning.describe(shape)[1] else "chunk_kernel",
+                LOG_N - (tuning.describe(shape)[0].bit_length() - 1)),
             "kernel_ms": kernel_ms,
             "bytes_per_launch": BYTES_PER_STEP,
         },

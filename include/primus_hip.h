@@ -302,9 +302,12 @@ int ph_server_run_poisson(ph_server *server, const ph_serve_request *requests, s
 #define PH_OP_NEGACYCLIC_MUL 4
 
 /* Process-wide tuning table: for (log_n, op, arith: 0 integer / 1 binary64) the
- * log2 of coefficients per thread of the single-pass kernels (3 or 4; 0 = the
- * default).  Results never depend on it. */
-int ph_tuning_set(uint32_t log_n, int op, int arith, int loge);
+ * launch shape of the single-pass kernels (N <= 2^13): log2 of coefficients
+ * per thread (3 or 4), plus PH_SHAPE_STREAM for the persistent streaming
+ * kernel (forward-first operations, N >= 2^10); 0 = the default (4).  Results
+ * never depend on it. */
+#define PH_SHAPE_STREAM 16
+int ph_tuning_set(uint32_t log_n, int op, int arith, int shape);
 int ph_tuning_get(uint32_t log_n, int op, int arith);
 int ph_tuning_save(const char *path);
 int ph_tuning_load(const char *path);

@@ -28,16 +28,34 @@ HIPCC_FLAGS = [
 
 
 def build(verbose=False):
-    """Compile every HIP source into primus_amd/libprimus_hip.so for gfx950."""
+    """Compile every HIP source into primus_amd/libprimus_hip.so for gfx950.
+
+    Each source is compiled to an object in parallel (one hipcc per source),
+    then the objects are linked; every call recompiles everything."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objdir = os.path.join(PKG_DIR, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    procs = []
+    for src in srcs:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        cmd = [hipcc] + compile_flags + ["-pthread", "-c", "-o", obj, src]
+        procs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    objs, errors = [], []
+    for obj, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            errors.append(out)
+        elif verbose:
+            print(out)
+        objs.append(obj)
+    if errors:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-pthread", "-o", tmp] + srcs
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    res = subprocess.run([hipcc] + HIPCC_FLAGS + ["-pthread", "-o", tmp] + objs, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
-    if verbose:
-        print(res.stdout + res.stderr)
+        raise RuntimeError("hipcc link failed:\n" + res.stdout + res.stderr)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -46,7 +46,7 @@ struct ph_operand {
     uint32_t L;
     uint64_t n;
     Shoup *d = nullptr;      // ShoupFactor form (integer path)
-    uint64_t *d_v = nullptr; // values alone (binary64 path)
+    ShoupF *d_f = nullptr;   // centred binary64 pairs {w, w/q} (binary64 path)
 };
 
 namespace {
@@ -225,6 +225,7 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
         }
         d.qd = (double)q;
         d.qinv = 1.0 / (double)q;
+        d.q_2p52 = (double)q + 0x1p52;
         ShoupF in = mkf(Shoup{t->inv_n, t->inv_n_q}, q);
         ShoupF sr = mkf(Shoup{d.sr, d.sr_q}, q);
         d.inv_n_f = in.w;
@@ -284,8 +285,8 @@ int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, 
     a.data = d_data;
     a.data2 = d_data2;
     a.operand = operand ? operand->d : nullptr;
-    a.operand_v = operand ? operand->d_v : nullptr;
-    if (op == Op::FwdMulInv && f64 && !a.operand_v) f64 = false;
+    a.operand_f = operand ? operand->d_f : nullptr;
+    if (op == Op::FwdMulInv && f64 && !a.operand_f) f64 = false;
     a.tabs = d_tabs;
     a.L = L;
     void *scratch = nullptr;
@@ -563,7 +564,12 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
     op->L = d->L;
     op->n = n;
     int rc = upload((void **)&op->d, h.data(), h.size() * sizeof(Shoup));
-    if (rc == PH_OK && d->f64) rc = upload((void **)&op->d_v, host_values, (size_t)d->L * n * sizeof(uint64_t));
+    if (rc == PH_OK && d->f64) {
+        std::vector<ShoupF> hf(h.size());
+        for (uint32_t l = 0; l < d->L; ++l)
+            for (uint64_t i = 0; i < n; ++i) hf[l * n + i] = mkf(h[l * n + i], d->tables[l]->q);
+        rc = upload((void **)&op->d_f, hf.data(), hf.size() * sizeof(ShoupF));
+    }
     if (rc != PH_OK) {
         ph_operand_free(op);
         return rc;
@@ -575,7 +581,7 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
 void ph_operand_free(ph_operand *op) {
     if (!op) return;
     if (op->d) (void)hipFree(op->d);
-    if (op->d_v) (void)hipFree(op->d_v);
+    if (op->d_f) (void)hipFree(op->d_f);
     delete op;
 }
 
@@ -878,10 +884,12 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
 }
 
-int ph_tuning_set(uint32_t log_n, int op, int arith, int loge) {
-    if (log_n > kMaxLogN || op < 0 || op > 4 || arith < 0 || arith > 1 || (loge != 0 && loge != 3 && loge != 4))
+static_assert(PH_SHAPE_STREAM == kShapeStream, "the header's shape flag is the launcher's");
+
+int ph_tuning_set(uint32_t log_n, int op, int arith, int shape) {
+    if (log_n > kMaxLogN || op < 0 || op > 4 || arith < 0 || arith > 1 || !valid_shape(shape))
         return fail(PH_ERR_INVALID, "bad tuning key or value");
-    g_tuning[log_n][op][arith].store(loge, std::memory_order_relaxed);
+    g_tuning[log_n][op][arith].store(shape, std::memory_order_relaxed);
     return PH_OK;
 }
 
@@ -894,7 +902,7 @@ int ph_tuning_save(const char *path) {
     if (!path) return fail(PH_ERR_INVALID, "null path");
     FILE *f = fopen(path, "w");
     if (!f) return fail(PH_ERR_INVALID, "cannot open tuning file for writing");
-    fprintf(f, "# primus_hip tuning: log_n op arith loge\n");
+    fprintf(f, "# primus_hip tuning: log_n op arith shape\n");
     for (uint32_t n = 0; n <= kMaxLogN; ++n)
         for (int op = 0; op < 5; ++op)
             for (int ar = 0; ar < 2; ++ar) {
@@ -938,11 +946,14 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = PH_OK;
     std::vector<Shoup> ones;
-    std::vector<uint64_t> onesv;
+    std::vector<ShoupF> onesv;
     ph_operand op_dev{};
     float best = 1e30f;
-    int best_loge = 0;
-    const int candidates[2] = {4, 3};
+    int best_shape = 0;
+    // 16 / 8 coefficients per thread, each as one workgroup per polynomial
+    // and as the persistent streaming kernel (forward-first operations)
+    const int candidates[4] = {4, 3, 4 | kShapeStream, 3 | kShapeStream};
+    const bool can_stream = o == Op::FwdLazy || o == Op::FwdCanon || o == Op::FwdMulInv;
     if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
         hipEventCreate(&e1) != hipSuccess) {
         rc = fail(PH_ERR_DEVICE, "autotune set-up failed");
@@ -968,16 +979,20 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
             goto done;
         }
         if (d->f64) {
-            onesv.assign(ones.size(), 1);
-            if (upload((void **)&op_dev.d_v, onesv.data(), onesv.size() * sizeof(uint64_t)) != PH_OK) {
+            onesv.resize(ones.size());
+            for (uint32_t l = 0; l < d->L; ++l)
+                for (uint64_t i = 0; i < n; ++i) onesv[l * n + i] = mkf(ones[l * n + i], d->tables[l]->q);
+            if (upload((void **)&op_dev.d_f, onesv.data(), onesv.size() * sizeof(ShoupF)) != PH_OK) {
                 rc = PH_ERR_DEVICE;
                 goto done;
             }
         }
     }
-    for (int loge : candidates) {
-        if (loge == 3 && (d->log_n < 10 || d->log_n > kMaxSinglePassLogN)) continue;
-        g_tuning[d->log_n][op][f64 ? 1 : 0].store(loge);
+    for (int shape : candidates) {
+        const bool single = d->log_n >= 10 && d->log_n <= kMaxSinglePassLogN;
+        if ((shape & 15) == 3 && !single) continue;
+        if ((shape & kShapeStream) && !(single && can_stream)) continue;
+        g_tuning[d->log_n][op][f64 ? 1 : 0].store(shape);
         // warm-up, then timed repetitions
         for (int r = 0; r <= reps; ++r) {
             if (r == 1) (void)hipEventRecord(e0, s);
@@ -995,15 +1010,15 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
         ms /= reps;
         if (ms < best) {
             best = ms;
-            best_loge = loge;
+            best_shape = shape;
         }
     }
-    g_tuning[d->log_n][op][f64 ? 1 : 0].store(best_loge);
-    if (chosen) *chosen = best_loge;
+    g_tuning[d->log_n][op][f64 ? 1 : 0].store(best_shape);
+    if (chosen) *chosen = best_shape;
     if (best_ms) *best_ms = best;
 done:
     if (op_dev.d) (void)hipFree(op_dev.d);
-    if (op_dev.d_v) (void)hipFree(op_dev.d_v);
+    if (op_dev.d_f) (void)hipFree(op_dev.d_f);
     for (size_t c = 0; c < copies; ++c) {
         if (bufs[c]) (void)hipFree(bufs[c]);
         if (bufs2[c]) (void)hipFree(bufs2[c]);

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
         for (int m = 0; m < (G::E >> R0); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R0); ++j) {
-                uint64_t v = src[fwd_slot<LOGN, LOGT, 0, R0>(t, m, j)];
+                uint64_t v = ld_stream(src + fwd_slot<LOGN, LOGT, 0, R0>(t, m, j));
                 if (g.has_split && v >= g.split) v += g.pow2_sub_q;
                 adj[(m << R0) + j] = v;
                 carry[(m << R0) + j] = g.drop_bits ? (uint32_t)((v >> (g.drop_bits - 1)) & 1u) : 0u;

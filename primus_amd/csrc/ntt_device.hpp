@@ -77,6 +77,7 @@ struct DevTable {
     const Shoup *ord;         // ordinal_root_powers, 2N entries
     // binary64 view, valid when q < 2^50 (fwd_f != nullptr)
     double qd, qinv;
+    double q_2p52;            // q + 2^52 (exact for q < 2^50)
     double inv_n_f, inv_n_p, sr_f, sr_p;
     const ShoupF *fwd_f;
     const ShoupF *inv_f;
@@ -190,14 +191,19 @@ struct F64Arith {
         a = mulmod(u + v, t.inv_n_f, t.inv_n_p, t.qd);
         b = mulmod(u - v, t.sr_f, t.sr_p, t.qd);
     }
-    // u64 in [0, 4q) -> centred lane value, |x| <= q/2 + 1
+    // u64 in [0, 4q) -> centred lane value, |x| <= q/2 + 1.  v < 2^52, so
+    // the binary64 with exponent field 2^52 and v as its mantissa is 2^52 + v
+    // exactly: one OR and one subtraction instead of a general conversion.
     __device__ __forceinline__ static T load(uint64_t v, const DevTable &t) {
-        return reduce((double)v, t);
+        const double biased = __longlong_as_double((long long)(v | 0x4330000000000000ull));
+        return reduce(biased - 0x1p52, t);
     }
-    // centred |x| < q -> [0, q)
+    // centred |x| < q -> [0, q): x + 2^52 (+ q when negative) lies in
+    // [2^52, 2^53), where binary64 holds every integer and the low 52 bits of
+    // the encoding are the integer minus 2^52.
     __device__ __forceinline__ static uint64_t to_canon(double x, const DevTable &t) {
-        const double r = x < 0.0 ? x + t.qd : x;
-        return (uint64_t)r;
+        const double y = x + (x < 0.0 ? t.q_2p52 : 0x1p52);
+        return (uint64_t)__double_as_longlong(y) & ((1ull << 52) - 1);
     }
     __device__ __forceinline__ static uint64_t canon_fwd(T v, const DevTable &t) {
         return to_canon(reduce(v, t), t);
@@ -205,12 +211,11 @@ struct F64Arith {
     __device__ __forceinline__ static uint64_t out_inv(T v, const DevTable &t) {
         return to_canon(v, t);  // |v| <= 0.744 q
     }
-    // Forward output (|x| <= 3.48 q, reduced here to q/2 + 1) times a canonical
-    // operand w < q with w/q formed on the fly (|wp - w/q| <= 2^-52):
-    // |x*w/q - k| <= 1/2 + 2^-3, so the product is <= 0.63 q.
-    __device__ __forceinline__ static T mul_operand(T x, uint64_t w, const DevTable &t) {
-        const double wd = (double)w;
-        return mulmod(reduce(x, t), wd, wd * t.qinv, t.qd);
+    // Forward output (|x| <= 3.48 q) times a resident operand stored like the
+    // twiddles (centred w, w/q correctly rounded): the bounds of mulmod hold
+    // as they are, and the product is <= 0.744 q.
+    __device__ __forceinline__ static T mul_operand(T x, const ShoupF &w, const DevTable &t) {
+        return mulmod(x, w.w, w.wp, t.qd);
     }
     // x*y mod q: both reduced to q/2 + 1, y/q formed on the fly (<= 0.57 q out).
     __device__ __forceinline__ static T mul_pair(T x, T y, const DevTable &t) {
@@ -279,6 +284,15 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
         if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);  // as in ct_round
     }
 }
+
+// Coefficient rows are read once and written once per launch: non-temporal
+// accesses stream them past the caches, which keep the twiddles, operands and
+// keys every workgroup re-reads (and leave less dirty data to write back when
+// the kernel ends).  Used where a wave's access covers whole contiguous rows;
+// per-thread strided accesses keep the default policy, whose write-back cache
+// merges them into full lines.
+__device__ __forceinline__ uint64_t ld_stream(const uint64_t *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(uint64_t *p, uint64_t v) { __builtin_nontemporal_store(v, p); }
 
 // LDS slot of local index k: one pad word per 16 keeps the stride-16 and
 // stride-1 access patterns of the rounds free of bank conflicts.

@@ -18,6 +18,9 @@
 //
 // A is the arithmetic policy (IntArith or F64Arith, ntt_device.hpp).  MODE
 // bits select what a chunk does between its load and its store.
+#include <algorithm>
+#include <atomic>
+
 #include "ntt_device.hpp"
 #include "ntt_launch.hpp"
 #include "ntt_rounds.hpp"
@@ -34,9 +37,7 @@ enum : int {
 };
 
 __device__ __forceinline__ const Shoup *operand_of(const KArgs &a, const IntArith *) { return a.operand; }
-__device__ __forceinline__ const uint64_t *operand_of(const KArgs &a, const F64Arith *) { return a.operand_v; }
-__device__ __forceinline__ Shoup ld_operand(gptr<Shoup> p, const IntArith *) { return ld(p); }
-__device__ __forceinline__ uint64_t ld_operand(gptr<uint64_t> p, const F64Arith *) { return *p; }
+__device__ __forceinline__ const ShoupF *operand_of(const KArgs &a, const F64Arith *) { return a.operand_f; }
 
 // Waves per SIMD the launch bounds ask the register allocator for: as many as
 // the LDS footprint lets resident, at most 4 (128 VGPRs: 8 waves per SIMD
@@ -52,6 +53,40 @@ struct Occupancy {
     static constexpr int per_simd = per_simd_raw < 1 ? 1 : (per_simd_raw > 4 ? 4 : per_simd_raw);
     static constexpr int value = (MODE & M_MUL_POLY) ? (per_simd / 2 < 1 ? 1 : per_simd / 2) : per_simd;
 };
+
+// What a chunk does after its forward rounds: the optional multiply by the
+// resident operand, then either the forward's store or the inverse rounds
+// and their store.  After the last forward round thread t holds local slots
+// t*E .. t*E+E-1.
+template <class A, int LOGC, int LOGT, int MODE>
+__device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, typename A::T *lds, uint32_t t,
+                                              uint32_t b, uint32_t limb, uint32_t n, const DevTable &tab,
+                                              uint64_t *data) {
+    using G = Geo<LOGC, LOGT>;
+    constexpr int LE = G::LOGE;
+    // Keep the scheduler from hoisting the next phase's loads into the last
+    // forward round: their registers would be live across it and spill.
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr ((MODE & M_MUL_SHOUP) != 0) {
+        const auto op = as_global(operand_of(a, (const A *)nullptr)) + (uint64_t)limb * n + (uint64_t)b * G::C +
+                        ((uint64_t)t << LE);
+#pragma unroll
+        for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], ld(op + j), tab);
+    }
+    if constexpr ((MODE & M_INV) == 0) {
+        uint64_t *dst = data + ((uint64_t)t << LE);
+#pragma unroll
+        for (int j = 0; j < G::E; ++j) {
+            if constexpr ((MODE & M_CANON) != 0) dst[j] = A::canon_fwd(x[j], tab);
+            else dst[j] = (uint64_t)x[j];  // IntArith only: lazy [0,4q)
+        }
+    } else {
+        constexpr bool LAST = (MODE & M_INV_LAST) != 0;
+        __builtin_amdgcn_sched_barrier(0);
+        InvRounds<A, LOGC, LOGT, 0, LAST>::run(x, lds, t, n, b, tab);
+        InvRounds<A, LOGC, LOGT, G::NR - 1, LAST>::store_global(data, x, t, tab);
+    }
+}
 
 template <class A, int LOGC, int LOGT, int MODE>
 __global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) void chunk_kernel(KArgs a) {
@@ -86,36 +121,60 @@ __global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) vo
             FwdRounds<A, LOGC, LOGT, 0>::load_global(data, x, t, tab);
             FwdRounds<A, LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab);
         }
-        // after the last forward round thread t holds local slots t*E .. t*E+E-1.
-        // Keep the scheduler from hoisting the next phase's loads into this one:
-        // their registers would be live across the whole round and spill.
-        __builtin_amdgcn_sched_barrier(0);
-        if constexpr ((MODE & M_MUL_SHOUP) != 0) {
-            const auto op = as_global(operand_of(a, (const A *)nullptr)) + (uint64_t)limb * n + (uint64_t)b * G::C +
-                            ((uint64_t)t << LE);
-#pragma unroll
-            for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], ld_operand(op + j, (const A *)nullptr), tab);
-        }
-        if constexpr ((MODE & M_INV) == 0) {
-            uint64_t *dst = data + ((uint64_t)t << LE);
-#pragma unroll
-            for (int j = 0; j < G::E; ++j) {
-                if constexpr ((MODE & M_CANON) != 0) dst[j] = A::canon_fwd(x[j], tab);
-                else dst[j] = (uint64_t)x[j];  // IntArith only: lazy [0,4q)
-            }
-            return;
-        }
+        after_forward<A, LOGC, LOGT, MODE>(a, x, lds, t, b, limb, n, tab, data);
     } else {
         const uint64_t *src = data + ((uint64_t)t << LE);
 #pragma unroll
         for (int j = 0; j < G::E; ++j) x[j] = A::load(src[j], tab);
-    }
-
-    if constexpr ((MODE & M_INV) != 0) {
         constexpr bool LAST = (MODE & M_INV_LAST) != 0;
         __builtin_amdgcn_sched_barrier(0);
         InvRounds<A, LOGC, LOGT, 0, LAST>::run(x, lds, t, n, b, tab);
         InvRounds<A, LOGC, LOGT, G::NR - 1, LAST>::store_global(data, x, t, tab);
+    }
+}
+
+// Waves per SIMD of the streaming kernel: as the chunk kernel's, but with
+// room for the next polynomial's coefficients (E more 64-bit registers), so
+// 16 coefficients per thread drop to two waves per SIMD (256 VGPRs).
+template <int LOGC, int LOGT, int MODE>
+struct StreamOccupancy {
+    static constexpr int base = Occupancy<LOGC, LOGT, MODE>::value;
+    static constexpr int value = (LOGC - LOGT) >= 4 ? (base > 2 ? 2 : base) : base;
+};
+
+// Persistent, software-pipelined form of the single-pass chunk kernel for
+// modes that start with forward stages and have no second operand.  Each
+// workgroup walks limb-polynomials blockIdx.x, +gridDim.x, ... (the grid is
+// at most one resident generation, so every workgroup has at least one) and
+// issues the loads of its next polynomial before the forward rounds of the
+// current one: they land while it computes, instead of every workgroup
+// loading at once at the start and storing at once at the end.  Outputs are
+// those of chunk_kernel with the same template arguments.
+template <class A, int LOGC, int LOGT, int MODE>
+__global__ __launch_bounds__(1 << LOGT, (StreamOccupancy<LOGC, LOGT, MODE>::value)) void stream_kernel(KArgs a) {
+    static_assert((MODE & M_FWD) != 0 && (MODE & M_MUL_POLY) == 0, "streaming covers forward-first modes");
+    using G = Geo<LOGC, LOGT>;
+    using T = typename A::T;
+    __shared__ T lds[G::LDS];
+    const uint32_t t = threadIdx.x;
+    uint64_t poly = blockIdx.x;
+    uint64_t raw[G::E];
+    FwdRounds<A, LOGC, LOGT, 0>::load_raw(a.data + poly * G::C, raw, t);
+    while (true) {
+        const uint32_t limb = (uint32_t)(poly % a.L);
+        const DevTable tab = a.tabs[limb];
+        T x[G::E];
+#pragma unroll
+        for (int j = 0; j < G::E; ++j) x[j] = A::load(raw[j], tab);
+        const uint64_t next = poly + gridDim.x;
+        const bool more = next < a.count;  // uniform over the workgroup
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) FwdRounds<A, LOGC, LOGT, 0>::load_raw(a.data + next * G::C, raw, t);
+        __builtin_amdgcn_sched_barrier(0);
+        FwdRounds<A, LOGC, LOGT, 0>::run(x, lds, t, 0, 0, tab);
+        after_forward<A, LOGC, LOGT, MODE>(a, x, lds, t, 0, limb, G::C, tab, a.data + poly * G::C);
+        if (!more) break;
+        poly = next;
     }
 }
 
@@ -200,6 +259,35 @@ static hipError_t launch_chunk(const KArgs &a, uint64_t count, hipStream_t s) {
     return hipGetLastError();
 }
 
+// One resident generation of stream_kernel workgroups on the current device.
+template <class A, int LOGC, int LOGT, int MODE>
+static uint32_t stream_slots() {
+    static std::atomic<uint32_t> cached{0};
+    uint32_t v = cached.load(std::memory_order_relaxed);
+    if (v) return v;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stream_kernel<A, LOGC, LOGT, MODE>, 1 << LOGT, 0) !=
+            hipSuccess)
+        return 0;
+    v = (uint32_t)std::max(1, cus * std::max(per_cu, 1));
+    cached.store(v, std::memory_order_relaxed);
+    return v;
+}
+
+template <class A, int LOGC, int LOGT, int MODE>
+static hipError_t launch_stream(const KArgs &a0, uint64_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    const uint32_t slots = stream_slots<A, LOGC, LOGT, MODE>();
+    if (slots == 0) return hipErrorInvalidDevice;
+    KArgs a = a0;
+    a.count = count;
+    const uint64_t grid = std::min<uint64_t>(count, slots);
+    stream_kernel<A, LOGC, LOGT, MODE><<<dim3((uint32_t)grid), dim3(1u << LOGT), 0, s>>>(a);
+    return hipGetLastError();
+}
+
 template <int LOGR, int MODE>
 static hipError_t launch_column(const KArgs &a, uint32_t log_n, uint64_t count, hipStream_t s) {
     const uint64_t threads = count << (log_n - LOGR);
@@ -208,31 +296,44 @@ static hipError_t launch_column(const KArgs &a, uint32_t log_n, uint64_t count, 
     return hipGetLastError();
 }
 
-// Single-pass chunk launch for log_n = LOGN <= 13, E = 2^LOGE coefficients per
-// thread, the mode chosen at run time.
+// Chunk or streaming launch of one forward-first mode.
+template <class A, int LOGN, int LOGT, int MODE>
+static hipError_t launch_fwd_first(const KArgs &a, uint64_t count, bool stream, hipStream_t s) {
+    if constexpr (LOGN >= 10) {
+        if (stream) return launch_stream<A, LOGN, LOGT, MODE>(a, count, s);
+    }
+    return launch_chunk<A, LOGN, LOGT, MODE>(a, count, s);
+}
+
+// Single-pass launch for log_n = LOGN <= 13, E = 2^LOGE coefficients per
+// thread, the mode chosen at run time; `stream` selects the persistent
+// streaming kernel where the mode allows it.
 template <int LOGN, int LOGE>
-static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, bool f64, hipStream_t s) {
+static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, bool f64, bool stream, hipStream_t s) {
     constexpr int LOGT = LOGN > LOGE ? LOGN - LOGE : 0;
     a.blocks_per_poly = 1;
     if constexpr (LOGN >= 6) {
         if (f64) {
             switch (op) {
                 case Op::FwdLazy: break;  // lazy values are defined by the integer butterflies
-                case Op::FwdCanon: return launch_chunk<F64Arith, LOGN, LOGT, M_FWD | M_CANON>(a, count, s);
+                case Op::FwdCanon:
+                    return launch_fwd_first<F64Arith, LOGN, LOGT, M_FWD | M_CANON>(a, count, stream, s);
                 case Op::Inv: return launch_chunk<F64Arith, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
                 case Op::FwdMulInv:
-                    return launch_chunk<F64Arith, LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count, s);
+                    return launch_fwd_first<F64Arith, LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(
+                        a, count, stream, s);
                 case Op::PolyMul:
                     return launch_chunk<F64Arith, LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
             }
         }
     }
     switch (op) {
-        case Op::FwdLazy: return launch_chunk<IntArith, LOGN, LOGT, M_FWD>(a, count, s);
-        case Op::FwdCanon: return launch_chunk<IntArith, LOGN, LOGT, M_FWD | M_CANON>(a, count, s);
+        case Op::FwdLazy: return launch_fwd_first<IntArith, LOGN, LOGT, M_FWD>(a, count, stream, s);
+        case Op::FwdCanon: return launch_fwd_first<IntArith, LOGN, LOGT, M_FWD | M_CANON>(a, count, stream, s);
         case Op::Inv: return launch_chunk<IntArith, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
         case Op::FwdMulInv:
-            return launch_chunk<IntArith, LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count, s);
+            return launch_fwd_first<IntArith, LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count,
+                                                                                                  stream, s);
         case Op::PolyMul:
             return launch_chunk<IntArith, LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
     }
@@ -242,11 +343,12 @@ static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, bool f64, hipStr
 // Coefficients per thread: 16 by default; 8 (twice the waves, one more LDS
 // round) for 2^10 <= N <= 2^13 when the tuning asks for it.
 template <int LOGN>
-static hipError_t single_pass(Op op, KArgs a, uint64_t count, bool f64, int loge, hipStream_t s) {
+static hipError_t single_pass(Op op, KArgs a, uint64_t count, bool f64, int shape, hipStream_t s) {
+    const bool stream = (shape & kShapeStream) != 0;
     if constexpr (LOGN >= 10) {
-        if (loge == 3) return single_pass_e<LOGN, 3>(op, a, count, f64, s);
+        if ((shape & 15) == 3) return single_pass_e<LOGN, 3>(op, a, count, f64, stream, s);
     }
-    return single_pass_e<LOGN, 4>(op, a, count, f64, s);
+    return single_pass_e<LOGN, 4>(op, a, count, f64, stream, s);
 }
 
 // Two-pass schedule for log_n = 12 + LOGR: columns of 2^LOGR, chunks of 4096.
@@ -301,21 +403,21 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
 bool needs_scratch(Op op, uint32_t log_n) { return op == Op::PolyMul && log_n > kMaxSinglePassLogN; }
 
 hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, bool f64,
-                            int loge, hipStream_t s) {
+                            int shape, hipStream_t s) {
     switch (log_n) {
-        case 1: return single_pass<1>(op, a, count, f64, loge, s);
-        case 2: return single_pass<2>(op, a, count, f64, loge, s);
-        case 3: return single_pass<3>(op, a, count, f64, loge, s);
-        case 4: return single_pass<4>(op, a, count, f64, loge, s);
-        case 5: return single_pass<5>(op, a, count, f64, loge, s);
-        case 6: return single_pass<6>(op, a, count, f64, loge, s);
-        case 7: return single_pass<7>(op, a, count, f64, loge, s);
-        case 8: return single_pass<8>(op, a, count, f64, loge, s);
-        case 9: return single_pass<9>(op, a, count, f64, loge, s);
-        case 10: return single_pass<10>(op, a, count, f64, loge, s);
-        case 11: return single_pass<11>(op, a, count, f64, loge, s);
-        case 12: return single_pass<12>(op, a, count, f64, loge, s);
-        case 13: return single_pass<13>(op, a, count, f64, loge, s);
+        case 1: return single_pass<1>(op, a, count, f64, shape, s);
+        case 2: return single_pass<2>(op, a, count, f64, shape, s);
+        case 3: return single_pass<3>(op, a, count, f64, shape, s);
+        case 4: return single_pass<4>(op, a, count, f64, shape, s);
+        case 5: return single_pass<5>(op, a, count, f64, shape, s);
+        case 6: return single_pass<6>(op, a, count, f64, shape, s);
+        case 7: return single_pass<7>(op, a, count, f64, shape, s);
+        case 8: return single_pass<8>(op, a, count, f64, shape, s);
+        case 9: return single_pass<9>(op, a, count, f64, shape, s);
+        case 10: return single_pass<10>(op, a, count, f64, shape, s);
+        case 11: return single_pass<11>(op, a, count, f64, shape, s);
+        case 12: return single_pass<12>(op, a, count, f64, shape, s);
+        case 13: return single_pass<13>(op, a, count, f64, shape, s);
         case 14: return two_pass<2>(op, a, count, scratch, s);
         case 15: return two_pass<3>(op, a, count, scratch, s);
         case 16: return two_pass<4>(op, a, count, scratch, s);

@@ -18,10 +18,11 @@ struct KArgs {
     uint64_t *data;          // in/out limb-polynomials, [count][N]
     const uint64_t *data2;   // second operand (poly-multiply) / column-pass source
     const Shoup *operand;    // Shoup-prepared NTT-domain operand, [L][N]
-    const uint64_t *operand_v; // the same operand's values alone (binary64 path, q < 2^50)
+    const ShoupF *operand_f; // the same operand as centred binary64 pairs (q < 2^50)
     const DevTable *tabs;    // L device tables; limb-polynomial i uses tabs[i % L]
     uint32_t L;
     uint32_t blocks_per_poly;
+    uint64_t count;          // limb-polynomials of the launch (persistent kernels)
 };
 
 enum class Op : int {
@@ -33,11 +34,17 @@ enum class Op : int {
 };
 
 bool needs_scratch(Op op, uint32_t log_n);
+// Launch shape of the single-pass kernels (N <= 2^13): log2 of the
+// coefficients per thread (3 or 4) in the low bits, kShapeStream set for the
+// persistent streaming kernel (forward-first operations, 2^10 <= N <= 2^13).
+// 0 is the default: 16 coefficients per thread, one workgroup per polynomial.
+constexpr int kShapeStream = 16;
+inline bool valid_shape(int shape) { return shape == 0 || shape == 3 || shape == 4 || shape == 19 || shape == 20; }
+
 // f64: use F64Arith where it applies (every modulus < 2^50, canonical outputs,
-// 2^6 <= N <= 2^13); otherwise IntArith.  loge: log2 of the coefficients per
-// thread of single-pass kernels (3 or 4; anything else means the default 4).
+// 2^6 <= N <= 2^13); otherwise IntArith.
 hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, bool f64,
-                            int loge, hipStream_t s);
+                            int shape, hipStream_t s);
 hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, uint64_t coeff,
                            uint64_t degree, int kind, hipStream_t s);
 hipError_t launch_pointwise(const DevTable *tabs, uint32_t L, uint32_t log_n, uint64_t total,

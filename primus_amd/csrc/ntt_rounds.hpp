@@ -70,7 +70,14 @@ struct FwdRounds {
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j)
-                x[(m << R) + j] = A::load(g[fwd_slot<LOGC, LOGT, S, R>(t, m, j)], tab);
+                x[(m << R) + j] = A::load(ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(t, m, j)), tab);
+    }
+    // The words load_global reads, unconverted, in the same register order.
+    __device__ __forceinline__ static void load_raw(const uint64_t *__restrict__ g, uint64_t *r, uint32_t t) {
+#pragma unroll
+        for (int m = 0; m < (G::E >> R); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R); ++j) r[(m << R) + j] = ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(t, m, j));
     }
 
     // Runs rounds I .. NR-1.  st0: global stage of local stage 0; b: chunk index
@@ -119,7 +126,7 @@ struct InvRounds {
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j)
-                g[inv_slot<LOGC, LOGT, P, R>(t, m, j)] = A::out_inv(x[(m << R) + j], tab);
+                st_stream(g + inv_slot<LOGC, LOGT, P, R>(t, m, j), A::out_inv(x[(m << R) + j], tab));
     }
 
     __device__ __forceinline__ static void run(T *x, T *lds, uint32_t t, uint32_t n, uint32_t b,

@@ -291,10 +291,10 @@ def test_launch_shapes_agree(log_n, bits):
     opv = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in moduli])
     op = pa.Operand(tab, opv)
     try:
-        for loge in (3, 4):
+        for shape in tuning.SHAPES:
             for name in ("forward", "inverse", "fwd_mul_inv", "negacyclic_mul", "forward_lazy"):
-                tuning.set_shape(log_n, name, arith, loge)
-                tuning.set_shape(log_n, name, "int", loge)
+                tuning.set_shape(log_n, name, arith, shape)
+                tuning.set_shape(log_n, name, "int", shape)
             d = pa.DeviceBuffer.from_array(a)
             pa.fwd_mul_inv_device(tab, d.ptr, count, op)
             pa.synchronize()
@@ -323,9 +323,59 @@ def test_launch_shapes_agree(log_n, bits):
             pa.inverse_transform_device(tab, d.ptr, count)
             pa.synchronize()
             np.testing.assert_array_equal(d.download((count, n)), a)
-        loge, ms = tuning.autotune(tab, "fwd_mul_inv", 64, reps=3)
-        assert loge in (3, 4) and ms > 0
+        shape, ms = tuning.autotune(tab, "fwd_mul_inv", 64, reps=3)
+        assert shape in tuning.SHAPES and ms > 0
     finally:
         for name in ("forward", "inverse", "fwd_mul_inv", "negacyclic_mul", "forward_lazy"):
             tuning.set_shape(log_n, name, "f64", 0)
+            tuning.set_shape(log_n, name, "int", 0)
+
+
+@pytest.mark.parametrize("log_n", [10, 11, 12, 13])
+@pytest.mark.parametrize("bits", [50, 59])
+def test_streaming_shapes_long_batch(log_n, bits):
+    """The persistent streaming kernels on batches several resident generations
+    long (every workgroup walks many polynomials): forward canonical, forward
+    lazy and fused forward -> multiply -> inverse equal the oracle."""
+    from primus_amd import tuning
+
+    n = 1 << log_n
+    L = 3
+    count = (1 << 23) // n + 5  # 8 Mi coefficients: above four generations of every shape
+    moduli = pa.ntt_primes_below(bits, 2 * n, L)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    tables = [O.Table(log_n, q) for q in moduli]
+    rng = np.random.default_rng(1000 + log_n + bits)
+    a = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    opv = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in moduli])
+    op = pa.Operand(tab, opv)
+    sample = sorted(set(range(6)) | set(range(count - 6, count)) |
+                    set(int(i) for i in rng.integers(0, count, size=12)))
+    d = pa.DeviceBuffer.from_array(a)
+    print("streaming", log_n, bits, count, flush=True)
+    try:
+        for shape in (3 | tuning.STREAM, 4 | tuning.STREAM):
+            for name in ("forward", "fwd_mul_inv", "forward_lazy"):
+                tuning.set_shape(log_n, name, tab.arith(), shape)
+                tuning.set_shape(log_n, name, "int", shape)
+            for kind in ("forward", "forward_lazy", "fwd_mul_inv"):
+                d.upload(a)
+                if kind == "fwd_mul_inv":
+                    pa.fwd_mul_inv_device(tab, d.ptr, count, op)
+                else:
+                    pa.transform_device(tab, d.ptr, count, lazy=(kind == "forward_lazy"))
+                pa.synchronize()
+                got = d.download((count, n))
+                for i in sample:
+                    t = tables[i % L]
+                    if kind == "forward":
+                        want = t.transform(a[i])
+                    elif kind == "forward_lazy":
+                        want = t.lazy_transform(a[i])
+                    else:
+                        want = t.inverse_transform(O.pointwise_mul(t.transform(a[i]), opv[i % L], t.q))
+                    np.testing.assert_array_equal(got[i], want, err_msg="%s shape %d poly %d" % (kind, shape, i))
+    finally:
+        for name in ("forward", "fwd_mul_inv", "forward_lazy"):
+            tuning.set_shape(log_n, name, tab.arith(), 0)
             tuning.set_shape(log_n, name, "int", 0)
