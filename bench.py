@@ -204,16 +204,27 @@ def secondary_rlwe(pa, timer, sp):
     da, dout = pa.DeviceBuffer.from_array(a), pa.DeviceBuffer(a_words)
     ds, de, dm = (pa.DeviceBuffer.from_array(as_words(v)) for v in (s, e, m))
 
+    # the batch layout is prepared once (work items, scratch); each step only
+    # enqueues the persistent kernel, whose grid the plan's autotuner picks
+    plan = ctx.plan(reqs)
+    grid, _ = plan.autotune(da.ptr, ds.ptr, de.ptr, dm.ptr, dout.ptr, reps=3)
+
     def step(i):
-        ctx.encrypt_device(reqs, da.ptr, ds.ptr, de.ptr, dm.ptr, dout.ptr, stream=sp)
+        plan.encrypt_device(da.ptr, ds.ptr, de.ptr, dm.ptr, dout.ptr, stream=sp)
 
     per, tot = timer.run(step, 100, 5)
     ms = tot / 100
+    try:
+        graph_ms = timer.run_graph(step, 4, 100, 4)
+    except Exception:  # capture unsupported: stream launches only
+        graph_ms = None
+    best = min(ms, graph_ms) if graph_ms else ms
     ntts = 2 * sum(L for _, L in shapes)
     return {"workload": "1024 RLWE encryptions, N~U{2^10..2^15}, L~U{1..12}, one persistent launch",
-            "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
-            "ntt_per_s": ntts / (ms * 1e-3), "coeff_limbs": int(a_words),
-            "algorithmic_gbps": (16 * a_words + 3 * small) / (ms * 1e-3) / 1e9}
+            "ms_per_batch": best, "stream_launch_ms": ms, "graph_replay_ms": graph_ms, "persistent_grid": grid,
+            "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+            "ntt_per_s": ntts / (best * 1e-3), "coeff_limbs": int(a_words),
+            "algorithmic_gbps": (16 * a_words + 3 * small) / (best * 1e-3) / 1e9}
 
 
 def secondary_ext_product(pa, timer, sp):
@@ -225,17 +236,58 @@ def secondary_ext_product(pa, timer, sp):
     rng = np.random.default_rng(SEED + 4)
     key = ExtKey(pa.UintNttTable.new(log_n, q), 8, 3, rng.integers(0, q, size=(2, 3, 2, n), dtype=np.uint64))
     x = rng.integers(0, q, size=(count, 2, n), dtype=np.uint64)
-    din, dout = pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(count * 2 * n)
+    # 6 rotating input/output pairs (384 MiB): the batch is streamed from HBM,
+    # not served from the 256 MiB Infinity Cache
+    rot = 6
+    pairs = [(pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(count * 2 * n)) for _ in range(rot)]
 
     def step(i):
+        din, dout = pairs[i % rot]
         key.product_device(din.ptr, dout.ptr, count, stream=sp)
 
     per, tot = timer.run(step, 1000, 20)
     ms = tot / 1000
+    try:
+        graph_ms = timer.run_graph(step, rot, 1000, 20)
+    except Exception:  # capture unsupported: stream launches only
+        graph_ms = None
+    best = min(ms, graph_ms) if graph_ms else ms
     return {"workload": "2048 RLWE x RGSW external products, N=1024, q=4294957057, B=2^8, l=3, one launch",
+            "ms_per_batch": best, "stream_launch_ms": ms, "graph_replay_ms": graph_ms,
+            "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+            "ntt_per_s": 8 * count / (best * 1e-3), "algorithmic_gbps": 4 * n * 8 * count / (best * 1e-3) / 1e9,
+            "rotating_buffers": rot, "arith": "f64"}
+
+
+def secondary_automorphism(pa, timer, sp):
+    """NTT-domain automorphism x -> x^5 on the headline batch shape (1024 limb-polys, N=4096)."""
+    moduli = pa.ntt_primes_below(50, 2 * N, LIMBS)
+    tab = pa.UintCrtNttTable.new(LOG_N, moduli)
+    rng = np.random.default_rng(SEED + 7)
+    x = np.stack([rng.integers(0, moduli[i % LIMBS], size=N, dtype=np.uint64) for i in range(COUNT)])
+    rot = 6  # 6 x 64 MiB in + out: streamed from HBM
+    pairs = [(pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(COUNT * N)) for _ in range(rot)]
+    L = pa.lib()
+
+    def step(i):
+        din, dout = pairs[i % rot]
+        L.ph_dcrt_automorphism_device(tab.handle, din.ptr, dout.ptr, COUNT, 5, sp)
+
+    per, tot = timer.run(step, 1000, 20)
+    ms = tot / 1000
+    return {"workload": "NTT-domain automorphism x -> x^5, 1024 limb-polys of N=4096, out of place, one launch",
             "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
-            "ntt_per_s": 8 * count / (ms * 1e-3), "algorithmic_gbps": 4 * n * 8 * count / (ms * 1e-3) / 1e9,
-            "arith": "f64"}
+            "algorithmic_gbps": 16 * COUNT * N / (ms * 1e-3) / 1e9,
+            "hbm_frac": 16 * COUNT * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, "rotating_buffers": rot}
+
+
+def kernel_signature(shape):
+    """The fused kernel a tuning shape launches, as rocprofv3 names it (without 'void ')."""
+    from primus_amd import tuning
+
+    per_thread, streaming = tuning.describe(shape)
+    return "ph::%s<ph::F64Arith, %d, %d, 39>" % ("stream_kernel" if streaming else "chunk_kernel", LOG_N,
+                                                 LOG_N - (per_thread.bit_length() - 1))
 
 
 def serving_pool(pa, grid, ext_q, pool, seed):
@@ -265,6 +317,12 @@ def serving_pool(pa, grid, ext_q, pool, seed):
     return reqs
 
 
+def request_bytes(r):
+    """(host->device, device->host) bytes of one serving request."""
+    up = sum(a.nbytes for a in (r.in0, r.in1, r.s, r.e, r.m) if a is not None)
+    return up, r.out.nbytes
+
+
 def serving_main(args):
     """Serving stream: Poisson arrivals at the offered rates, continuous batching
     against the plain path (one request at a time, one launch per
@@ -280,6 +338,8 @@ def serving_main(args):
     key_np = rng.integers(0, ext_q, size=(2, 3, 2, 1024), dtype=np.uint64)
     key = ExtKey(pa.UintNttTable.new(10, ext_q), 8, 3, key_np)
     pool = serving_pool(pa, grid, ext_q, args.serving_pool, SEED + 6)
+    if not args.serving_pageable:
+        pool = [r.pinned_copy() for r in pool]  # page-locked: no staging copies in the server
     budget = 64 << 30
 
     def run(srv, rate, count, seed):
@@ -321,11 +381,30 @@ def serving_main(args):
                 return 1
         checked += 1
 
+    # the host link both paths share: page-locked 256 MiB copies each way
+    link = {}
+    hbuf = sv.PinnedBuffer((32 << 20,), np.uint64)
+    dbuf = pa.DeviceBuffer(32 << 20)
+    Lh = pa.lib()
+    for name, fn, a, b in (("h2d_gbps", Lh.ph_memcpy_htod, dbuf.ptr, hbuf.array.ctypes.data_as(ctypes.c_void_p)),
+                           ("d2h_gbps", Lh.ph_memcpy_dtoh, hbuf.array.ctypes.data_as(ctypes.c_void_p), dbuf.ptr)):
+        fn(a, b, 256 << 20)
+        t0 = time.perf_counter()
+        for _ in range(4):
+            fn(a, b, 256 << 20)
+        link[name] = 4 * (256 << 20) / (time.perf_counter() - t0) / 1e9
+    del hbuf, dbuf
+
     out = {"metric": "serving stream: p50/p99 latency and throughput, continuous batching vs plain per-request launches",
            "workload": "Poisson arrivals; mix 50% negacyclic product / 30% RLWE encryption (N~U{2^10..2^16}, "
                        "L~U{1..16}, 59-bit primes) / 20% external product (N=1024, q=4294957057, B=2^8, l=3); "
-                       "host buffers in and out (PCIe included); 64 GiB HBM admission budget",
-           "pool_distinct_requests": len(pool), "max_batch": args.serving_max_batch, "rates": []}
+                       "host buffers in and out (PCIe included; " +
+                       ("pageable, staged by the server" if args.serving_pageable else "page-locked, copied directly") +
+                       "); 64 GiB HBM admission budget",
+           "pool_distinct_requests": len(pool), "max_batch": args.serving_max_batch, "host_link": link,
+           "mean_request_bytes": {"h2d": float(np.mean([request_bytes(r)[0] for r in pool])),
+                                  "d2h": float(np.mean([request_bytes(r)[1] for r in pool]))},
+           "rates": []}
     for k, rate in enumerate(args.serving_rates):
         count = int(min(args.serving_requests, rate * args.serving_seconds))
         b = run(batched, rate, count, seed=11 + k)
@@ -355,6 +434,9 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="counter runs: only the fused batch kernel (--steps launches) and one 256 MiB "
                          "device-to-device copy as the byte-count calibration")
+    ap.add_argument("--shape", type=int, default=None,
+                    help="launch shape of the fused kernel instead of the autotuner's choice "
+                         "(3/4: 8/16 coefficients per thread; +16: streaming)")
     ap.add_argument("--serving", action="store_true",
                     help="the serving stream benchmark (Poisson arrivals, batched vs plain) instead of the headline")
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[1000.0, 10000.0, 50000.0])
@@ -364,6 +446,8 @@ def main():
     ap.add_argument("--serving-plain-requests", type=int, default=3000)
     ap.add_argument("--serving-pool", type=int, default=256)
     ap.add_argument("--serving-max-batch", type=int, default=256)
+    ap.add_argument("--serving-pageable", action="store_true",
+                    help="requests in ordinary (pageable) host memory, staged by the server")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -403,7 +487,11 @@ def main():
             L.ph_dcrt_fwd_mul_inv_device(tab.handle, ptrs[i % ROTATE], COUNT, operand_.handle, sp)
         return step
 
-    shape, shape_ms = tuning.autotune(table, "fwd_mul_inv", COUNT, reps=20)
+    if args.shape is None:
+        shape, shape_ms = tuning.autotune(table, "fwd_mul_inv", COUNT, reps=20)
+    else:  # a fixed launch shape (counter profiles of one kernel)
+        shape, shape_ms = args.shape, None
+        tuning.set_shape(LOG_N, "fwd_mul_inv", table.arith(), shape)
     step = launcher(table, op)
     int_table = pa.UintCrtNttTable.new(LOG_N, moduli, integer_only=True)
     int_op = pa.Operand(int_table, operand)
@@ -434,8 +522,8 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)  # calibration: 256 MiB read + 256 MiB written
         torch.cuda.synchronize()
-        print(json.dumps({"profile": True, "kernel_ms": total / K, "launches": K + args.warmup,
-                          "calibration_copy_bytes": src.numel() * 8}))
+        print(json.dumps({"profile": True, "kernel": kernel_signature(shape), "kernel_ms": total / K,
+                          "launches": K + args.warmup, "calibration_copy_bytes": src.numel() * 8}))
         return 0
 
     t_wall = time.perf_counter()
@@ -470,10 +558,15 @@ def main():
     plain_per, plain_total = timer.run(plain_step, args.plain_steps, 1)
     plain_ms = plain_total / args.plain_steps
 
-    traffic = None
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated),
+    # used only when they were taken on the kernel this run selected
+    kernel_sig = kernel_signature(shape)
+    traffic, traffic_note = None, "no counter profile of %s" % kernel_sig
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("traffic_bytes")
+            prof = json.load(f).get(kernel_sig)
+        if prof:
+            traffic, traffic_note = prof.get("traffic_bytes"), "counter profile of %s" % kernel_sig
 
     out = {
         "metric": "NTTs/sec + HBM GB/s at N=4096x4 limbs; p50/p99 latency, batched vs plain",
@@ -528,9 +621,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": traffic,
-            "kernel": "%s<F64Arith,12,%d,FWD|MUL_SHOUP|INV|INV_LAST> (fused fwd -> mul -> inv)" % (
-                "stream_kernel" if tuning.describe(shape)[1] else "chunk_kernel",
-                LOG_N - (tuning.describe(shape)[0].bit_length() - 1)),
+            "traffic_note": traffic_note,
+            "kernel": kernel_sig + " (MODE 39: fused forward -> Shoup multiply -> inverse)",
             "kernel_ms": kernel_ms,
             "bytes_per_launch": BYTES_PER_STEP,
         },
@@ -540,6 +632,7 @@ def main():
             "plain_n1024": secondary_plain_products(pa, timer, sp),
             "rlwe_uneven": secondary_rlwe(pa, timer, sp),
             "ext_product": secondary_ext_product(pa, timer, sp),
+            "automorphism": secondary_automorphism(pa, timer, sp),
         }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(moduli, polys, operand)

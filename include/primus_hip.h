@@ -178,6 +178,15 @@ int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *table, const uint64_t *d_a
 int ph_dcrt_pointwise_mul_add_device(const ph_dcrt_table *table, uint64_t *d_acc, const uint64_t *d_a,
                                      const uint64_t *d_b, size_t count, void *stream);
 
+/* NTT-domain automorphism x -> x^degree (degree odd) of `count` limb-polynomials
+ * in bit-reversed NTT order, out of place (d_in and d_out must not overlap):
+ * dcrt_poly_ntt_auto_inplace (primus_fhe_core/src/automorphism/glwe/dcrt.rs:119-131)
+ * with the permutation of generate_ntt_permutation (:43-61), computed per
+ * element instead of read from a table.  The permutation is the same for
+ * every limb; the table only fixes N. */
+int ph_dcrt_automorphism_device(const ph_dcrt_table *table, const uint64_t *d_in, uint64_t *d_out, size_t count,
+                                uint64_t degree, void *stream);
+
 /* ---- external product (RLWE x RGSW, k = 1, one modulus) ---- */
 
 typedef struct ph_ext_key ph_ext_key; /* device-resident RGSW/GGSW key in NTT form */
@@ -236,6 +245,25 @@ uint64_t ph_rlwe_ctx_modulus(const ph_rlwe_ctx *ctx, uint32_t log_n, uint32_t li
 int ph_rlwe_encrypt_device(const ph_rlwe_ctx *ctx, const ph_rlwe_request *requests, uint32_t n_requests,
                            const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m,
                            uint64_t *d_out, void *stream);
+
+/* A batch layout prepared once (no reference counterpart: launch planning):
+ * the per-limb work items with their Delta, sorted, uploaded, and the
+ * launch's scratch allocated.  ph_rlwe_encrypt_plan_device then only
+ * enqueues the persistent kernel -- no allocation, copy or synchronisation,
+ * so it can be captured in a HIP graph -- for any buffers laid out as the
+ * plan's requests.  The context must outlive the plan. */
+typedef struct ph_rlwe_plan ph_rlwe_plan;
+int ph_rlwe_plan_new(const ph_rlwe_ctx *ctx, const ph_rlwe_request *requests, uint32_t n_requests,
+                     ph_rlwe_plan **out);
+void ph_rlwe_plan_free(ph_rlwe_plan *plan);
+int ph_rlwe_encrypt_plan_device(const ph_rlwe_plan *plan, const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e,
+                                const uint8_t *d_m, uint64_t *d_out, void *stream);
+/* Times persistent grids of 1..4 workgroups per compute unit on the given
+ * buffers (outputs are overwritten) and keeps the fastest in the plan. */
+int ph_rlwe_plan_autotune(ph_rlwe_plan *plan, const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e,
+                          const uint8_t *d_m, uint64_t *d_out, int reps, uint32_t *grid, float *best_ms);
+/* Workgroups of the plan's launch. */
+uint32_t ph_rlwe_plan_grid(const ph_rlwe_plan *plan);
 
 /* ---- serving: continuous batching (no reference counterpart; the reference is a
  * single-threaded library, this is the request loop around the batched forms) ---- */
@@ -322,6 +350,11 @@ int ph_device_free(void *d_ptr);
 int ph_memcpy_htod(void *d_dst, const void *h_src, size_t bytes);
 int ph_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes);
 int ph_stream_synchronize(void *stream);
+/* Page-locked host memory (hipHostMalloc).  Serving requests whose buffers
+ * all come from here are copied to and from the device directly, without
+ * the server's staging copies. */
+int ph_host_alloc(size_t bytes, void **h_ptr);
+int ph_host_free(void *h_ptr);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,8 @@ def lib():
             ("po_basis_new", ctypes.c_int, [_u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
             ("po_decompose", None, [ctypes.c_void_p, _p64, ctypes.c_size_t, _p64]),
             ("po_ext_product", None, [ctypes.c_void_p, ctypes.c_void_p, _p64, _p64, _p64, ctypes.c_int]),
+            ("po_ntt_automorphism", None, [_p64, _p64, ctypes.c_size_t, _u64]),
+            ("po_coeff_automorphism", None, [_p64, _p64, ctypes.c_size_t, _u64, _u64]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -280,3 +282,19 @@ def rlwe_delta(moduli, qi):
     for q in moduli:
         Q *= q
     return (Q // 256) % qi
+
+
+def ntt_automorphism(values, degree):
+    """x -> x^degree (degree odd) on one polynomial in bit-reversed NTT order."""
+    x = np.ascontiguousarray(values, dtype=np.uint64)
+    out = np.zeros_like(x)
+    lib().po_ntt_automorphism(ptr(x), ptr(out), x.size, degree)
+    return out
+
+
+def coeff_automorphism(values, degree, q):
+    """a(X) -> a(X^degree) mod (X^N + 1, q) in the coefficient domain."""
+    x = np.ascontiguousarray(values, dtype=np.uint64)
+    out = np.zeros_like(x)
+    lib().po_coeff_automorphism(ptr(x), ptr(out), x.size, degree, q)
+    return out

@@ -599,3 +599,38 @@ void po_rlwe_encrypt(const po_table *t, const int8_t *s, const int8_t *e, const 
     po_pointwise_mul_add(b_hat, a_hat, sh, n, q);
     free(sh);
 }
+
+/* ------------------------------------------------------------------------- */
+/* NTT-domain automorphism                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* x -> x^degree on a polynomial in bit-reversed NTT order, degree odd.
+ * generate_ntt_permutation (primus_fhe_core/src/automorphism/glwe/dcrt.rs:43-61):
+ * natural slot i holds the evaluation at psi^(2i+1); sigma(f) evaluated there
+ * is f at psi^(degree*(2i+1) mod 2N), natural slot (j-1)/2; in bit-reversed
+ * storage out[rev(i)] = in[rev((j-1)/2)].  ntt_poly_auto_inplace (:97-112)
+ * applies the gather. */
+void po_ntt_automorphism(const uint64_t *in, uint64_t *out, size_t n, uint64_t degree) {
+    uint32_t log_n = 0;
+    while (((size_t)1 << log_n) < n) ++log_n;
+    const uint64_t two_n = 2 * (uint64_t)n;
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t j = (uint64_t)(((unsigned __int128)(degree % two_n) * (2 * i + 1)) % two_n);
+        const uint64_t target = (j - 1) / 2;
+        out[po_reverse_lsbs(i, log_n)] = in[po_reverse_lsbs(target, log_n)];
+    }
+}
+
+/* The same automorphism in the coefficient domain, a(X) -> a(X^degree) mod
+ * (X^N + 1, q): coefficient j moves to j*degree mod 2N, negated when that is
+ * N or more (X^N = -1).  Used to pin po_ntt_automorphism through the
+ * transform: NTT(a(X^k)) must equal the permuted NTT(a). */
+void po_coeff_automorphism(const uint64_t *in, uint64_t *out, size_t n, uint64_t degree, uint64_t q) {
+    const uint64_t two_n = 2 * (uint64_t)n;
+    for (size_t j = 0; j < n; ++j) {
+        const uint64_t e = (uint64_t)(((unsigned __int128)(degree % two_n) * j) % two_n);
+        const uint64_t v = in[j] % q;
+        if (e < n) out[e] = v;
+        else out[e - n] = v ? q - v : 0;
+    }
+}

@@ -117,6 +117,20 @@ void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, c
 }
 #endif
 #endif
+#ifndef PRIMUS_ORACLE_AUTO_H
+#define PRIMUS_ORACLE_AUTO_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* NTT-domain automorphism x -> x^degree (degree odd), bit-reversed order:
+ * primus_fhe_core/src/automorphism/glwe/dcrt.rs:43-61, 97-112 */
+void po_ntt_automorphism(const uint64_t *in, uint64_t *out, size_t n, uint64_t degree);
+/* The coefficient-domain automorphism a(X) -> a(X^degree) mod (X^N+1, q) (for pinning) */
+void po_coeff_automorphism(const uint64_t *in, uint64_t *out, size_t n, uint64_t degree, uint64_t q);
+#ifdef __cplusplus
+}
+#endif
+#endif
 #ifndef PRIMUS_ORACLE_RLWE_H
 #define PRIMUS_ORACLE_RLWE_H
 #ifdef __cplusplus

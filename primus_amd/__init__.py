@@ -14,6 +14,7 @@ from .ntt import (  # noqa: F401
     Operand,
     UintCrtNttTable,
     UintNttTable,
+    automorphism_device,
     fwd_mul_inv_device,
     inverse_transform_device,
     negacyclic_mul_device,

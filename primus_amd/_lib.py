@@ -142,6 +142,7 @@ def _declare(L):
         "ph_dcrt_negacyclic_mul_device": (i, [vp, vp, vp, sz, vp]),
         "ph_dcrt_pointwise_mul_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_pointwise_mul_add_device": (i, [vp, vp, vp, vp, sz, vp]),
+        "ph_dcrt_automorphism_device": (i, [vp, vp, vp, sz, u64, vp]),
         "ph_device_alloc": (i, [sz, pvp]),
         "ph_device_free": (i, [vp]),
         "ph_memcpy_htod": (i, [vp, vp, sz]),
@@ -155,6 +156,13 @@ def _declare(L):
         "ph_rlwe_ctx_free": (None, [vp]),
         "ph_rlwe_ctx_modulus": (u64, [vp, u32, u32]),
         "ph_rlwe_encrypt_device": (i, [vp, vp, u32, vp, vp, vp, vp, vp, vp]),
+        "ph_rlwe_plan_new": (i, [vp, vp, u32, pvp]),
+        "ph_rlwe_plan_free": (None, [vp]),
+        "ph_rlwe_encrypt_plan_device": (i, [vp, vp, vp, vp, vp, vp, vp]),
+        "ph_rlwe_plan_autotune": (i, [vp, vp, vp, vp, vp, vp, i, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float)]),
+        "ph_rlwe_plan_grid": (u32, [vp]),
+        "ph_host_alloc": (i, [sz, pvp]),
+        "ph_host_free": (i, [vp]),
         "ph_tuning_set": (i, [u32, i, i, i]),
         "ph_tuning_get": (i, [u32, i, i]),
         "ph_tuning_save": (i, [ctypes.c_char_p]),

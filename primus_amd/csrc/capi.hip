@@ -614,6 +614,19 @@ int ph_dcrt_pointwise_mul_add_device(const ph_dcrt_table *d, uint64_t *d_acc, co
     return e == hipSuccess ? PH_OK : hip_fail(e, "pointwise kernel");
 }
 
+int ph_dcrt_automorphism_device(const ph_dcrt_table *d, const uint64_t *d_in, uint64_t *d_out, size_t count,
+                                uint64_t degree, void *stream) {
+    if (!d || ((!d_in || !d_out) && count)) return fail(PH_ERR_INVALID, "null argument");
+    if ((degree & 1) == 0) return fail(PH_ERR_INVALID, "automorphism degree must be odd");
+    if (count == 0) return PH_OK;
+    const uint64_t bytes = (uint64_t)count << (d->log_n + 3);
+    const uintptr_t a = (uintptr_t)d_in, b = (uintptr_t)d_out;
+    if (a < b + bytes && b < a + bytes) return fail(PH_ERR_INVALID, "input and output overlap");
+    const uint32_t k = (uint32_t)(degree & ((2ull << d->log_n) - 1));
+    hipError_t e = launch_automorphism(d_in, d_out, d->log_n, k, count, (hipStream_t)stream);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "automorphism kernel");
+}
+
 int ph_device_alloc(size_t bytes, void **d_ptr) {
     if (!d_ptr) return fail(PH_ERR_INVALID, "null output");
     PH_HIP(hipMalloc(d_ptr, bytes));
@@ -637,6 +650,17 @@ int ph_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes) {
 
 int ph_stream_synchronize(void *stream) {
     PH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return PH_OK;
+}
+
+int ph_host_alloc(size_t bytes, void **h_ptr) {
+    if (!h_ptr) return fail(PH_ERR_INVALID, "null output");
+    PH_HIP(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault));
+    return PH_OK;
+}
+
+int ph_host_free(void *h_ptr) {
+    PH_HIP(hipHostFree(h_ptr));
     return PH_OK;
 }
 
@@ -823,13 +847,11 @@ uint64_t ph_rlwe_ctx_modulus(const ph_rlwe_ctx *c, uint32_t log_n, uint32_t limb
     return c->tables[(log_n - c->min_log_n) * c->max_limbs + limb]->q;
 }
 
-int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs, const uint64_t *d_a,
-                           const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m, uint64_t *d_out, void *stream) {
-    if (!c || (!reqs && n_reqs)) return fail(PH_ERR_INVALID, "null argument");
-    if (n_reqs == 0) return PH_OK;
-    if (!d_a || !d_s || !d_e || !d_m || !d_out) return fail(PH_ERR_INVALID, "null device buffer");
-    hipStream_t s = (hipStream_t)stream;
-    std::vector<RlweItem> items;
+// One work item per (request, limb), largest ring degrees first (the
+// workgroups' strided walk then balances), with Delta mod q_i.
+static int rlwe_items(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs,
+                      std::vector<RlweItem> &items) {
+    items.clear();
     for (uint32_t r = 0; r < n_reqs; ++r) {
         const ph_rlwe_request &q = reqs[r];
         if (q.log_n < c->min_log_n || q.log_n > c->max_log_n || q.limbs == 0 || q.limbs > c->max_limbs)
@@ -854,9 +876,20 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
             items.push_back(it);
         }
     }
-    // largest transforms first: the workgroups' strided walk then balances
     std::stable_sort(items.begin(), items.end(),
                      [](const RlweItem &x, const RlweItem &y) { return x.log_n > y.log_n; });
+    return PH_OK;
+}
+
+int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs, const uint64_t *d_a,
+                           const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m, uint64_t *d_out, void *stream) {
+    if (!c || (!reqs && n_reqs)) return fail(PH_ERR_INVALID, "null argument");
+    if (n_reqs == 0) return PH_OK;
+    if (!d_a || !d_s || !d_e || !d_m || !d_out) return fail(PH_ERR_INVALID, "null device buffer");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<RlweItem> items;
+    int rc = rlwe_items(c, reqs, n_reqs, items);
+    if (rc != PH_OK) return rc;
     // persistent: two resident workgroups per compute unit (the kernel's occupancy)
     const uint32_t grid = (uint32_t)std::min<uint64_t>(items.size(), (uint64_t)std::max(c->cus, 1) * 2);
     RlweItem *d_items = nullptr;
@@ -883,6 +916,113 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     (void)hipFreeAsync(d_items, s);
     return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
 }
+
+struct ph_rlwe_plan {
+    const ph_rlwe_ctx *ctx = nullptr;
+    RlweItem *d_items = nullptr;
+    uint32_t n_items = 0;
+    uint32_t grid = 0, max_grid = 0;
+    uint64_t *scratch = nullptr;
+    uint64_t scratch_words = 0;
+};
+
+void ph_rlwe_plan_free(ph_rlwe_plan *p) {
+    if (!p) return;
+    if (p->d_items) (void)hipFree(p->d_items);
+    if (p->scratch) (void)hipFree(p->scratch);
+    delete p;
+}
+
+int ph_rlwe_plan_new(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs, ph_rlwe_plan **out) {
+    if (!c || !out || !reqs || n_reqs == 0) return fail(PH_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    std::vector<RlweItem> items;
+    int rc = rlwe_items(c, reqs, n_reqs, items);
+    if (rc != PH_OK) return rc;
+    ph_rlwe_plan *p = new (std::nothrow) ph_rlwe_plan();
+    if (!p) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    p->ctx = c;
+    p->n_items = (uint32_t)items.size();
+    // room for up to four workgroups per compute unit (the autotuner's range);
+    // two, the kernel's occupancy, by default
+    const uint64_t cus = (uint64_t)std::max(c->cus, 1);
+    p->max_grid = (uint32_t)std::min<uint64_t>(items.size(), 4 * cus);
+    p->grid = (uint32_t)std::min<uint64_t>(items.size(), 2 * cus);
+    p->scratch_words = items.front().log_n > 12 ? 1ull << items.front().log_n : 0;
+    rc = upload((void **)&p->d_items, items.data(), items.size() * sizeof(RlweItem));
+    if (rc == PH_OK && p->scratch_words && hipMalloc((void **)&p->scratch, (size_t)p->max_grid * p->scratch_words * 8) !=
+                                               hipSuccess)
+        rc = fail(PH_ERR_NO_MEMORY, "RLWE scratch");
+    if (rc != PH_OK) {
+        ph_rlwe_plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return PH_OK;
+}
+
+int ph_rlwe_encrypt_plan_device(const ph_rlwe_plan *p, const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e,
+                                const uint8_t *d_m, uint64_t *d_out, void *stream) {
+    if (!p) return fail(PH_ERR_INVALID, "null plan");
+    if (!d_a || !d_s || !d_e || !d_m || !d_out) return fail(PH_ERR_INVALID, "null device buffer");
+    RlweArgs a{};
+    a.items = p->d_items;
+    a.n_items = p->n_items;
+    a.tabs = p->ctx->d_tabs;
+    a.a = d_a;
+    a.out = d_out;
+    a.s = d_s;
+    a.e = d_e;
+    a.m = d_m;
+    a.scratch = p->scratch;
+    a.scratch_words = p->scratch_words;
+    hipError_t e = launch_rlwe_encrypt(a, p->grid, (hipStream_t)stream);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
+}
+
+int ph_rlwe_plan_autotune(ph_rlwe_plan *p, const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e,
+                          const uint8_t *d_m, uint64_t *d_out, int reps, uint32_t *grid, float *best_ms) {
+    if (!p || reps < 1) return fail(PH_ERR_INVALID, "bad autotune request");
+    const uint64_t cus = (uint64_t)std::max(p->ctx->cus, 1);
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PH_OK;
+    float best = 1e30f;
+    uint32_t best_grid = p->grid;
+    if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail(PH_ERR_DEVICE, "autotune set-up failed");
+    for (uint64_t per_cu = 1; per_cu <= 4 && rc == PH_OK; ++per_cu) {
+        const uint32_t g = (uint32_t)std::min<uint64_t>(p->n_items, per_cu * cus);
+        p->grid = g;
+        for (int r = 0; r <= reps && rc == PH_OK; ++r) {
+            if (r == 1) (void)hipEventRecord(e0, s);
+            rc = ph_rlwe_encrypt_plan_device(p, d_a, d_s, d_e, d_m, d_out, s);
+        }
+        if (rc != PH_OK) break;
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(PH_ERR_DEVICE, "autotune timing failed");
+            break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= reps;
+        if (ms < best) {
+            best = ms;
+            best_grid = g;
+        }
+        if (g == p->n_items) break;  // larger grids would be the same launch
+    }
+    p->grid = best_grid;
+    if (grid) *grid = best_grid;
+    if (best_ms) *best_ms = best;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
+}
+
+uint32_t ph_rlwe_plan_grid(const ph_rlwe_plan *p) { return p ? p->grid : 0; }
 
 static_assert(PH_SHAPE_STREAM == kShapeStream, "the header's shape flag is the launcher's");
 

@@ -234,6 +234,43 @@ __global__ void monomial_kernel(const DevTable *tabs, uint64_t *out, uint64_t co
     out[k] = v;
 }
 
+// Source slot of output slot d under x -> x^k in bit-reversed NTT order:
+// natural slot i = rev(d) holds the evaluation at psi^(2i+1); sigma_k(f) there
+// is f at psi^(k(2i+1) mod 2N), natural slot (j-1)/2 (generate_ntt_permutation,
+// primus_fhe_core/src/automorphism/glwe/dcrt.rs:43-61).  k is odd and < 2N.
+__device__ __forceinline__ uint32_t auto_source(uint32_t d, uint32_t log_n, uint32_t k) {
+    const uint32_t i = __builtin_bitreverse32(d) >> (32 - log_n);
+    const uint32_t j = (uint32_t)(((uint64_t)k * (2u * i + 1u)) & ((2ull << log_n) - 1));
+    return __builtin_bitreverse32((j - 1u) >> 1) >> (32 - log_n);
+}
+
+// out[d] = in[src(d)] on every limb-polynomial (ntt_poly_auto_inplace,
+// dcrt.rs:97-112).  Rows of up to 2^12 words are staged in LDS so that HBM
+// sees only whole-row reads and writes; longer rows gather from global
+// memory, where the row's lines stay in L2 between their reads.
+__global__ __launch_bounds__(256) void automorphism_kernel(const uint64_t *in, uint64_t *out, uint32_t log_n,
+                                                           uint32_t k, uint64_t rows) {
+    __shared__ uint64_t lds[1u << 12];
+    const uint32_t n = 1u << log_n;
+    const uint32_t t = threadIdx.x;
+    if (log_n <= 12) {
+        // one workgroup per row
+        const uint64_t row = blockIdx.x;
+        const uint64_t *src = in + row * n;
+        uint64_t *dst = out + row * n;
+        for (uint32_t d = t; d < n; d += 256) lds[d] = ld_stream(src + d);
+        __syncthreads();
+        for (uint32_t d = t; d < n; d += 256) st_stream(dst + d, lds[auto_source(d, log_n, k)]);
+        return;
+    }
+    // 2^12 < N: 256 consecutive output slots per workgroup
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + t;
+    if (g >= (rows << log_n)) return;
+    const uint64_t row = g >> log_n;
+    const uint32_t d = (uint32_t)(g & (n - 1));
+    st_stream(out + g, in[(row << log_n) + auto_source(d, log_n, k)]);
+}
+
 // out = a * b (mod q) or out = acc + a * b (mod q), limb-major [count][N],
 // limb = poly % L.  primus_poly/src/ntt/mul.rs:87-119, primus_poly/src/ntt/mod.rs:97-108.
 __global__ void pointwise_kernel(const DevTable *tabs, uint32_t L, uint32_t log_n, uint64_t total,
@@ -431,6 +468,14 @@ hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, 
     const uint32_t n = 1u << log_n;
     const uint32_t tpb = n < 256 ? n : 256;
     monomial_kernel<<<dim3((n + tpb - 1) / tpb), dim3(tpb), 0, s>>>(tabs, out, coeff, degree, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_automorphism(const uint64_t *in, uint64_t *out, uint32_t log_n, uint32_t k, uint64_t rows,
+                               hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    const uint64_t blocks = log_n <= 12 ? rows : ((rows << log_n) + 255) / 256;
+    automorphism_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(in, out, log_n, k, rows);
     return hipGetLastError();
 }
 

@@ -47,6 +47,9 @@ hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t coun
                             int shape, hipStream_t s);
 hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, uint64_t coeff,
                            uint64_t degree, int kind, hipStream_t s);
+// NTT-domain automorphism x -> x^k (k odd, < 2N) of `rows` rows of 2^log_n words.
+hipError_t launch_automorphism(const uint64_t *in, uint64_t *out, uint32_t log_n, uint32_t k, uint64_t rows,
+                               hipStream_t s);
 hipError_t launch_pointwise(const DevTable *tabs, uint32_t L, uint32_t log_n, uint64_t total,
                             const uint64_t *a, const uint64_t *b, uint64_t *out, int mac,
                             hipStream_t s);

@@ -42,6 +42,7 @@ struct Ticket {
     Clock::time_point arrival, done_at;
     int status = -1;  // -1 pending, else a PH_* code
     uint64_t bytes = 0;
+    bool direct = false;  // every buffer page-locked: copied without staging
 };
 
 struct Part {          // one request's place in a batch
@@ -182,6 +183,36 @@ struct ph_server {
 
 namespace {
 
+// True when [p, p + bytes) starts and ends in page-locked host memory, which
+// the DMA engines read and write directly.
+bool page_locked(const void *p, uint64_t bytes) {
+    if (!p || bytes == 0) return false;
+    for (const char *q : {(const char *)p, (const char *)p + bytes - 1}) {
+        hipPointerAttribute_t at{};
+        if (hipPointerGetAttributes(&at, q) != hipSuccess) {
+            (void)hipGetLastError();  // pageable memory reports an error: clear it
+            return false;
+        }
+        if (at.type != hipMemoryTypeHost) return false;
+    }
+    return true;
+}
+
+// Whether every buffer of a (validated) request is page-locked.
+bool request_direct(const ph_serve_request &r) {
+    const uint64_t n = 1ull << r.log_n;
+    switch (r.kind) {
+        case PH_REQ_POLY_MUL:
+            return page_locked(r.in0, r.limbs * n * 8) && page_locked(r.in1, r.limbs * n * 8) &&
+                   page_locked(r.out, r.limbs * n * 8);
+        case PH_REQ_RLWE_ENCRYPT:
+            return page_locked(r.in0, r.limbs * n * 8) && page_locked(r.s, n) && page_locked(r.e, n) &&
+                   page_locked(r.m, n) && page_locked(r.out, r.limbs * n * 8);
+        case PH_REQ_EXT_PRODUCT: return page_locked(r.in0, 2 * n * 8) && page_locked(r.out, 2 * n * 8);
+    }
+    return false;
+}
+
 // Device bytes a request holds while in flight (inputs, outputs, scratch).
 uint64_t request_bytes(const ph_serve_request &r) {
     const uint64_t n = 1ull << r.log_n;
@@ -309,19 +340,29 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
     }
     b.arena_bytes = off;
 
-    if (ensure_pinned(sl, off) != PH_OK) return PH_ERR_NO_MEMORY;
+    // Inputs: page-locked requests are copied to the arena directly; the
+    // others through the slot's staging mirror of the arena (one copy, issued
+    // first so that the direct copies land over it).
+    size_t staged = 0;
+    for (const Ticket *t : tk) staged += t->direct ? 0 : 1;
+    if (staged && ensure_pinned(sl, off) != PH_OK) return PH_ERR_NO_MEMORY;
     char *host = (char *)sl.pinned;
-    std::vector<CopyJob> jobs;
+    std::vector<CopyJob> jobs;  // staging copies
+    std::vector<std::tuple<uint64_t, const void *, uint64_t>> direct;  // (arena offset, source, bytes)
     for (size_t i = 0; i < tk.size(); ++i) {
         const ph_serve_request &r = tk[i]->req;
         const Part &p = b.parts[i];
         const uint64_t n = 1ull << r.log_n;
-        jobs.push_back({host + p.in0, r.in0, p.out_bytes});
-        if (r.kind == PH_REQ_POLY_MUL) jobs.push_back({host + p.in1, r.in1, p.out_bytes});
+        auto add = [&](uint64_t at, const void *src, uint64_t bytes) {
+            if (tk[i]->direct) direct.emplace_back(at, src, bytes);
+            else jobs.push_back({host + at, src, bytes});
+        };
+        add(p.in0, r.in0, p.out_bytes);
+        if (r.kind == PH_REQ_POLY_MUL) add(p.in1, r.in1, p.out_bytes);
         if (r.kind == PH_REQ_RLWE_ENCRYPT) {
-            jobs.push_back({host + rs + p.small, r.s, n});
-            jobs.push_back({host + re + p.small, r.e, n});
-            jobs.push_back({host + rm + p.small, r.m, n});
+            add(rs + p.small, r.s, n);
+            add(re + p.small, r.e, n);
+            add(rm + p.small, r.m, n);
         }
     }
     s->copy_in.run(jobs);
@@ -329,7 +370,11 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
     if (hipMallocAsync(&arena, std::max<uint64_t>(off, kAlign), st) != hipSuccess) return PH_ERR_NO_MEMORY;
     char *dev = (char *)arena;
     int rc = PH_OK;
-    if (hipMemcpyAsync(dev, host, off, hipMemcpyHostToDevice, st) != hipSuccess) rc = PH_ERR_DEVICE;
+    if (staged && hipMemcpyAsync(dev, host, off, hipMemcpyHostToDevice, st) != hipSuccess) rc = PH_ERR_DEVICE;
+    for (const auto &[at, src, bytes] : direct) {
+        if (rc != PH_OK) break;
+        if (hipMemcpyAsync(dev + at, src, bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = PH_ERR_DEVICE;
+    }
     for (const GroupAt &g : gat) {
         if (rc != PH_OK) break;
         const uint32_t L = (uint32_t)ph_dcrt_moduli_count(g.t);
@@ -357,16 +402,29 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
                                    0, st);
         ++s->stats.launches;
     }
-    // results back to the staging area, one copy per contiguous result region
-    // (each product group's a blocks, the RLWE b_hat rows, the external
-    // products' outputs), then the event the completer waits on
+    // Results: page-locked requests straight into the caller's buffer; the
+    // others into the staging area (one copy per contiguous result region --
+    // each product group's a blocks, the RLWE b_hat rows, the external
+    // products' outputs -- when no request of the batch is direct), for the
+    // completion thread to hand over.  Then the event the completer waits on.
     std::vector<std::pair<uint64_t, uint64_t>> regions;  // (offset, bytes)
-    for (const GroupAt &g : gat) regions.push_back({g.a, g.b - g.a});
-    if (!rlwe.empty()) regions.push_back({rout, a_words * 8});
-    if (!ext.empty()) regions.push_back({xout, ext_row * ext.size()});
+    if (staged == tk.size()) {
+        for (const GroupAt &g : gat) regions.push_back({g.a, g.b - g.a});
+        if (!rlwe.empty()) regions.push_back({rout, a_words * 8});
+        if (!ext.empty()) regions.push_back({xout, ext_row * ext.size()});
+    } else {
+        for (size_t i = 0; i < tk.size(); ++i)
+            if (!tk[i]->direct) regions.push_back({b.parts[i].out, b.parts[i].out_bytes});
+    }
     for (const auto &rg : regions) {
         if (rc != PH_OK || rg.second == 0) continue;
         if (hipMemcpyAsync(host + rg.first, dev + rg.first, rg.second, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = PH_ERR_DEVICE;
+    }
+    for (size_t i = 0; i < tk.size() && rc == PH_OK; ++i) {
+        if (!tk[i]->direct) continue;
+        const Part &p = b.parts[i];
+        if (hipMemcpyAsync(tk[i]->req.out, dev + p.out, p.out_bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
             rc = PH_ERR_DEVICE;
     }
     (void)hipFreeAsync(arena, st);
@@ -447,8 +505,10 @@ void completer_loop(ph_server *s) {
         Slot &sl = s->slots[b.slot];
         // the callers' buffers, read while the ticket list cannot grow
         std::vector<CopyJob> jobs;
-        for (const Part &p : b.parts)
-            jobs.push_back({s->tickets[p.ticket].req.out, (char *)sl.pinned + p.out, p.out_bytes});
+        for (const Part &p : b.parts) {
+            const Ticket &t = s->tickets[p.ticket];
+            if (!t.direct) jobs.push_back({t.req.out, (char *)sl.pinned + p.out, p.out_bytes});
+        }
         lk.unlock();
         int st = b.status;
         if (hipEventSynchronize(ev) != hipSuccess && st == PH_OK) st = PH_ERR_DEVICE;
@@ -550,10 +610,12 @@ int ph_server_submit(ph_server *s, const ph_serve_request *req, uint64_t *ticket
     const uint64_t bytes = request_bytes(*req);
     const uint64_t scratch = req->kind == PH_REQ_RLWE_ENCRYPT ? rlwe_scratch_bytes(req->log_n) : 0;
     if (bytes + scratch > s->cfg.hbm_budget_bytes) return PH_ERR_NO_MEMORY;  // can never be admitted
+    const bool direct = request_direct(*req);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stopping) return PH_ERR_INVALID;
     Ticket t;
     t.req = *req;
+    t.direct = direct;
     t.arrival = Clock::now();
     t.bytes = bytes;
     s->tickets.push_back(t);

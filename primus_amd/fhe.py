@@ -91,6 +91,43 @@ class RlweContext:
         _check(lib().ph_rlwe_encrypt_device(self._h, ctypes.cast(arr, ctypes.c_void_p), len(requests), d_a, d_s,
                                             d_e, d_m, d_out, _stream(stream)))
 
+    def plan(self, requests):
+        """A prepared launch for this batch layout (RlwePlan)."""
+        return RlwePlan(self, requests)
+
+
+class RlwePlan:
+    """ph_rlwe_plan: the batch's work items and scratch resident on the device;
+    encrypt_device only enqueues the persistent kernel (capturable in a HIP graph)."""
+
+    def __init__(self, ctx, requests):
+        arr = (RlweRequest * len(requests))(*[RlweRequest(*r) for r in requests])
+        h = ctypes.c_void_p()
+        _check(lib().ph_rlwe_plan_new(ctx._h, ctypes.cast(arr, ctypes.c_void_p), len(requests), ctypes.byref(h)))
+        self._h, self._ctx = h, ctx
+
+    @property
+    def grid(self):
+        return lib().ph_rlwe_plan_grid(self._h)
+
+    def encrypt_device(self, d_a, d_s, d_e, d_m, d_out, stream=None):
+        _check(lib().ph_rlwe_encrypt_plan_device(self._h, d_a, d_s, d_e, d_m, d_out, _stream(stream)))
+
+    def autotune(self, d_a, d_s, d_e, d_m, d_out, reps=5):
+        """Times persistent grids of 1..4 workgroups per CU; returns (grid, ms)."""
+        g, ms = ctypes.c_uint32(), ctypes.c_float()
+        _check(lib().ph_rlwe_plan_autotune(self._h, d_a, d_s, d_e, d_m, d_out, reps, ctypes.byref(g),
+                                           ctypes.byref(ms)))
+        return g.value, ms.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().ph_rlwe_plan_free(self._h)
+            except Exception:
+                pass
+            self._h = None
+
 
 def rlwe_layout(shapes):
     """Offsets for a batch of (log_n, limbs) requests: (requests, a_words, small_elems)."""

@@ -313,5 +313,10 @@ def pointwise_mul_add_device(table, dacc, da, db, count, stream=None):
     _check(lib().ph_dcrt_pointwise_mul_add_device(table.handle, dacc, da, db, count, _stream(stream)))
 
 
+def automorphism_device(table, d_in, d_out, count, degree, stream=None):
+    """x -> x^degree (odd) on `count` limb-polynomials in NTT form, out of place."""
+    _check(lib().ph_dcrt_automorphism_device(table.handle, d_in, d_out, count, degree, _stream(stream)))
+
+
 def synchronize(stream=None):
     _check(lib().ph_stream_synchronize(_stream(stream)))

@@ -2,11 +2,12 @@
 
     python -m primus_amd.profile_summary stats RESULTS_DB
         per-kernel dispatch count, grid, average / min / max duration
-    python -m primus_amd.profile_summary pmc FETCH_DB WRITE_DB KERNEL_SUBSTRING COPY_BYTES
+    python -m primus_amd.profile_summary pmc FETCH_DB WRITE_DB KERNEL_SUBSTRING COPY_BYTES [JSON_FILE]
         FETCH_SIZE and WRITE_SIZE per dispatch of the kernel, calibrated by the
         byte counts the same counters give for the largest device-to-device
         copy in the run (COPY_BYTES read and COPY_BYTES written); prints a
-        markdown table and one JSON line with the calibrated HBM traffic.
+        markdown table and one JSON line with the calibrated HBM traffic, and
+        merges it into JSON_FILE (keyed by kernel) when given.
 """
 import json
 import sqlite3
@@ -48,7 +49,7 @@ def _per_dispatch(db, counter):
                      "order by dispatch_id" % counter)
 
 
-def pmc(fetch_db, write_db, kernel, copy_bytes):
+def pmc(fetch_db, write_db, kernel, copy_bytes, json_file=None):
     copy_bytes = int(copy_bytes)
     fetch = _per_dispatch(fetch_db, "FETCH_SIZE")
     write = _per_dispatch(write_db, "WRITE_SIZE")
@@ -67,15 +68,28 @@ def pmc(fetch_db, write_db, kernel, copy_bytes):
     print("| FETCH_SIZE | %.1f | %.1f | %.4f | %.0f |" % (f, cf, fetch_scale, f * 1024 * fetch_scale))
     print("| WRITE_SIZE | %.1f | %.1f | %.4f | %.0f |" % (w, cw, write_scale, w * 1024 * write_scale))
     print()
-    print(json.dumps({"kernel": kernel, "fetch_kib": f, "write_kib": w, "fetch_scale": fetch_scale,
-                      "write_scale": write_scale, "traffic_bytes": traffic, "dispatches": len(kf)}))
+    entry = {"kernel": kernel, "fetch_kib": f, "write_kib": w, "fetch_scale": fetch_scale,
+             "write_scale": write_scale, "traffic_bytes": traffic, "dispatches": len(kf)}
+    print(json.dumps(entry))
+    if json_file:
+        try:
+            with open(json_file) as fh:
+                table = json.load(fh)
+        except (OSError, ValueError):
+            table = {}
+        if "kernel" in table:  # a single-entry file: key it by its kernel
+            table = {table["kernel"]: table}
+        table[kernel] = entry
+        with open(json_file, "w") as fh:
+            json.dump(table, fh, indent=1)
+            fh.write("\n")
 
 
 if __name__ == "__main__":
     if len(sys.argv) >= 3 and sys.argv[1] == "stats":
         stats(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 0)
     elif len(sys.argv) >= 6 and sys.argv[1] == "pmc":
-        pmc(*sys.argv[2:6])
+        pmc(*sys.argv[2:7])
     else:
         print(__doc__)
         sys.exit(2)

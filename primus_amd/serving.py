@@ -2,7 +2,9 @@
 
 Requests are negacyclic products, RLWE encryptions and external products of
 mixed ring degrees and limb counts.  Host arrays passed in a request are kept
-referenced here until the server has finished with them.
+referenced here until the server has finished with them.  Requests whose
+arrays live in page-locked memory (`Request(..., pinned=True)`, allocated with
+ph_host_alloc) are copied to and from the device without staging copies.
 """
 import ctypes
 
@@ -48,6 +50,10 @@ def _declare():
     L.ph_server_run_poisson.argtypes = [_vp, ctypes.POINTER(ServeRequest), ctypes.c_size_t, ctypes.c_double,
                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]
+    L.ph_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
+    L.ph_host_alloc.restype = ctypes.c_int
+    L.ph_host_free.argtypes = [_vp]
+    L.ph_host_free.restype = ctypes.c_int
     for f in ("ph_server_submit", "ph_server_wait", "ph_server_latency", "ph_server_drain", "ph_server_stats_get",
               "ph_server_run_poisson"):
         getattr(L, f).restype = ctypes.c_int
@@ -59,10 +65,42 @@ def _ptr(a):
     return a.ctypes.data_as(_vp) if a is not None else None
 
 
+class PinnedBuffer:
+    """Page-locked host memory (ph_host_alloc) with a numpy view; freed with the object."""
+
+    def __init__(self, shape, dtype):
+        L = _declare()
+        dtype = np.dtype(dtype)
+        nbytes = max(int(np.prod(shape)) * dtype.itemsize, 1)
+        p = _vp()
+        _check(L.ph_host_alloc(nbytes, ctypes.byref(p)))
+        self._p = p
+        raw = (ctypes.c_uint8 * nbytes).from_address(p.value)
+        self.array = np.frombuffer(raw, dtype=np.uint8, count=int(np.prod(shape)) * dtype.itemsize).view(
+            dtype).reshape(shape)
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            try:
+                lib().ph_host_free(self._p)
+            except Exception:
+                pass
+            self._p = None
+
+
+def _pinned_copy(a, keep):
+    if a is None:
+        return None
+    buf = PinnedBuffer(a.shape, a.dtype)
+    buf.array[...] = a
+    keep.append(buf)
+    return buf.array
+
+
 class Request:
     """A request with its host arrays (kept alive by this object)."""
 
-    def __init__(self, kind, log_n, limbs=1, in0=None, in1=None, s=None, e=None, m=None):
+    def __init__(self, kind, log_n, limbs=1, in0=None, in1=None, s=None, e=None, m=None, pinned=False):
         n = 1 << log_n
         self.kind, self.log_n, self.limbs = kind, log_n, limbs
         self.in0 = np.ascontiguousarray(in0, dtype=np.uint64)
@@ -72,6 +110,14 @@ class Request:
         self.m = None if m is None else np.ascontiguousarray(m, dtype=np.uint8)
         words = 2 * n if kind == EXT_PRODUCT else limbs * n
         self.out = np.zeros(words, dtype=np.uint64)
+        self._pinned = []
+        if pinned:
+            for name in ("in0", "in1", "s", "e", "m", "out"):
+                setattr(self, name, _pinned_copy(getattr(self, name), self._pinned))
+
+    def pinned_copy(self):
+        """The same request with every array in page-locked memory."""
+        return Request(self.kind, self.log_n, self.limbs, self.in0, self.in1, self.s, self.e, self.m, pinned=True)
 
     def c(self):
         return ServeRequest(self.kind, self.log_n, self.limbs, _ptr(self.in0), _ptr(self.in1), _ptr(self.s),

@@ -3,6 +3,8 @@
 Every comparison is bit-exact: the lazy forward output too, since the kernels
 run the reference's Harvey butterflies one for one.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -379,3 +381,32 @@ def test_streaming_shapes_long_batch(log_n, bits):
         for name in ("forward", "fwd_mul_inv", "forward_lazy"):
             tuning.set_shape(log_n, name, tab.arith(), 0)
             tuning.set_shape(log_n, name, "int", 0)
+
+
+@pytest.mark.parametrize("log_n", [1, 3, 9, 12, 13, 16, 17])
+def test_automorphism_device(log_n):
+    """NTT-domain automorphisms on limb-major batches equal the oracle's
+    permutation, row by row, for odd degrees including 1, 2N-1 and N+1."""
+    n = 1 << log_n
+    L, count = 2, 5
+    moduli = pa.ntt_primes_below(59, 2 * n, L)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    rng = np.random.default_rng(log_n + 77)
+    x = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    din, dout = pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(count * n)
+    for k in sorted({1, 3, 2 * n - 1, n + 1, 4 * n + 5, int(rng.integers(0, n)) * 2 + 1}):
+        pa.automorphism_device(tab, din.ptr, dout.ptr, count, k)
+        pa.synchronize()
+        got = dout.download((count, n))
+        for i in range(count):
+            np.testing.assert_array_equal(got[i], O.ntt_automorphism(x[i], k), err_msg="k=%d row %d" % (k, i))
+
+
+def test_automorphism_rejects():
+    n = 1 << 10
+    tab = pa.UintCrtNttTable.new(10, pa.ntt_primes_below(59, 2 * n, 1))
+    d = pa.DeviceBuffer(4 * n)
+    with pytest.raises(ValueError):
+        pa.automorphism_device(tab, d.ptr, pa.DeviceBuffer(n).ptr, 1, 4)  # even degree
+    with pytest.raises(ValueError):
+        pa.automorphism_device(tab, d.ptr, ctypes.c_void_p(d.ptr.value + 8 * n), 2, 3)  # overlapping rows

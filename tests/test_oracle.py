@@ -254,3 +254,35 @@ def test_batch_matches_single():
         want[i] = t.inverse_transform(O.pointwise_mul(f, op[i % 2], t.q))
     got = O.fwd_mul_inv_batch(tables, polys.copy(), op, opq, threads=3)
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("q,log_n", [(17, 3), (12289, 9), (1125899906826241, 12), (576460752303421441, 10)])
+def test_ntt_automorphism_pinned_by_transform(q, log_n):
+    """The NTT-domain permutation (dcrt.rs:43-61) is pinned through the transform:
+    NTT(a(X^k) mod X^N+1) == permuted NTT(a) for odd k, including k = 1, 2N-1
+    (conjugation) and k = N+1."""
+    n = 1 << log_n
+    t = O.Table(log_n, q)
+    rng = np.random.default_rng(log_n)
+    a = rng.integers(0, q, size=n, dtype=np.uint64)
+    fa = t.transform(a)
+    for k in sorted({1, 3, 5, 2 * n - 1, n + 1, 2 * n + 5, int(rng.integers(0, n)) * 2 + 1}):
+        np.testing.assert_array_equal(O.ntt_automorphism(fa, k), t.transform(O.coeff_automorphism(a, k, q)),
+                                      err_msg="k=%d" % k)
+    # composition: sigma_k1 after sigma_k2 is sigma_(k1*k2)
+    np.testing.assert_array_equal(O.ntt_automorphism(O.ntt_automorphism(fa, 3), 5),
+                                  O.ntt_automorphism(fa, 15 % (2 * n)))
+
+
+def test_coeff_automorphism_brute_force():
+    """a(X^k) mod (X^N + 1) by expanding every monomial with Python integers."""
+    q, n = 97, 8
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, q, size=n, dtype=np.uint64)
+    for k in (1, 3, 5, 7, 9, 15):
+        want = [0] * n
+        for j, c in enumerate(a.tolist()):
+            e = j * k
+            sign = -1 if (e // n) % 2 else 1  # X^N = -1
+            want[e % n] = (want[e % n] + sign * c) % q
+        np.testing.assert_array_equal(O.coeff_automorphism(a, k, q), np.array(want, dtype=np.uint64))

@@ -107,6 +107,27 @@ def test_server_mix_matches_oracle_and_alone():
     alone.close()
 
 
+def test_server_page_locked_requests():
+    """Requests in page-locked memory (copied without staging) mixed with
+    ordinary ones, in one batch and alone: identical results, equal to the oracle."""
+    srv, grid, key_np, key = _server(max_batch=64, streams=2)
+    reqs = _mix(grid, key_np, 24, seed=8, max_limbs=4)
+    mixed = [r.pinned_copy() if i % 2 else r for i, r in enumerate(reqs)]
+    assert all(isinstance(r.out, np.ndarray) for r in mixed)
+    tickets = [srv.submit(r) for r in mixed]
+    for t in tickets:
+        assert srv.wait(t) == 0
+    for r in mixed:
+        np.testing.assert_array_equal(r.out, _expected(r, grid, key_np))
+    # all page-locked: a batch without any staging copy
+    pinned = [r.pinned_copy() for r in reqs]
+    for t in [srv.submit(r) for r in pinned]:
+        assert srv.wait(t) == 0
+    for r, m in zip(pinned, mixed):
+        np.testing.assert_array_equal(r.out, m.out)
+    srv.close()
+
+
 def test_server_poisson_load_and_budget():
     srv, grid, key_np, key = _server(max_batch=32, streams=2, budget=256 << 20)
     reqs = _mix(grid, key_np, 60, seed=6, max_limbs=4)
