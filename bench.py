@@ -281,6 +281,37 @@ def secondary_automorphism(pa, timer, sp):
             "hbm_frac": 16 * COUNT * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, "rotating_buffers": rot}
 
 
+def secondary_base_conversion(pa, timer, sp):
+    """RNS base conversion of 256 CRT polynomials of N=4096: fast 4 -> 4 limbs
+    (the headline's 50-bit moduli to four 59-bit ones) and exact 4 -> 1."""
+    from primus_amd.rns import BaseConverter
+
+    polys = POLYS
+    moduli = pa.ntt_primes_below(50, 2 * N, LIMBS)
+    obase = pa.ntt_primes_below(59, 2 * N, LIMBS)
+    rng = np.random.default_rng(SEED + 8)
+    x = np.stack([np.stack([rng.integers(0, q, size=N, dtype=np.uint64) for q in moduli]) for _ in range(polys)])
+    rot = 4  # 4 x (32 + 32) MiB: streamed from HBM
+    out = {}
+    for name, ob, exact in (("fast_4_to_4", obase, False), ("exact_4_to_1", obase[:1], True)):
+        conv = BaseConverter(moduli, ob)
+        pairs = [(pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(polys * len(ob) * N)) for _ in range(rot)]
+        fn = conv.exact_convert_device if exact else conv.fast_convert_device
+
+        def step(i, pairs=pairs, fn=fn):
+            din, dout = pairs[i % rot]
+            fn(din.ptr, dout.ptr, LOG_N, polys, stream=sp)
+
+        per, tot = timer.run(step, 500, 20)
+        ms = tot / 500
+        bytes_ = polys * N * 8 * (LIMBS + len(ob))
+        out[name] = {"ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)),
+                     "p99_ms": float(np.percentile(per, 99)), "algorithmic_gbps": bytes_ / (ms * 1e-3) / 1e9,
+                     "hbm_frac": bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}
+    out["workload"] = "256 CRT polynomials, N=4096, 4 input limbs (50-bit); one launch per conversion"
+    return out
+
+
 def kernel_signature(shape):
     """The fused kernel a tuning shape launches, as rocprofv3 names it (without 'void ')."""
     from primus_amd import tuning
@@ -633,6 +664,7 @@ def main():
             "rlwe_uneven": secondary_rlwe(pa, timer, sp),
             "ext_product": secondary_ext_product(pa, timer, sp),
             "automorphism": secondary_automorphism(pa, timer, sp),
+            "base_conversion": secondary_base_conversion(pa, timer, sp),
         }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(moduli, polys, operand)

@@ -187,6 +187,28 @@ int ph_dcrt_pointwise_mul_add_device(const ph_dcrt_table *table, uint64_t *d_acc
 int ph_dcrt_automorphism_device(const ph_dcrt_table *table, const uint64_t *d_in, uint64_t *d_out, size_t count,
                                 uint64_t degree, void *stream);
 
+/* ---- RNS base conversion ---- */
+
+typedef struct ph_base_converter ph_base_converter; /* BaseConverter<u64, BarrettModulus<u64>> */
+
+/* BaseConverter::new (primus_rns/src/converter.rs:21-46) over RNSBase::new
+ * (primus_rns/src/base.rs:50-90): input moduli pairwise coprime (else
+ * PH_ERR_INVALID, RNSError::CoPrimeError), all moduli in (1, 2^62), at most 32
+ * per base.  [(Q/q_i)^-1]_{q_i} and (Q/q_i) mod p_j stay on the device. */
+int ph_base_converter_new(const uint64_t *ibase, uint32_t in_count, const uint64_t *obase, uint32_t out_count,
+                          ph_base_converter **out);
+void ph_base_converter_free(ph_base_converter *converter);
+/* BaseConverter::fast_convert_array (converter.rs:148-177) on `count` CRT
+ * polynomials of 2^log_n coefficients: d_in [count][in_count][N] ->
+ * d_out [count][out_count][N], canonical.  Buffers must not overlap. */
+int ph_fast_convert_array_device(const ph_base_converter *converter, const uint64_t *d_in, uint64_t *d_out,
+                                 uint32_t log_n, size_t count, void *stream);
+/* BaseConverter::exact_convert_array (converter.rs:222-316): one output modulus;
+ * the binary64 rounding of sum_i y_i/q_i is the reference's, operation for
+ * operation, so the centred result matches bit for bit. */
+int ph_exact_convert_array_device(const ph_base_converter *converter, const uint64_t *d_in, uint64_t *d_out,
+                                  uint32_t log_n, size_t count, void *stream);
+
 /* ---- external product (RLWE x RGSW, k = 1, one modulus) ---- */
 
 typedef struct ph_ext_key ph_ext_key; /* device-resident RGSW/GGSW key in NTT form */

@@ -72,6 +72,11 @@ def lib():
             ("po_decompose", None, [ctypes.c_void_p, _p64, ctypes.c_size_t, _p64]),
             ("po_ext_product", None, [ctypes.c_void_p, ctypes.c_void_p, _p64, _p64, _p64, ctypes.c_int]),
             ("po_ntt_automorphism", None, [_p64, _p64, ctypes.c_size_t, _u64]),
+            ("po_base_converter_new", ctypes.c_int,
+             [_p64, ctypes.c_uint32, _p64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
+            ("po_base_converter_free", None, [ctypes.c_void_p]),
+            ("po_fast_convert_array", None, [ctypes.c_void_p, _p64, _p64, ctypes.c_size_t]),
+            ("po_exact_convert_array", None, [ctypes.c_void_p, _p64, _p64, ctypes.c_size_t]),
             ("po_coeff_automorphism", None, [_p64, _p64, ctypes.c_size_t, _u64, _u64]),
         ]:
             f = getattr(L, name)
@@ -298,3 +303,36 @@ def coeff_automorphism(values, degree, q):
     out = np.zeros_like(x)
     lib().po_coeff_automorphism(ptr(x), ptr(out), x.size, degree, q)
     return out
+
+
+class BaseConverter:
+    """po_base_converter: RNS base conversion from ibase to obase (limb-major rows)."""
+
+    def __init__(self, ibase, obase):
+        self.ibase = np.array(ibase, dtype=np.uint64)
+        self.obase = np.array(obase, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        rc = lib().po_base_converter_new(ptr(self.ibase), len(ibase), ptr(self.obase), len(obase), ctypes.byref(h))
+        if rc != 0:
+            raise OracleError(rc)
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().po_base_converter_free(self._h)
+            self._h = None
+
+    def _run(self, fn, x, rows_out):
+        x = np.ascontiguousarray(x, dtype=np.uint64)
+        n = x.shape[-1]
+        out = np.zeros((rows_out, n), dtype=np.uint64)
+        fn(self._h, ptr(x), ptr(out), n)
+        return out
+
+    def fast_convert(self, x):
+        """[len(ibase)][n] residues -> [len(obase)][n]."""
+        return self._run(lib().po_fast_convert_array, x, len(self.obase))
+
+    def exact_convert(self, x):
+        """[len(ibase)][n] residues -> [1][n] (obase holds one modulus)."""
+        return self._run(lib().po_exact_convert_array, x, 1)

@@ -634,3 +634,136 @@ void po_coeff_automorphism(const uint64_t *in, uint64_t *out, size_t n, uint64_t
         else out[e - n] = v ? q - v : 0;
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* RNS base conversion                                                       */
+/* ------------------------------------------------------------------------- */
+
+struct po_base_converter {
+    uint32_t in_count, out_count;
+    uint64_t *q, *p;          /* input / output moduli */
+    uint64_t *inv, *inv_q;    /* [(Q/q_i)^-1]_{q_i} as ShoupFactor */
+    uint64_t *m;              /* [out][in]: (Q/q_i) mod p_j */
+    uint64_t *q_mod_p;        /* [out]: Q mod p_j */
+};
+
+static uint64_t gcd_u64(uint64_t a, uint64_t b) {
+    while (b) {
+        uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+static uint64_t mulmod_u64(uint64_t a, uint64_t b, uint64_t m) {
+    return (uint64_t)(((unsigned __int128)a * b) % m);
+}
+
+void po_base_converter_free(po_base_converter *c) {
+    if (!c) return;
+    free(c->q);
+    free(c->p);
+    free(c->inv);
+    free(c->inv_q);
+    free(c->m);
+    free(c->q_mod_p);
+    free(c);
+}
+
+/* BaseConverter::new (primus_rns/src/converter.rs:21-46) over RNSBase::new
+ * (primus_rns/src/base.rs:50-90): the input moduli pairwise coprime, the
+ * punctured products Q/q_i kept modulo each modulus they meet. */
+int po_base_converter_new(const uint64_t *ibase, uint32_t in_count, const uint64_t *obase, uint32_t out_count,
+                          po_base_converter **out) {
+    if (!out || !ibase || !obase || in_count == 0 || out_count == 0) return PO_ERR_INVALID;
+    *out = NULL;
+    for (uint32_t i = 0; i < in_count; ++i) {
+        if (ibase[i] <= 1 || ibase[i] >= (1ull << 62)) return PO_ERR_INVALID;
+        for (uint32_t k = i + 1; k < in_count; ++k)
+            if (gcd_u64(ibase[i], ibase[k]) != 1) return PO_ERR_INVALID; /* RNSError::CoPrimeError */
+    }
+    for (uint32_t j = 0; j < out_count; ++j)
+        if (obase[j] <= 1 || obase[j] >= (1ull << 62)) return PO_ERR_INVALID;
+    po_base_converter *c = (po_base_converter *)calloc(1, sizeof *c);
+    if (!c) return PO_ERR_INVALID;
+    c->in_count = in_count;
+    c->out_count = out_count;
+    c->q = (uint64_t *)malloc(in_count * 8);
+    c->p = (uint64_t *)malloc(out_count * 8);
+    c->inv = (uint64_t *)malloc(in_count * 8);
+    c->inv_q = (uint64_t *)malloc(in_count * 8);
+    c->m = (uint64_t *)malloc((size_t)in_count * out_count * 8);
+    c->q_mod_p = (uint64_t *)malloc(out_count * 8);
+    for (uint32_t i = 0; i < in_count; ++i) {
+        c->q[i] = ibase[i];
+        uint64_t punct = 1 % ibase[i];
+        for (uint32_t k = 0; k < in_count; ++k)
+            if (k != i) punct = mulmod_u64(punct, ibase[k] % ibase[i], ibase[i]);
+        c->inv[i] = po_inv_mod(punct, ibase[i]); /* try_inv_modulo, base.rs:78 */
+        c->inv_q[i] = po_shoup_quotient(c->inv[i], ibase[i]);
+    }
+    for (uint32_t j = 0; j < out_count; ++j) {
+        c->p[j] = obase[j];
+        uint64_t all = 1 % obase[j];
+        for (uint32_t k = 0; k < in_count; ++k) all = mulmod_u64(all, ibase[k] % obase[j], obase[j]);
+        c->q_mod_p[j] = all;
+        for (uint32_t i = 0; i < in_count; ++i) {
+            uint64_t punct = 1 % obase[j];
+            for (uint32_t k = 0; k < in_count; ++k)
+                if (k != i) punct = mulmod_u64(punct, ibase[k] % obase[j], obase[j]);
+            c->m[(size_t)j * in_count + i] = punct; /* m_i.modulo(modulus), converter.rs:35-41 */
+        }
+    }
+    *out = c;
+    return PO_OK;
+}
+
+/* y_i = [x_i * (Q/q_i)^-1]_{q_i}: fill_fast_convert_array_buffer
+ * (converter.rs:103-146); a factor of one reduces x instead of multiplying. */
+static uint64_t bc_scaled(const po_base_converter *c, uint32_t i, uint64_t x) {
+    if (c->inv[i] == 1) return x % c->q[i];
+    return po_shoup_mul(c->inv[i], c->inv_q[i], x, c->q[i]);
+}
+
+/* BaseConverter::fast_convert_array (converter.rs:148-177): in [in_count][n],
+ * out [out_count][n]; out_j[k] = sum_i y_i[k] * ((Q/q_i) mod p_j) mod p_j
+ * (reduce_dot_product: the canonical dot product). */
+void po_fast_convert_array(const po_base_converter *c, const uint64_t *in, uint64_t *out, size_t n) {
+    uint64_t *y = (uint64_t *)malloc(c->in_count * 8);
+    for (size_t k = 0; k < n; ++k) {
+        for (uint32_t i = 0; i < c->in_count; ++i) y[i] = bc_scaled(c, i, in[(size_t)i * n + k]);
+        for (uint32_t j = 0; j < c->out_count; ++j) {
+            unsigned __int128 acc = 0;
+            for (uint32_t i = 0; i < c->in_count; ++i)
+                acc = (acc + (unsigned __int128)y[i] * c->m[(size_t)j * c->in_count + i]) % c->p[j];
+            out[(size_t)j * n + k] = (uint64_t)acc;
+        }
+    }
+    free(y);
+}
+
+/* BaseConverter::exact_convert_array (converter.rs:222-316), one output
+ * modulus p: v = round(sum_i y_i / q_i) in binary64 (each term divided as
+ * written, summed in limb order, plus 0.5, truncated), then
+ * out = (sum_i y_i * ((Q/q_i) mod p) - v * (Q mod p)) mod p. */
+void po_exact_convert_array(const po_base_converter *c, const uint64_t *in, uint64_t *out, size_t n) {
+    const uint64_t p = c->p[0];
+    uint64_t *y = (uint64_t *)malloc(c->in_count * 8);
+    for (size_t k = 0; k < n; ++k) {
+        double sum = -0.0;
+        for (uint32_t i = 0; i < c->in_count; ++i) {
+            /* the else-branch multiplies with a full 128-bit remainder (mul_modulo,
+             * converter.rs:268-276); both equal the Shoup product's canonical value */
+            y[i] = bc_scaled(c, i, in[(size_t)i * n + k]);
+            sum += (double)y[i] / (double)c->q[i];
+        }
+        const uint64_t v = (uint64_t)(sum + 0.5);
+        unsigned __int128 acc = 0;
+        for (uint32_t i = 0; i < c->in_count; ++i) acc = (acc + (unsigned __int128)y[i] * c->m[i]) % p;
+        const uint64_t vq = mulmod_u64(v % p, c->q_mod_p[0], p);
+        const uint64_t s = (uint64_t)acc;
+        out[k] = s >= vq ? s - vq : s + p - vq;
+    }
+    free(y);
+}

@@ -131,6 +131,24 @@ void po_coeff_automorphism(const uint64_t *in, uint64_t *out, size_t n, uint64_t
 }
 #endif
 #endif
+#ifndef PRIMUS_ORACLE_RNS_H
+#define PRIMUS_ORACLE_RNS_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* BaseConverter (primus_rns/src/converter.rs): new :21-46, fast_convert_array
+ * :148-177, exact_convert_array :222-316 (one output modulus).  Limb-major
+ * [in_count][n] -> [out_count][n]. */
+typedef struct po_base_converter po_base_converter;
+int po_base_converter_new(const uint64_t *ibase, uint32_t in_count, const uint64_t *obase, uint32_t out_count,
+                          po_base_converter **out);
+void po_base_converter_free(po_base_converter *c);
+void po_fast_convert_array(const po_base_converter *c, const uint64_t *in, uint64_t *out, size_t n);
+void po_exact_convert_array(const po_base_converter *c, const uint64_t *in, uint64_t *out, size_t n);
+#ifdef __cplusplus
+}
+#endif
+#endif
 #ifndef PRIMUS_ORACLE_RLWE_H
 #define PRIMUS_ORACLE_RLWE_H
 #ifdef __cplusplus

@@ -11,7 +11,7 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libprimus_hip.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "primus_hip.h")
 CSRC = os.path.join(PKG_DIR, "csrc")
-SOURCES = ["capi.hip", "ntt_kernels.hip", "fhe_kernels.hip", "serving.cpp"]
+SOURCES = ["capi.hip", "ntt_kernels.hip", "fhe_kernels.hip", "rns_kernels.hip", "serving.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -143,6 +143,10 @@ def _declare(L):
         "ph_dcrt_pointwise_mul_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_pointwise_mul_add_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_automorphism_device": (i, [vp, vp, vp, sz, u64, vp]),
+        "ph_base_converter_new": (i, [p64, u32, p64, u32, pvp]),
+        "ph_base_converter_free": (None, [vp]),
+        "ph_fast_convert_array_device": (i, [vp, vp, vp, u32, sz, vp]),
+        "ph_exact_convert_array_device": (i, [vp, vp, vp, u32, sz, vp]),
         "ph_device_alloc": (i, [sz, pvp]),
         "ph_device_free": (i, [vp]),
         "ph_memcpy_htod": (i, [vp, vp, sz]),

@@ -20,6 +20,7 @@
 #include "arith.hpp"
 #include "fhe_launch.hpp"
 #include "ntt_launch.hpp"
+#include "rns_launch.hpp"
 
 using namespace ph;
 
@@ -625,6 +626,114 @@ int ph_dcrt_automorphism_device(const ph_dcrt_table *d, const uint64_t *d_in, ui
     const uint32_t k = (uint32_t)(degree & ((2ull << d->log_n) - 1));
     hipError_t e = launch_automorphism(d_in, d_out, d->log_n, k, count, (hipStream_t)stream);
     return e == hipSuccess ? PH_OK : hip_fail(e, "automorphism kernel");
+}
+
+// ---------------------------------------------------------------------------
+// RNS base conversion
+// ---------------------------------------------------------------------------
+
+struct ph_base_converter {
+    uint32_t in_count = 0, out_count = 0;
+    void *d_mem = nullptr;  // every BaseConvDev array, one allocation
+    BaseConvDev dev{};
+};
+
+void ph_base_converter_free(ph_base_converter *c) {
+    if (!c) return;
+    if (c->d_mem) (void)hipFree(c->d_mem);
+    delete c;
+}
+
+static uint64_t gcd64(uint64_t a, uint64_t b) {
+    while (b) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+int ph_base_converter_new(const uint64_t *ibase, uint32_t in_count, const uint64_t *obase, uint32_t out_count,
+                          ph_base_converter **out) {
+    if (!out || !ibase || !obase || in_count == 0 || out_count == 0) return fail(PH_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    if (in_count > kMaxRnsModuli || out_count > kMaxRnsModuli)
+        return fail(PH_ERR_UNSUPPORTED, "at most 32 moduli per base");
+    for (uint32_t i = 0; i < in_count; ++i) {
+        if (ibase[i] <= 1 || ibase[i] >= (1ull << 62)) return fail(PH_ERR_INVALID, "modulus outside (1, 2^62)");
+        for (uint32_t k = i + 1; k < in_count; ++k)
+            if (gcd64(ibase[i], ibase[k]) != 1) return fail(PH_ERR_INVALID, "input moduli not pairwise coprime");
+    }
+    for (uint32_t j = 0; j < out_count; ++j)
+        if (obase[j] <= 1 || obase[j] >= (1ull << 62)) return fail(PH_ERR_INVALID, "modulus outside (1, 2^62)");
+    // host image of the device arrays, in BaseConvDev order
+    const size_t I = in_count, J = out_count;
+    std::vector<uint64_t> img(I * 5 + I + J * 3 + J * I + J);
+    uint64_t *q = img.data(), *q_r0 = q + I, *q_r1 = q_r0 + I, *inv = q_r1 + I, *inv_q = inv + I;
+    double *qd = (double *)(inv_q + I);
+    uint64_t *p = inv_q + 2 * I, *p_r0 = p + J, *p_r1 = p_r0 + J, *m = p_r1 + J, *q_mod_p = m + J * I;
+    for (size_t i = 0; i < I; ++i) {
+        q[i] = ibase[i];
+        h_barrett_ratio(q[i], &q_r0[i], &q_r1[i]);
+        uint64_t punct = 1 % q[i];  // (Q / q_i) mod q_i, RNSBase::new (primus_rns/src/base.rs:66-82)
+        for (size_t k = 0; k < I; ++k)
+            if (k != i) punct = h_mulmod(punct, ibase[k] % q[i], q[i]);
+        inv[i] = inv_mod(punct, q[i]);
+        inv_q[i] = h_shoup_quotient(inv[i], q[i]);
+        qd[i] = (double)q[i];
+    }
+    for (size_t j = 0; j < J; ++j) {
+        p[j] = obase[j];
+        h_barrett_ratio(p[j], &p_r0[j], &p_r1[j]);
+        uint64_t all = 1 % p[j];
+        for (size_t k = 0; k < I; ++k) all = h_mulmod(all, ibase[k] % p[j], p[j]);
+        q_mod_p[j] = all;
+        for (size_t i = 0; i < I; ++i) {  // BaseConverter::new (primus_rns/src/converter.rs:31-41)
+            uint64_t punct = 1 % p[j];
+            for (size_t k = 0; k < I; ++k)
+                if (k != i) punct = h_mulmod(punct, ibase[k] % p[j], p[j]);
+            m[j * I + i] = punct;
+        }
+    }
+    ph_base_converter *c = new (std::nothrow) ph_base_converter();
+    if (!c) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
+    c->in_count = in_count;
+    c->out_count = out_count;
+    int rc = upload(&c->d_mem, img.data(), img.size() * 8);
+    if (rc != PH_OK) {
+        ph_base_converter_free(c);
+        return rc;
+    }
+    const uint64_t *base = (const uint64_t *)c->d_mem;
+    auto at = [&](const void *h) { return base + ((const uint64_t *)h - img.data()); };
+    c->dev = BaseConvDev{in_count, out_count, at(q), at(q_r0), at(q_r1), at(inv), at(inv_q),
+                         (const double *)at(qd), at(p), at(p_r0), at(p_r1), at(m), at(q_mod_p)};
+    *out = c;
+    return PH_OK;
+}
+
+static int base_convert(const ph_base_converter *c, const uint64_t *d_in, uint64_t *d_out, uint32_t log_n,
+                        size_t count, bool exact, void *stream) {
+    if (!c || ((!d_in || !d_out) && count)) return fail(PH_ERR_INVALID, "null argument");
+    if (exact && c->out_count != 1) return fail(PH_ERR_INVALID, "exact conversion has one output modulus");
+    if (log_n > 20) return fail(PH_ERR_UNSUPPORTED, "ring degree above 2^20");
+    if (count == 0) return PH_OK;
+    const uint64_t in_bytes = (uint64_t)count * c->in_count << (log_n + 3);
+    const uint64_t out_bytes = (uint64_t)count * c->out_count << (log_n + 3);
+    const uintptr_t a = (uintptr_t)d_in, b = (uintptr_t)d_out;
+    if (a < b + out_bytes && b < a + in_bytes) return fail(PH_ERR_INVALID, "input and output overlap");
+    hipError_t e = launch_base_convert(c->dev, d_in, d_out, log_n, count, exact, (hipStream_t)stream);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "base conversion kernel");
+}
+
+int ph_fast_convert_array_device(const ph_base_converter *c, const uint64_t *d_in, uint64_t *d_out, uint32_t log_n,
+                                 size_t count, void *stream) {
+    return base_convert(c, d_in, d_out, log_n, count, false, stream);
+}
+
+int ph_exact_convert_array_device(const ph_base_converter *c, const uint64_t *d_in, uint64_t *d_out,
+                                  uint32_t log_n, size_t count, void *stream) {
+    return base_convert(c, d_in, d_out, log_n, count, true, stream);
 }
 
 int ph_device_alloc(size_t bytes, void **d_ptr) {

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
         for (uint32_t d = 0; d < g.l; ++d) {
             T x[G::E];
 #pragma unroll
-            for (int i = 0; i < G::E; ++i) x[i] = A::load(gadget_digit(adj[i], d, carry[i], g), tab);
+            for (int i = 0; i < G::E; ++i) x[i] = A::load_fwd(gadget_digit(adj[i], d, carry[i], g), tab);
             // The twiddles are the same for every digit; without this the
             // compiler hoists all their loads out of the digit loop and keeps
             // them live in registers for the whole kernel.

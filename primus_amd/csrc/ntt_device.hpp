@@ -119,6 +119,8 @@ struct IntArith {
     }
     // u64 in [0, 4q) -> lane value
     __device__ __forceinline__ static T load(uint64_t v, const DevTable &) { return v; }
+    // the same, for values that go straight into forward rounds
+    __device__ __forceinline__ static T load_fwd(uint64_t v, const DevTable &) { return v; }
     // lane value (forward output, [0,4q)) -> canonical u64
     __device__ __forceinline__ static uint64_t canon_fwd(T v, const DevTable &t) {
         return sub_if_ge(sub_if_ge(v, t.q2), t.q);
@@ -194,9 +196,13 @@ struct F64Arith {
     // u64 in [0, 4q) -> centred lane value, |x| <= q/2 + 1.  v < 2^52, so
     // the binary64 with exponent field 2^52 and v as its mantissa is 2^52 + v
     // exactly: one OR and one subtraction instead of a general conversion.
-    __device__ __forceinline__ static T load(uint64_t v, const DevTable &t) {
+    __device__ __forceinline__ static T load(uint64_t v, const DevTable &t) { return reduce(load_fwd(v, t), t); }
+    // u64 in [0, 4q) -> lane value in [0, 4q), unreduced: forward rounds reduce
+    // every coefficient as they start, so a value loaded for them needs no
+    // reduction of its own.
+    __device__ __forceinline__ static T load_fwd(uint64_t v, const DevTable &) {
         const double biased = __longlong_as_double((long long)(v | 0x4330000000000000ull));
-        return reduce(biased - 0x1p52, t);
+        return biased - 0x1p52;
     }
     // centred |x| < q -> [0, q): x + 2^52 (+ q when negative) lies in
     // [2^52, 2^53), where binary64 holds every integer and the low 52 bits of

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(1 << LOGT, (StreamOccupancy<LOGC, LOGT, MODE>::valu
         const DevTable tab = a.tabs[limb];
         T x[G::E];
 #pragma unroll
-        for (int j = 0; j < G::E; ++j) x[j] = A::load(raw[j], tab);
+        for (int j = 0; j < G::E; ++j) x[j] = A::load_fwd(raw[j], tab);
         const uint64_t next = poly + gridDim.x;
         const bool more = next < a.count;  // uniform over the workgroup
         __builtin_amdgcn_sched_barrier(0);

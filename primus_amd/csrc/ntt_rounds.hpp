@@ -70,9 +70,10 @@ struct FwdRounds {
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j)
-                x[(m << R) + j] = A::load(ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(t, m, j)), tab);
+                x[(m << R) + j] = A::load_fwd(ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(t, m, j)), tab);
     }
-    // The words load_global reads, unconverted, in the same register order.
+    // The words load_global reads, unconverted, in the same register order
+    // (converted with A::load_fwd before the rounds).
     __device__ __forceinline__ static void load_raw(const uint64_t *__restrict__ g, uint64_t *r, uint32_t t) {
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)

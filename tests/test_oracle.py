@@ -286,3 +286,66 @@ def test_coeff_automorphism_brute_force():
             sign = -1 if (e // n) % 2 else 1  # X^N = -1
             want[e % n] = (want[e % n] + sign * c) % q
         np.testing.assert_array_equal(O.coeff_automorphism(a, k, q), np.array(want, dtype=np.uint64))
+
+
+def _crt_value(residues, moduli):
+    Q = 1
+    for q in moduli:
+        Q *= int(q)
+    x = 0
+    for r, q in zip(residues, moduli):
+        qi = Q // int(q)
+        x += int(r) * qi * pow(qi, -1, int(q))
+    return x % Q, Q
+
+
+@pytest.mark.parametrize("ibase,obase", [
+    ([31, 37, 41, 43], [47, 53, 59, 61]),  # primus_rns/src/converter.rs:331-332
+    ([576460752303421441, 576460752303419393, 576460752303415297], [1125899906826241, 1125899906629633]),
+    ([2147473409, 4294957057, 1152921504606830593, 12289, 17], [576460752303421441]),
+])
+def test_fast_base_conversion_pinned(ibase, obase):
+    """fast_convert_array against big integers: with y_i = [x_i (Q/q_i)^-1]_{q_i},
+    S = sum_i y_i Q/q_i is congruent to the CRT value X mod Q, 0 <= S - X < len(ibase) Q,
+    and every output residue is S mod p_j."""
+    c = O.BaseConverter(ibase, obase)
+    rng = np.random.default_rng(len(ibase))
+    n = 64
+    x = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in ibase])
+    out = c.fast_convert(x)
+    for k in range(n):
+        X, Q = _crt_value(x[:, k], ibase)
+        S = 0
+        for i, q in enumerate(ibase):
+            qi = Q // q
+            S += (int(x[i, k]) * pow(qi, -1, q) % q) * qi
+        assert S % Q == X and 0 <= S - X < len(ibase) * Q
+        for j, p in enumerate(obase):
+            assert int(out[j, k]) == S % p
+
+
+@pytest.mark.parametrize("ibase,p", [
+    ([31, 37, 41, 43], 47),
+    ([576460752303421441, 576460752303419393, 576460752303415297, 576460752303403009], 1125899906826241),
+    ([1125899906826241, 1125899906629633], 4294957057),
+])
+def test_exact_base_conversion_pinned(ibase, p):
+    """exact_convert_array equals the centred CRT value (X - Q when X > Q/2: v
+    rounds S/Q to nearest) reduced mod p, for inputs whose fraction X/Q is not
+    within rounding of 1/2 (all random ones)."""
+    c = O.BaseConverter(ibase, [p])
+    rng = np.random.default_rng(p % 1000)
+    n = 256
+    x = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in ibase])
+    x[:, 0] = 0  # zero
+    x[:, 1] = [q - 1 for q in ibase]  # Q - 1
+    out = c.exact_convert(x)[0]
+    for k in range(n):
+        X, Q = _crt_value(x[:, k], ibase)
+        centred = X - Q if 2 * X > Q else X
+        assert int(out[k]) == centred % p
+
+
+def test_base_converter_rejects():
+    with pytest.raises(O.OracleError):
+        O.BaseConverter([6, 9], [7])  # not coprime
