@@ -11,7 +11,7 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libprimus_hip.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "primus_hip.h")
 CSRC = os.path.join(PKG_DIR, "csrc")
-SOURCES = ["capi.hip", "ntt_kernels.hip", "fhe_kernels.hip", "rns_kernels.hip", "serving.cpp"]
+SOURCES = ["capi.hip", "ntt_kernels.hip", "fhe_kernels.hip", "rns_kernels.hip", "copy_kernels.hip", "serving.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

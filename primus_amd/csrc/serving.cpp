@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/primus_hip.h"
+#include "copy_launch.hpp"
 
 namespace {
 
@@ -63,6 +64,8 @@ struct Slot {
     hipStream_t stream = nullptr;
     void *pinned = nullptr;
     uint64_t pinned_cap = 0;
+    ph::CopySpan *spans = nullptr;  // page-locked: read by the copy kernels
+    uint64_t spans_cap = 0;
     bool busy = false;
 };
 
@@ -183,8 +186,8 @@ struct ph_server {
 
 namespace {
 
-// True when [p, p + bytes) starts and ends in page-locked host memory, which
-// the DMA engines read and write directly.
+// True when [p, p + bytes) starts and ends in page-locked host memory mapped
+// at the same address on the device, which kernels read and write directly.
 bool page_locked(const void *p, uint64_t bytes) {
     if (!p || bytes == 0) return false;
     for (const char *q : {(const char *)p, (const char *)p + bytes - 1}) {
@@ -193,7 +196,7 @@ bool page_locked(const void *p, uint64_t bytes) {
             (void)hipGetLastError();  // pageable memory reports an error: clear it
             return false;
         }
-        if (at.type != hipMemoryTypeHost) return false;
+        if (at.type != hipMemoryTypeHost || at.devicePointer != (const void *)q) return false;
     }
     return true;
 }
@@ -252,6 +255,24 @@ uint64_t rlwe_scratch_bytes(uint32_t log_n) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) cus = 256;
     return (uint64_t)std::max(cus, 1) * 2 * (1ull << log_n) * 8;
+}
+
+int ensure_spans(Slot &sl, uint64_t count) {
+    if (sl.spans_cap >= count) return PH_OK;
+    if (sl.spans) (void)hipHostFree(sl.spans);
+    sl.spans = nullptr;
+    sl.spans_cap = 0;
+    const uint64_t cap = std::max<uint64_t>(count, 4096);
+    if (hipHostMalloc((void **)&sl.spans, cap * sizeof(ph::CopySpan), hipHostMallocDefault) != hipSuccess)
+        return PH_ERR_NO_MEMORY;
+    sl.spans_cap = cap;
+    return PH_OK;
+}
+
+// Appends the spans of one copy, split at kCopySpanMax bytes.
+void add_spans(std::vector<ph::CopySpan> &v, uint64_t src, uint64_t dst, uint64_t bytes) {
+    for (uint64_t o = 0; o < bytes; o += ph::kCopySpanMax)
+        v.push_back({src + o, dst + o, std::min<uint64_t>(ph::kCopySpanMax, bytes - o)});
 }
 
 int ensure_pinned(Slot &sl, uint64_t bytes) {
@@ -371,9 +392,21 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
     char *dev = (char *)arena;
     int rc = PH_OK;
     if (staged && hipMemcpyAsync(dev, host, off, hipMemcpyHostToDevice, st) != hipSuccess) rc = PH_ERR_DEVICE;
-    for (const auto &[at, src, bytes] : direct) {
-        if (rc != PH_OK) break;
-        if (hipMemcpyAsync(dev + at, src, bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = PH_ERR_DEVICE;
+    // page-locked requests: every copy of the batch, in and out, as spans of
+    // two copy-kernel launches (inputs now, results after the compute)
+    std::vector<ph::CopySpan> spans_in, spans_out;
+    for (const auto &[at, src, bytes] : direct) add_spans(spans_in, (uint64_t)(uintptr_t)src, (uint64_t)(uintptr_t)(dev + at), bytes);
+    for (size_t i = 0; i < tk.size(); ++i)
+        if (tk[i]->direct)
+            add_spans(spans_out, (uint64_t)(uintptr_t)(dev + b.parts[i].out), (uint64_t)(uintptr_t)tk[i]->req.out,
+                      b.parts[i].out_bytes);
+    if (rc == PH_OK && (!spans_in.empty() || !spans_out.empty())) {
+        if (ensure_spans(sl, spans_in.size() + spans_out.size()) != PH_OK) rc = PH_ERR_NO_MEMORY;
+        else {
+            std::memcpy(sl.spans, spans_in.data(), spans_in.size() * sizeof(ph::CopySpan));
+            std::memcpy(sl.spans + spans_in.size(), spans_out.data(), spans_out.size() * sizeof(ph::CopySpan));
+            if (ph::launch_copy_spans(sl.spans, (uint32_t)spans_in.size(), st) != hipSuccess) rc = PH_ERR_DEVICE;
+        }
     }
     for (const GroupAt &g : gat) {
         if (rc != PH_OK) break;
@@ -421,12 +454,9 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
         if (hipMemcpyAsync(host + rg.first, dev + rg.first, rg.second, hipMemcpyDeviceToHost, st) != hipSuccess)
             rc = PH_ERR_DEVICE;
     }
-    for (size_t i = 0; i < tk.size() && rc == PH_OK; ++i) {
-        if (!tk[i]->direct) continue;
-        const Part &p = b.parts[i];
-        if (hipMemcpyAsync(tk[i]->req.out, dev + p.out, p.out_bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
-            rc = PH_ERR_DEVICE;
-    }
+    if (rc == PH_OK && !spans_out.empty() &&
+        ph::launch_copy_spans(sl.spans + spans_in.size(), (uint32_t)spans_out.size(), st) != hipSuccess)
+        rc = PH_ERR_DEVICE;
     (void)hipFreeAsync(arena, st);
     if (hipEventRecord(b.done, st) != hipSuccess && rc == PH_OK) rc = PH_ERR_DEVICE;
     return rc;
@@ -540,6 +570,7 @@ void destroy(ph_server *s) {
     for (Slot &sl : s->slots) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.pinned) (void)hipHostFree(sl.pinned);
+        if (sl.spans) (void)hipHostFree(sl.spans);
     }
     delete s;
 }

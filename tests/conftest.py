@@ -4,6 +4,15 @@ import sys
 import numpy as np
 import pytest
 
+# torch bundles its own HIP runtime under the same soname (libamdhip64.so.7)
+# as the system one libprimus_hip.so links against; whichever loads first
+# serves the process, and torch only initialises its devices on its own.  So
+# torch is loaded before any test loads the library (the graph tests use it).
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
