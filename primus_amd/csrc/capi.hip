@@ -30,6 +30,7 @@ struct ph_ntt_table {
     std::vector<Shoup> fwd, inv, ord;  // host copies (accessors)
     Shoup *d_fwd = nullptr, *d_inv = nullptr, *d_ord = nullptr;
     ShoupF *d_fwd_f = nullptr, *d_inv_f = nullptr;  // binary64 twiddles (q < 2^50)
+    ShoupF *d_perm[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [loge - 3][fwd, inv], register order
     bool f64 = false;                                // canonical ops may use F64Arith
     DevTable dev{};                    // device view (pointers into d_*)
     DevTable *d_tab = nullptr;         // the view itself in device memory (L = 1)
@@ -132,6 +133,9 @@ void free_table(ph_ntt_table *t) {
     if (t->d_ord) (void)hipFree(t->d_ord);
     if (t->d_fwd_f) (void)hipFree(t->d_fwd_f);
     if (t->d_inv_f) (void)hipFree(t->d_inv_f);
+    for (auto &row : t->d_perm)
+        for (ShoupF *p : row)
+            if (p) (void)hipFree(p);
     if (t->d_tab) (void)hipFree(t->d_tab);
     delete t;
 }
@@ -140,6 +144,60 @@ int upload(void **dst, const void *src, size_t bytes) {
     PH_HIP(hipMalloc(dst, bytes));
     PH_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
     return PH_OK;
+}
+
+// The register rounds of a single-pass launch with 2^loge coefficients per
+// thread (Geo in ntt_rounds.hpp): forward round i covers stages
+// [fs(i), fs(i) + fr(i)), inverse round i stages [is(i), is(i) + ir(i)).
+struct RoundPlan {
+    int loge, rem, nr;
+    RoundPlan(uint32_t log_n, int loge_req) {
+        loge = (int)log_n > loge_req ? loge_req : (int)log_n;
+        rem = (int)log_n % loge;
+        nr = (int)log_n / loge + (rem ? 1 : 0);
+    }
+    int fr(int i) const { return (rem && i == 0) ? rem : loge; }
+    int fs(int i) const { return i == 0 ? 0 : (rem ? rem + (i - 1) * loge : i * loge); }
+    int ir(int i) const { return (rem && i == nr - 1) ? rem : loge; }
+    int is(int i) const { return i * loge; }
+};
+
+// Twiddle tables in register order for one launch shape.  A round starting at
+// stage S gives the group of chunk c, at its stage S+u, the twiddles of chunks
+// c*2^u + sub (sub < 2^u); the CPU order stores them together per c, so
+// neighbouring lanes (neighbouring c) read addresses 2^u entries apart and a
+// wave touches up to 64 lines per load.  Stored instead at
+// 2^(S+u) + sub*2^S + c, each load of a wave reads consecutive entries.  The
+// inverse does the same within each stage's segment, with the chunks counted
+// at the round's last stage (n >> (P+R) of them).  Entries are the same
+// values; only positions change.
+void register_order(const std::vector<ShoupF> &fwd, const std::vector<ShoupF> &inv, uint32_t log_n, int loge,
+                     std::vector<ShoupF> &pf, std::vector<ShoupF> &pi) {
+    const uint64_t n = 1ull << log_n;
+    const RoundPlan g(log_n, loge);
+    pf.assign(n, fwd[0]);
+    pi.assign(n, inv[0]);
+    for (int i = 0; i < g.nr; ++i) {
+        const int S = g.fs(i), R = g.fr(i);
+        for (int u = 0; u < R; ++u) {
+            const uint64_t s = (uint64_t)(S + u);
+            for (uint64_t c = 0; c < (1ull << S); ++c)
+                for (uint64_t sub = 0; sub < (1ull << u); ++sub)
+                    pf[(1ull << s) + (sub << S) + c] = fwd[(1ull << s) + (c << u) + sub];
+        }
+    }
+    for (int i = 0; i < g.nr; ++i) {
+        const int P = g.is(i), R = g.ir(i);
+        const uint64_t C = n >> (P + R);
+        for (int u = 0; u < R; ++u) {
+            const uint64_t s = (uint64_t)(P + u);
+            if (s + 1 == log_n) continue;  // the last stage folds N^-1 and uses no table entry
+            const uint64_t base = 1 + n - (n >> s);
+            const uint64_t subs = 1ull << (R - u - 1);
+            for (uint64_t c = 0; c < C; ++c)
+                for (uint64_t sub = 0; sub < subs; ++sub) pi[base + sub * C + c] = inv[base + c * subs + sub];
+        }
+    }
 }
 
 // binary64 twiddle, centred: w' = w or w - q (|w'| <= q/2) and w'/q from the
@@ -220,6 +278,14 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
         }
         rc = upload((void **)&t->d_fwd_f, ff.data(), n * sizeof(ShoupF));
         if (rc == PH_OK) rc = upload((void **)&t->d_inv_f, fi.data(), n * sizeof(ShoupF));
+        // register-order copies for the single-pass launch shapes (8 and 16
+        // coefficients per thread)
+        for (int loge = 3; loge <= 4 && rc == PH_OK && log_n <= kMaxSinglePassLogN; ++loge) {
+            std::vector<ShoupF> pf, pi;
+            register_order(ff, fi, log_n, loge, pf, pi);
+            rc = upload((void **)&t->d_perm[loge - 3][0], pf.data(), n * sizeof(ShoupF));
+            if (rc == PH_OK) rc = upload((void **)&t->d_perm[loge - 3][1], pi.data(), n * sizeof(ShoupF));
+        }
         if (rc != PH_OK) {
             free_table(t);
             return rc;
@@ -235,6 +301,10 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
         d.sr_p = sr.wp;
         d.fwd_f = t->d_fwd_f;
         d.inv_f = t->d_inv_f;
+        for (int e = 0; e < 2; ++e) {
+            d.fwd_p[e] = t->d_perm[e][0];
+            d.inv_p[e] = t->d_perm[e][1];
+        }
     }
     rc = upload((void **)&t->d_tab, &d, sizeof(DevTable));
     if (rc != PH_OK) {

@@ -81,6 +81,10 @@ struct DevTable {
     double inv_n_f, inv_n_p, sr_f, sr_p;
     const ShoupF *fwd_f;
     const ShoupF *inv_f;
+    // the same in register order for single-pass launches with 2^(e+3)
+    // coefficients per thread (e = 0, 1); see ct_round / gs_round
+    const ShoupF *fwd_p[2];
+    const ShoupF *inv_p[2];
     uint32_t log_n;
     uint32_t pad_;
 };
@@ -95,6 +99,9 @@ struct IntArith {
 
     __device__ __forceinline__ static gptr<Tw> fwd(const DevTable &t) { return as_global(t.fwd); }
     __device__ __forceinline__ static gptr<Tw> inv(const DevTable &t) { return as_global(t.inv); }
+    static constexpr bool kRegisterOrder = false;  // integer tables keep the CPU order
+    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t, int) { return fwd(t); }
+    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t, int) { return inv(t); }
 
     // Harvey forward butterfly, primitive.rs:225-228.
     __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
@@ -148,6 +155,14 @@ struct F64Arith {
 
     __device__ __forceinline__ static gptr<Tw> fwd(const DevTable &t) { return as_global(t.fwd_f); }
     __device__ __forceinline__ static gptr<Tw> inv(const DevTable &t) { return as_global(t.inv_f); }
+    // register-order tables of the single-pass shape with 2^loge coefficients per thread
+    static constexpr bool kRegisterOrder = true;
+    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t, int loge) {
+        return as_global(t.fwd_p[loge - 3]);
+    }
+    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t, int loge) {
+        return as_global(t.inv_p[loge - 3]);
+    }
 
     // x*w mod q, centred.  Twiddles are stored centred (|w| <= q/2, |wp| < 1/2
     // with |wp - w/q| <= d = 2^-54 + 2^-64), so |x*wp| < 2^51 for |x| <= 3.9 q
@@ -236,7 +251,10 @@ struct F64Arith {
 
 // R forward stages on one register group x[0 .. 2^R), global stages [S, S+R),
 // chunk c at stage S.  FENCE: schedule stage by stage (see below).
-template <class A, int R, bool FENCE = true>
+// PERM: tw is a register-order table (DevTable::fwd_p), in which the twiddle of
+// chunk c*2^u + sub at stage S+u sits at 2^(S+u) + sub*2^S + c, so lanes with
+// consecutive c read consecutive entries (requires c < 2^S: single pass).
+template <class A, int R, bool FENCE = true, bool PERM = false>
 __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t c,
                                          gptr<typename A::Tw> tw, const DevTable &t) {
     if constexpr (A::kReduceAtRoundStart) {
@@ -246,10 +264,11 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
 #pragma unroll
     for (int u = 0; u < R; ++u) {
         const int half = 1 << (R - u - 1);
-        const uint32_t base = (1u << (S + u)) + (c << u);
+        const uint32_t base = PERM ? (1u << (S + u)) + c : (1u << (S + u)) + (c << u);
+        const uint32_t step = PERM ? (1u << S) : 1u;
 #pragma unroll
         for (int sub = 0; sub < (1 << u); ++sub) {
-            const typename A::Tw w = ld(tw + base + sub);
+            const typename A::Tw w = ld(tw + base + sub * step);
 #pragma unroll
             for (int jj = 0; jj < half; ++jj) {
                 const int j0 = sub * 2 * half + jj;
@@ -265,8 +284,10 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
 
 // R inverse stages on one register group, global inverse stages [P, P+R)
 // (gaps 2^P .. 2^(P+R-1)), chunk c = k >> (P+R).  LAST: the round ends with the
-// transform's final stage (gap N/2).
-template <class A, int R, bool LAST, bool FENCE = true>
+// transform's final stage (gap N/2).  PERM: itw is a register-order table
+// (DevTable::inv_p): within stage P+u's segment the twiddle of chunk
+// c*2^(R-u-1) + sub sits at sub*(N >> (P+R)) + c.
+template <class A, int R, bool LAST, bool FENCE = true, bool PERM = false>
 __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t c, uint32_t n,
                                          gptr<typename A::Tw> itw, const DevTable &t) {
 #pragma unroll
@@ -276,10 +297,11 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
 #pragma unroll
             for (int jj = 0; jj < half; ++jj) A::gs_last(x[jj], x[jj + half], t);
         } else {
-            const uint32_t base = 1u + n - (n >> (P + u)) + (c << (R - u - 1));
+            const uint32_t base = 1u + n - (n >> (P + u)) + (PERM ? c : (c << (R - u - 1)));
+            const uint32_t step = PERM ? (n >> (P + R)) : 1u;
 #pragma unroll
             for (int sub = 0; sub < (1 << (R - u - 1)); ++sub) {
-                const typename A::Tw w = ld(itw + base + sub);
+                const typename A::Tw w = ld(itw + base + sub * step);
 #pragma unroll
                 for (int jj = 0; jj < half; ++jj) {
                     const int j0 = sub * 2 * half + jj;

@@ -83,8 +83,8 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
     } else {
         constexpr bool LAST = (MODE & M_INV_LAST) != 0;
         __builtin_amdgcn_sched_barrier(0);
-        InvRounds<A, LOGC, LOGT, 0, LAST>::run(x, lds, t, n, b, tab);
-        InvRounds<A, LOGC, LOGT, G::NR - 1, LAST>::store_global(data, x, t, tab);
+        InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder>::run(x, lds, t, n, b, tab);
+        InvRounds<A, LOGC, LOGT, G::NR - 1, LAST, A::kRegisterOrder>::store_global(data, x, t, tab);
     }
 }
 
@@ -110,16 +110,16 @@ __global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) vo
         if constexpr ((MODE & M_MUL_POLY) != 0) {
             // transform the second operand first and keep it in registers
             T y[G::E];
-            FwdRounds<A, LOGC, LOGT, 0>::load_global(a.data2 + off, y, t, tab);
-            FwdRounds<A, LOGC, LOGT, 0>::run(y, lds, t, st0, b, tab);
-            FwdRounds<A, LOGC, LOGT, 0>::load_global(data, x, t, tab);
-            FwdRounds<A, LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_global(a.data2 + off, y, t, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::run(y, lds, t, st0, b, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_global(data, x, t, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, t, st0, b, tab);
             // NttPolynomial::mul_assign (primus_poly/src/ntt/mul.rs:87-95)
 #pragma unroll
             for (int j = 0; j < G::E; ++j) x[j] = A::mul_pair(x[j], y[j], tab);
         } else {
-            FwdRounds<A, LOGC, LOGT, 0>::load_global(data, x, t, tab);
-            FwdRounds<A, LOGC, LOGT, 0>::run(x, lds, t, st0, b, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_global(data, x, t, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, t, st0, b, tab);
         }
         after_forward<A, LOGC, LOGT, MODE>(a, x, lds, t, b, limb, n, tab, data);
     } else {
@@ -128,8 +128,8 @@ __global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) vo
         for (int j = 0; j < G::E; ++j) x[j] = A::load(src[j], tab);
         constexpr bool LAST = (MODE & M_INV_LAST) != 0;
         __builtin_amdgcn_sched_barrier(0);
-        InvRounds<A, LOGC, LOGT, 0, LAST>::run(x, lds, t, n, b, tab);
-        InvRounds<A, LOGC, LOGT, G::NR - 1, LAST>::store_global(data, x, t, tab);
+        InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder>::run(x, lds, t, n, b, tab);
+        InvRounds<A, LOGC, LOGT, G::NR - 1, LAST, A::kRegisterOrder>::store_global(data, x, t, tab);
     }
 }
 
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(1 << LOGT, (StreamOccupancy<LOGC, LOGT, MODE>::valu
     const uint32_t t = threadIdx.x;
     uint64_t poly = blockIdx.x;
     uint64_t raw[G::E];
-    FwdRounds<A, LOGC, LOGT, 0>::load_raw(a.data + poly * G::C, raw, t);
+    FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_raw(a.data + poly * G::C, raw, t);
     while (true) {
         const uint32_t limb = (uint32_t)(poly % a.L);
         const DevTable tab = a.tabs[limb];
@@ -169,9 +169,9 @@ __global__ __launch_bounds__(1 << LOGT, (StreamOccupancy<LOGC, LOGT, MODE>::valu
         const uint64_t next = poly + gridDim.x;
         const bool more = next < a.count;  // uniform over the workgroup
         __builtin_amdgcn_sched_barrier(0);
-        if (more) FwdRounds<A, LOGC, LOGT, 0>::load_raw(a.data + next * G::C, raw, t);
+        if (more) FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_raw(a.data + next * G::C, raw, t);
         __builtin_amdgcn_sched_barrier(0);
-        FwdRounds<A, LOGC, LOGT, 0>::run(x, lds, t, 0, 0, tab);
+        FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, t, 0, 0, tab);
         after_forward<A, LOGC, LOGT, MODE>(a, x, lds, t, 0, limb, G::C, tab, a.data + poly * G::C);
         if (!more) break;
         poly = next;

@@ -45,8 +45,9 @@ __device__ __forceinline__ uint32_t inv_slot(uint32_t t, int m, int j) {
 }
 
 // FENCE: schedule each round stage by stage (ct_round); off for kernels that
-// keep two transforms in flight and need the scheduler's freedom.
-template <class A, int LOGC, int LOGT, int I, bool FENCE = true>
+// keep two transforms in flight and need the scheduler's freedom.  PERM: read
+// the register-order twiddle tables of this launch shape (single pass only).
+template <class A, int LOGC, int LOGT, int I, bool FENCE = true, bool PERM = false>
 struct FwdRounds {
     using G = Geo<LOGC, LOGT>;
     using T = typename A::T;
@@ -87,7 +88,7 @@ struct FwdRounds {
                                                const DevTable &tab) {
         if constexpr (I > 0) {
             __syncthreads();
-            FwdRounds<A, LOGC, LOGT, I - 1, FENCE>::store(lds, x, t);
+            FwdRounds<A, LOGC, LOGT, I - 1, FENCE, PERM>::store(lds, x, t);
             __syncthreads();
             load(lds, x, t);
         }
@@ -97,13 +98,16 @@ struct FwdRounds {
             uint32_t cl;
             if constexpr (OB >= LOGT) cl = (uint32_t)m >> (OB - LOGT);  // wave-uniform
             else cl = (t + ((uint32_t)m << LOGT)) >> OB;
-            ct_round<A, R, FENCE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
+            if constexpr (PERM)
+                ct_round<A, R, FENCE, true>(x + (m << R), st0 + S, (b << S) + cl, A::fwd_reg(tab, G::LOGE), tab);
+            else
+                ct_round<A, R, FENCE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
         }
-        if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1, FENCE>::run(x, lds, t, st0, b, tab);
+        if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1, FENCE, PERM>::run(x, lds, t, st0, b, tab);
     }
 };
 
-template <class A, int LOGC, int LOGT, int I, bool LAST>
+template <class A, int LOGC, int LOGT, int I, bool LAST, bool PERM = false>
 struct InvRounds {
     using G = Geo<LOGC, LOGT>;
     using T = typename A::T;
@@ -134,7 +138,7 @@ struct InvRounds {
                                                const DevTable &tab) {
         if constexpr (I > 0) {
             __syncthreads();
-            InvRounds<A, LOGC, LOGT, I - 1, LAST>::store(lds, x, t);
+            InvRounds<A, LOGC, LOGT, I - 1, LAST, PERM>::store(lds, x, t);
             __syncthreads();
             load(lds, x, t);
         }
@@ -144,9 +148,13 @@ struct InvRounds {
             uint32_t cl;
             if constexpr (P >= LOGT) cl = (uint32_t)m >> (P - LOGT);  // wave-uniform
             else cl = (t + ((uint32_t)m << LOGT)) >> P;
-            gs_round<A, R, last_here>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n, A::inv(tab), tab);
+            if constexpr (PERM)
+                gs_round<A, R, last_here, true, true>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n,
+                                                      A::inv_reg(tab, G::LOGE), tab);
+            else
+                gs_round<A, R, last_here>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n, A::inv(tab), tab);
         }
-        if constexpr (I + 1 < G::NR) InvRounds<A, LOGC, LOGT, I + 1, LAST>::run(x, lds, t, n, b, tab);
+        if constexpr (I + 1 < G::NR) InvRounds<A, LOGC, LOGT, I + 1, LAST, PERM>::run(x, lds, t, n, b, tab);
     }
 };
 
