@@ -49,6 +49,7 @@ __device__ __forceinline__ gptr<ShoupF> ext_key(const ExtArgs &g, const F64Arith
 // Hides the table's twiddle pointers from loop-invariant code motion.
 __device__ __forceinline__ void opaque(DevTable &t) {
     asm volatile("" : "+s"(t.fwd), "+s"(t.inv), "+s"(t.fwd_f), "+s"(t.inv_f));
+    asm volatile("" : "+s"(t.fwd_p[0]), "+s"(t.fwd_p[1]), "+s"(t.inv_p[0]), "+s"(t.inv_p[1]));
 }
 __device__ __forceinline__ void ext_finish(uint64_t &, const DevTable &) {}
 __device__ __forceinline__ void ext_finish(double &acc, const DevTable &t) { acc = F64Arith::reduce(acc, t); }
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
             opaque(td);
             uint32_t tl = t;
             asm volatile("" : "+v"(tl));
-            FwdRounds<A, LOGN, LOGT, 0>::run(x, lds, tl, 0, 0, td);
+            FwdRounds<A, LOGN, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, tl, 0, 0, td);
             // thread t now holds slots t*E + j; key rows are stored permuted so
             // that (row, t*E + j) sits at row*N + j*T + t: coalesced loads.
             // Loads are issued a quarter at a time to bound live registers.
@@ -126,8 +127,8 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
 #pragma unroll
             for (int j = 0; j < G::E; ++j) dst[((uint64_t)t << G::LOGE) + j] = A::out_inv(y[j], tab);
         } else {
-            InvRounds<A, LOGN, LOGT, 0, true>::run(y, lds, t, n, 0, tab);
-            InvRounds<A, LOGN, LOGT, G::NR - 1, true>::store_global(dst, y, t, tab);
+            InvRounds<A, LOGN, LOGT, 0, true, A::kRegisterOrder>::run(y, lds, t, n, 0, tab);
+            InvRounds<A, LOGN, LOGT, G::NR - 1, true, A::kRegisterOrder>::store_global(dst, y, t, tab);
         }
     }
 }
