@@ -48,7 +48,8 @@ struct ph_operand {
     uint32_t L;
     uint64_t n;
     Shoup *d = nullptr;      // ShoupFactor form (integer path)
-    ShoupF *d_f = nullptr;   // centred binary64 pairs {w, w/q} (binary64 path)
+    ShoupF *d_f[2] = {nullptr, nullptr};  // centred binary64 pairs {w, w/q} (binary64 path), register
+                                          // order of 8 / 16 coefficients per thread
 };
 
 namespace {
@@ -198,6 +199,21 @@ void register_order(const std::vector<ShoupF> &fwd, const std::vector<ShoupF> &i
                 for (uint64_t sub = 0; sub < subs; ++sub) pi[base + sub * C + c] = inv[base + c * subs + sub];
         }
     }
+}
+
+// A resident [L][N] operand in the register order of single-pass launches
+// with 2^loge coefficients per thread: thread t of a workgroup holds local
+// slots t*E .. t*E+E-1 after the forward rounds and multiplies slot t*E + j
+// by the entry stored at j*T + t, so the j-th operand load of a wave reads
+// consecutive entries.
+std::vector<ShoupF> operand_register_order(const std::vector<ShoupF> &f, uint32_t L, uint32_t log_n, int loge) {
+    const uint64_t n = 1ull << log_n;
+    const uint64_t E = 1ull << std::min<uint32_t>((uint32_t)loge, log_n), T = n / E;
+    std::vector<ShoupF> out(f.size());
+    for (uint64_t l = 0; l < L; ++l)
+        for (uint64_t t = 0; t < T; ++t)
+            for (uint64_t j = 0; j < E; ++j) out[l * n + j * T + t] = f[l * n + t * E + j];
+    return out;
 }
 
 // binary64 twiddle, centred: w' = w or w - q (|w'| <= q/2) and w'/q from the
@@ -356,8 +372,9 @@ int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, 
     a.data = d_data;
     a.data2 = d_data2;
     a.operand = operand ? operand->d : nullptr;
-    a.operand_f = operand ? operand->d_f : nullptr;
-    if (op == Op::FwdMulInv && f64 && !a.operand_f) f64 = false;
+    a.operand_f[0] = operand ? operand->d_f[0] : nullptr;
+    a.operand_f[1] = operand ? operand->d_f[1] : nullptr;
+    if (op == Op::FwdMulInv && f64 && (!a.operand_f[0] || !a.operand_f[1])) f64 = false;
     a.tabs = d_tabs;
     a.L = L;
     void *scratch = nullptr;
@@ -639,7 +656,10 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
         std::vector<ShoupF> hf(h.size());
         for (uint32_t l = 0; l < d->L; ++l)
             for (uint64_t i = 0; i < n; ++i) hf[l * n + i] = mkf(h[l * n + i], d->tables[l]->q);
-        rc = upload((void **)&op->d_f, hf.data(), hf.size() * sizeof(ShoupF));
+        for (int e = 0; e < 2 && rc == PH_OK; ++e) {
+            const std::vector<ShoupF> reg = operand_register_order(hf, d->L, d->log_n, e + 3);
+            rc = upload((void **)&op->d_f[e], reg.data(), reg.size() * sizeof(ShoupF));
+        }
     }
     if (rc != PH_OK) {
         ph_operand_free(op);
@@ -652,7 +672,8 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
 void ph_operand_free(ph_operand *op) {
     if (!op) return;
     if (op->d) (void)hipFree(op->d);
-    if (op->d_f) (void)hipFree(op->d_f);
+    for (ShoupF *p : op->d_f)
+        if (p) (void)hipFree(p);
     delete op;
 }
 
@@ -1301,7 +1322,9 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
             onesv.resize(ones.size());
             for (uint32_t l = 0; l < d->L; ++l)
                 for (uint64_t i = 0; i < n; ++i) onesv[l * n + i] = mkf(ones[l * n + i], d->tables[l]->q);
-            if (upload((void **)&op_dev.d_f, onesv.data(), onesv.size() * sizeof(ShoupF)) != PH_OK) {
+            // all ones: every register order is the same array
+            if (upload((void **)&op_dev.d_f[0], onesv.data(), onesv.size() * sizeof(ShoupF)) != PH_OK ||
+                upload((void **)&op_dev.d_f[1], onesv.data(), onesv.size() * sizeof(ShoupF)) != PH_OK) {
                 rc = PH_ERR_DEVICE;
                 goto done;
             }
@@ -1337,7 +1360,8 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
     if (best_ms) *best_ms = best;
 done:
     if (op_dev.d) (void)hipFree(op_dev.d);
-    if (op_dev.d_f) (void)hipFree(op_dev.d_f);
+    for (ShoupF *p : op_dev.d_f)
+        if (p) (void)hipFree(p);
     for (size_t c = 0; c < copies; ++c) {
         if (bufs[c]) (void)hipFree(bufs[c]);
         if (bufs2[c]) (void)hipFree(bufs2[c]);

@@ -36,8 +36,10 @@ enum : int {
     M_INV_LAST = 32,  // the chunk holds the inverse's final stage (single pass)
 };
 
-__device__ __forceinline__ const Shoup *operand_of(const KArgs &a, const IntArith *) { return a.operand; }
-__device__ __forceinline__ const ShoupF *operand_of(const KArgs &a, const F64Arith *) { return a.operand_f; }
+__device__ __forceinline__ const Shoup *operand_of(const KArgs &a, const IntArith *, int) { return a.operand; }
+__device__ __forceinline__ const ShoupF *operand_of(const KArgs &a, const F64Arith *, int loge) {
+    return a.operand_f[loge - 3];
+}
 
 // Waves per SIMD the launch bounds ask the register allocator for: as many as
 // the LDS footprint lets resident, at most 4 (128 VGPRs: 8 waves per SIMD
@@ -68,10 +70,18 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
     // forward round: their registers would be live across it and spill.
     __builtin_amdgcn_sched_barrier(0);
     if constexpr ((MODE & M_MUL_SHOUP) != 0) {
-        const auto op = as_global(operand_of(a, (const A *)nullptr)) + (uint64_t)limb * n + (uint64_t)b * G::C +
-                        ((uint64_t)t << LE);
+        if constexpr (A::kRegisterOrder) {
+            // single pass: the operand is stored in register order, entry
+            // (t, j) at j*T + t, so each of the E loads of a wave is one row
+            const auto op = as_global(operand_of(a, (const A *)nullptr, LE)) + (uint64_t)limb * n + t;
 #pragma unroll
-        for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], ld(op + j), tab);
+            for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], ld(op + j * G::T), tab);
+        } else {
+            const auto op = as_global(operand_of(a, (const A *)nullptr, LE)) + (uint64_t)limb * n +
+                            (uint64_t)b * G::C + ((uint64_t)t << LE);
+#pragma unroll
+            for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], ld(op + j), tab);
+        }
     }
     if constexpr ((MODE & M_INV) == 0) {
         uint64_t *dst = data + ((uint64_t)t << LE);

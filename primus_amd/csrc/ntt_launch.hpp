@@ -18,7 +18,8 @@ struct KArgs {
     uint64_t *data;          // in/out limb-polynomials, [count][N]
     const uint64_t *data2;   // second operand (poly-multiply) / column-pass source
     const Shoup *operand;    // Shoup-prepared NTT-domain operand, [L][N]
-    const ShoupF *operand_f; // the same operand as centred binary64 pairs (q < 2^50)
+    const ShoupF *operand_f[2]; // the same operand as centred binary64 pairs (q < 2^50), in the
+                                // register order of 8 / 16 coefficients per thread
     const DevTable *tabs;    // L device tables; limb-polynomial i uses tabs[i % L]
     uint32_t L;
     uint32_t blocks_per_poly;
