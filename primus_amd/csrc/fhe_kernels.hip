@@ -170,12 +170,35 @@ __device__ __forceinline__ uint64_t rlwe_x(const RlweArgs &a, const RlweItem &it
     return add_mod(e, shoup_full(it.delta, it.delta_q, a.m[it.small + k], t.q), t.q);
 }
 
-// b_hat = x_hat + a_hat (.) s_hat, NttPolynomial::add_mul_assign (rlwe.rs:267):
-// Barrett reduce_mul_add on canonical operands.
-__device__ __forceinline__ void rlwe_mac(const RlweArgs &a, const RlweItem &it, uint64_t k, uint64_t x, uint64_t s,
-                                         const DevTable &t) {
-    const uint64_t xs = IntArith::canon_fwd(x, t), ss = IntArith::canon_fwd(s, t);
-    a.out[it.row + k] = barrett_mul_add(a.a[it.row + k], ss, xs, t.q, t.br0, t.br1);
+// b_hat = x_hat + a_hat (.) s_hat over one chunk of C = 2^LOGC slots starting
+// at slot `base` of the row, NttPolynomial::add_mul_assign (rlwe.rs:267), the
+// canonical (a*s + x) mod q.  After the forward rounds thread t holds slots
+// t*E + j of s_hat and x_hat; both go through LDS so that the MAC runs in row
+// order and a_hat and b_hat move as whole rows (a wave's load or store covers
+// 64 consecutive words instead of 64 words E apart).
+template <int LOGC, int LOGT>
+__device__ __forceinline__ void rlwe_mac_rows(const RlweArgs &a, const RlweItem &it, uint64_t base, const uint64_t *x,
+                                              const uint64_t *sx, uint64_t *lds, uint32_t tid, const DevTable &t) {
+    using G = Geo<LOGC, LOGT>;
+    uint64_t as[G::E];
+    __syncthreads();  // the rounds' last LDS reads are done
+#pragma unroll
+    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = IntArith::canon_fwd(sx[j], t);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < G::E; ++i) {
+        const uint32_t k = tid + ((uint32_t)i << LOGT);
+        as[i] = barrett_mul(ld_stream(a.a + it.row + base + k), lds[lds_pos(k)], t.q, t.br0, t.br1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = IntArith::canon_fwd(x[j], t);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < G::E; ++i) {
+        const uint32_t k = tid + ((uint32_t)i << LOGT);
+        st_stream(a.out + it.row + base + k, add_mod(as[i], lds[lds_pos(k)], t.q));
+    }
 }
 
 // N <= 2^12: both transforms in LDS, 256 threads, E = N / 256.
@@ -196,8 +219,7 @@ __device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &i
         }
     FwdRounds<IntArith, LOGN, LOGT, 0, false>::run(sx, lds, tid, 0, 0, t);
     FwdRounds<IntArith, LOGN, LOGT, 0, false>::run(x, lds, tid, 0, 0, t);
-#pragma unroll
-    for (int j = 0; j < G::E; ++j) rlwe_mac(a, it, ((uint64_t)tid << G::LOGE) + j, x[j], sx[j], t);
+    rlwe_mac_rows<LOGN, LOGT>(a, it, 0, x, sx, lds, tid, t);
 }
 
 // N = 2^(12+R): column pass (stages 0..R-1) of s into scratch and of x into
@@ -233,8 +255,7 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
         FwdRounds<IntArith, LOGC, LOGT, 0, false>::run(sx, lds, tid, R, b, t);
         FwdRounds<IntArith, LOGC, LOGT, 0, false>::load_global(a.out + it.row + (uint64_t)b * C, x, tid, t);
         FwdRounds<IntArith, LOGC, LOGT, 0, false>::run(x, lds, tid, R, b, t);
-#pragma unroll
-        for (int j = 0; j < G::E; ++j) rlwe_mac(a, it, (uint64_t)b * C + ((uint64_t)tid << G::LOGE) + j, x[j], sx[j], t);
+        rlwe_mac_rows<LOGC, LOGT>(a, it, (uint64_t)b * C, x, sx, lds, tid, t);
     }
 }
 
