@@ -66,6 +66,27 @@ __device__ __forceinline__ gptr<T> as_global(const T *p) {
 __device__ __forceinline__ Shoup ld(gptr<Shoup> p) { return Shoup{p->w, p->wq}; }
 __device__ __forceinline__ ShoupF ld(gptr<ShoupF> p) { return ShoupF{p->w, p->wp}; }
 
+// Lane-indexed loads of 16-byte table entries through a buffer resource: the
+// table base sits in scalar registers, the wave-uniform part of the byte
+// offset in soffset and the per-lane part in one VGPR, so a load needs no
+// 64-bit address arithmetic of its own.  Word 3 is the CDNA raw-buffer
+// format (32-bit data format, no swizzle); the record count only bounds
+// offsets, which stay inside the table.  p must be the same on every lane:
+// it is read from the first one, so that the compiler never has to loop over
+// distinct resources.
+template <class Tw>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(gptr<Tw> p) {
+    const uint64_t a = (uint64_t)(const Tw *)p;
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return __builtin_amdgcn_make_buffer_rsrc((void *)((hi << 32) | lo), (short)0, 0x7fffffff, 0x00020000);
+}
+template <class Tw>
+__device__ __forceinline__ Tw ld_lane(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes, uint32_t uniform_bytes) {
+    static_assert(sizeof(Tw) == 16, "16-byte table entries");
+    return __builtin_bit_cast(Tw, __builtin_amdgcn_raw_buffer_load_b128(r, lane_bytes, uniform_bytes, 0));
+}
+
 // Per-modulus device view of one UintNttTable.
 struct DevTable {
     uint64_t q, q2;
@@ -254,21 +275,30 @@ struct F64Arith {
 // PERM: tw is a register-order table (DevTable::fwd_p), in which the twiddle of
 // chunk c*2^u + sub at stage S+u sits at 2^(S+u) + sub*2^S + c, so lanes with
 // consecutive c read consecutive entries (requires c < 2^S: single pass).
-template <class A, int R, bool FENCE = true, bool PERM = false>
+template <class A, int R, bool FENCE = true, bool PERM = false, bool LANE = false>
 __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t c,
                                          gptr<typename A::Tw> tw, const DevTable &t) {
+    using Tw = typename A::Tw;
     if constexpr (A::kReduceAtRoundStart) {
 #pragma unroll
         for (int j = 0; j < (1 << R); ++j) A::round_start(x[j], t);
     }
+    // LANE: c differs between lanes; the loads go through a buffer resource
+    // with the c-term as the lane offset (ld_lane)
+    const __amdgpu_buffer_rsrc_t rs = table_rsrc(tw);  // unused unless LANE
 #pragma unroll
     for (int u = 0; u < R; ++u) {
         const int half = 1 << (R - u - 1);
-        const uint32_t base = PERM ? (1u << (S + u)) + c : (1u << (S + u)) + (c << u);
+        const uint32_t lane = PERM ? c : (c << u);
+        const uint32_t base = (1u << (S + u)) + lane;
         const uint32_t step = PERM ? (1u << S) : 1u;
 #pragma unroll
         for (int sub = 0; sub < (1 << u); ++sub) {
-            const typename A::Tw w = ld(tw + base + sub * step);
+            Tw w;
+            if constexpr (LANE)
+                w = ld_lane<Tw>(rs, lane * (uint32_t)sizeof(Tw), ((1u << (S + u)) + sub * step) * (uint32_t)sizeof(Tw));
+            else
+                w = ld(tw + base + sub * step);
 #pragma unroll
             for (int jj = 0; jj < half; ++jj) {
                 const int j0 = sub * 2 * half + jj;
@@ -287,9 +317,11 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
 // transform's final stage (gap N/2).  PERM: itw is a register-order table
 // (DevTable::inv_p): within stage P+u's segment the twiddle of chunk
 // c*2^(R-u-1) + sub sits at sub*(N >> (P+R)) + c.
-template <class A, int R, bool LAST, bool FENCE = true, bool PERM = false>
+template <class A, int R, bool LAST, bool FENCE = true, bool PERM = false, bool LANE = false>
 __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t c, uint32_t n,
                                          gptr<typename A::Tw> itw, const DevTable &t) {
+    using Tw = typename A::Tw;
+    const __amdgpu_buffer_rsrc_t rs = table_rsrc(itw);  // LANE: as in ct_round
 #pragma unroll
     for (int u = 0; u < R; ++u) {
         const int half = 1 << u;
@@ -297,11 +329,17 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
 #pragma unroll
             for (int jj = 0; jj < half; ++jj) A::gs_last(x[jj], x[jj + half], t);
         } else {
-            const uint32_t base = 1u + n - (n >> (P + u)) + (PERM ? c : (c << (R - u - 1)));
+            const uint32_t lane = PERM ? c : (c << (R - u - 1));
+            const uint32_t seg = 1u + n - (n >> (P + u));
+            const uint32_t base = seg + lane;
             const uint32_t step = PERM ? (n >> (P + R)) : 1u;
 #pragma unroll
             for (int sub = 0; sub < (1 << (R - u - 1)); ++sub) {
-                const typename A::Tw w = ld(itw + base + sub * step);
+                Tw w;
+                if constexpr (LANE)
+                    w = ld_lane<Tw>(rs, lane * (uint32_t)sizeof(Tw), (seg + sub * step) * (uint32_t)sizeof(Tw));
+                else
+                    w = ld(itw + base + sub * step);
 #pragma unroll
                 for (int jj = 0; jj < half; ++jj) {
                     const int j0 = sub * 2 * half + jj;

@@ -20,6 +20,7 @@
 // bits select what a chunk does between its load and its store.
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 
 #include "ntt_device.hpp"
 #include "ntt_launch.hpp"
@@ -59,11 +60,12 @@ struct Occupancy {
 // What a chunk does after its forward rounds: the optional multiply by the
 // resident operand, then either the forward's store or the inverse rounds
 // and their store.  After the last forward round thread t holds local slots
-// t*E .. t*E+E-1.
-template <class A, int LOGC, int LOGT, int MODE>
+// t*E .. t*E+E-1.  hook() runs once on the way (InvRounds::run), or first
+// when there are no inverse rounds.
+template <class A, int LOGC, int LOGT, int MODE, class H = NoHook>
 __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, typename A::T *lds, uint32_t t,
                                               uint32_t b, uint32_t limb, uint32_t n, const DevTable &tab,
-                                              uint64_t *data) {
+                                              uint64_t *data, const H &hook = H()) {
     using G = Geo<LOGC, LOGT>;
     constexpr int LE = G::LOGE;
     // Keep the scheduler from hoisting the next phase's loads into the last
@@ -73,9 +75,11 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
         if constexpr (A::kRegisterOrder) {
             // single pass: the operand is stored in register order, entry
             // (t, j) at j*T + t, so each of the E loads of a wave is one row
-            const auto op = as_global(operand_of(a, (const A *)nullptr, LE)) + (uint64_t)limb * n + t;
+            const auto rs = table_rsrc(as_global(operand_of(a, (const A *)nullptr, LE)) + (uint64_t)limb * n);
+            using W = typename A::Tw;
 #pragma unroll
-            for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], ld(op + j * G::T), tab);
+            for (int j = 0; j < G::E; ++j)
+                x[j] = A::mul_operand(x[j], ld_lane<W>(rs, t * (uint32_t)sizeof(W), j * G::T * (uint32_t)sizeof(W)), tab);
         } else {
             const auto op = as_global(operand_of(a, (const A *)nullptr, LE)) + (uint64_t)limb * n +
                             (uint64_t)b * G::C + ((uint64_t)t << LE);
@@ -84,6 +88,7 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
         }
     }
     if constexpr ((MODE & M_INV) == 0) {
+        hook();
         uint64_t *dst = data + ((uint64_t)t << LE);
 #pragma unroll
         for (int j = 0; j < G::E; ++j) {
@@ -93,8 +98,13 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
     } else {
         constexpr bool LAST = (MODE & M_INV_LAST) != 0;
         __builtin_amdgcn_sched_barrier(0);
-        InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder>::run(x, lds, t, n, b, tab);
-        InvRounds<A, LOGC, LOGT, G::NR - 1, LAST, A::kRegisterOrder>::store_global(data, x, t, tab);
+        // With a hook (the streaming kernel, which holds the next
+        // polynomial's words through the last rounds) the inverse forms its
+        // addresses afresh rather than keeping the forward's alive.
+        uint32_t ti = t;
+        if constexpr (!std::is_same<H, NoHook>::value) asm volatile("" : "+v"(ti));
+        InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder>::run(x, lds, ti, n, b, tab, hook);
+        InvRounds<A, LOGC, LOGT, G::NR - 1, LAST, A::kRegisterOrder>::store_global(data, x, ti, tab);
     }
 }
 
@@ -143,34 +153,39 @@ __global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) vo
     }
 }
 
-// Waves per SIMD of the streaming kernel: as the chunk kernel's, but with
-// room for the next polynomial's coefficients (E more 64-bit registers), so
-// 16 coefficients per thread drop to two waves per SIMD (256 VGPRs).
+// Waves per SIMD of the streaming kernel: as the chunk kernel's.  The next
+// polynomial's coefficients are held only through the last rounds, whose
+// twiddles sit in scalar registers, so they need no room of their own.
 template <int LOGC, int LOGT, int MODE>
 struct StreamOccupancy {
-    static constexpr int base = Occupancy<LOGC, LOGT, MODE>::value;
-    static constexpr int value = (LOGC - LOGT) >= 4 ? (base > 2 ? 2 : base) : base;
+    static constexpr int value = Occupancy<LOGC, LOGT, MODE>::value;
 };
 
 // Persistent, software-pipelined form of the single-pass chunk kernel for
 // modes that start with forward stages and have no second operand.  Each
 // workgroup walks limb-polynomials blockIdx.x, +gridDim.x, ... (the grid is
-// at most one resident generation, so every workgroup has at least one) and
-// issues the loads of its next polynomial before the forward rounds of the
-// current one: they land while it computes, instead of every workgroup
-// loading at once at the start and storing at once at the end.  Outputs are
-// those of chunk_kernel with the same template arguments.
+// at most one resident generation, so every workgroup has at least one).
+// The loads of its next polynomial are issued as the inverse reaches its
+// rounds with scalar twiddle loads, so no twiddle wait of the current
+// polynomial waits for them (vector-memory counters complete in order); they
+// land while those rounds and the stores run, and the stores in turn drain
+// during the next polynomial's first rounds.  Outputs are those of
+// chunk_kernel with the same template arguments.
 template <class A, int LOGC, int LOGT, int MODE>
 __global__ __launch_bounds__(1 << LOGT, (StreamOccupancy<LOGC, LOGT, MODE>::value)) void stream_kernel(KArgs a) {
     static_assert((MODE & M_FWD) != 0 && (MODE & M_MUL_POLY) == 0, "streaming covers forward-first modes");
     using G = Geo<LOGC, LOGT>;
     using T = typename A::T;
     __shared__ T lds[G::LDS];
-    const uint32_t t = threadIdx.x;
+    const uint32_t t0 = threadIdx.x;
     uint64_t poly = blockIdx.x;
     uint64_t raw[G::E];
-    FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_raw(a.data + poly * G::C, raw, t);
+    FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_raw(a.data + poly * G::C, raw, t0);
     while (true) {
+        // an opaque copy of the thread index: the addresses derived from it
+        // are formed afresh in each iteration instead of held across the loop
+        uint32_t t = t0;
+        asm volatile("" : "+v"(t));
         const uint32_t limb = (uint32_t)(poly % a.L);
         const DevTable tab = a.tabs[limb];
         T x[G::E];
@@ -178,11 +193,11 @@ __global__ __launch_bounds__(1 << LOGT, (StreamOccupancy<LOGC, LOGT, MODE>::valu
         for (int j = 0; j < G::E; ++j) x[j] = A::load_fwd(raw[j], tab);
         const uint64_t next = poly + gridDim.x;
         const bool more = next < a.count;  // uniform over the workgroup
-        __builtin_amdgcn_sched_barrier(0);
-        if (more) FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_raw(a.data + next * G::C, raw, t);
-        __builtin_amdgcn_sched_barrier(0);
+        const auto prefetch = [&]() {
+            if (more) FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_raw(a.data + next * G::C, raw, t);
+        };
         FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, t, 0, 0, tab);
-        after_forward<A, LOGC, LOGT, MODE>(a, x, lds, t, 0, limb, G::C, tab, a.data + poly * G::C);
+        after_forward<A, LOGC, LOGT, MODE>(a, x, lds, t, 0, limb, G::C, tab, a.data + poly * G::C, prefetch);
         if (!more) break;
         poly = next;
     }

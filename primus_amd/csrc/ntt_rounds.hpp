@@ -22,6 +22,14 @@ struct Geo {
     // inverse rounds: full rounds, then the short one
     static constexpr int ir(int i) { return (REM != 0 && i == NR - 1) ? REM : LOGE; }
     static constexpr int is(int i) { return i * LOGE; }
+    // First inverse round whose twiddle indices are equal on all lanes of a
+    // wave (gaps of 64 and up, or of a whole smaller workgroup): it and the
+    // rounds after it read their twiddles with scalar loads.  NR if none.
+    static constexpr int first_scalar_inv() {
+        for (int i = 0; i < NR; ++i)
+            if (is(i) >= (LOGT < 6 ? LOGT : 6)) return i;
+        return NR;
+    }
     static_assert(LOGE >= 1, "a thread needs at least two coefficients");
 };
 
@@ -65,21 +73,27 @@ struct FwdRounds {
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))];
     }
+    // Global accesses split each slot into its thread part, fwd_slot(t, 0, 0),
+    // and its register part, fwd_slot(0, m, j), a constant (the slot is their
+    // sum): every access is then a uniform base plus one 32-bit lane offset.
     __device__ __forceinline__ static void load_global(const uint64_t *__restrict__ g, T *x, uint32_t t,
                                                        const DevTable &tab) {
+        const uint32_t lt = fwd_slot<LOGC, LOGT, S, R>(t, 0, 0);
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j)
-                x[(m << R) + j] = A::load_fwd(ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(t, m, j)), tab);
+                x[(m << R) + j] = A::load_fwd(ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(0, m, j) + lt), tab);
     }
     // The words load_global reads, unconverted, in the same register order
     // (converted with A::load_fwd before the rounds).
     __device__ __forceinline__ static void load_raw(const uint64_t *__restrict__ g, uint64_t *r, uint32_t t) {
+        const uint32_t lt = fwd_slot<LOGC, LOGT, S, R>(t, 0, 0);
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j) r[(m << R) + j] = ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(t, m, j));
+            for (int j = 0; j < (1 << R); ++j)
+                r[(m << R) + j] = ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(0, m, j) + lt);
     }
 
     // Runs rounds I .. NR-1.  st0: global stage of local stage 0; b: chunk index
@@ -96,15 +110,24 @@ struct FwdRounds {
         for (int m = 0; m < (G::E >> R); ++m) {
             constexpr int OB = LOGC - S - R;
             uint32_t cl;
-            if constexpr (OB >= LOGT) cl = (uint32_t)m >> (OB - LOGT);  // wave-uniform
+            if constexpr (OB >= LOGT) cl = (uint32_t)m >> (OB - LOGT);  // workgroup-uniform
             else cl = (t + ((uint32_t)m << LOGT)) >> OB;
+            // a wave's 64 lanes share cl: scalar twiddle loads
+            if constexpr (OB >= 6 && OB < LOGT) cl = __builtin_amdgcn_readfirstlane(cl);
+            constexpr bool LANE = OB < 6 && OB < LOGT;  // cl differs between lanes
             if constexpr (PERM)
-                ct_round<A, R, FENCE, true>(x + (m << R), st0 + S, (b << S) + cl, A::fwd_reg(tab, G::LOGE), tab);
+                ct_round<A, R, FENCE, true, LANE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd_reg(tab, G::LOGE),
+                                                  tab);
             else
-                ct_round<A, R, FENCE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
+                ct_round<A, R, FENCE, false, LANE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
         }
         if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1, FENCE, PERM>::run(x, lds, t, st0, b, tab);
     }
+};
+
+// The default hook of InvRounds::run: nothing.
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
 };
 
 template <class A, int LOGC, int LOGT, int I, bool LAST, bool PERM = false>
@@ -125,36 +148,51 @@ struct InvRounds {
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))];
     }
+    // thread part plus constant register part, as FwdRounds::load_global
     __device__ __forceinline__ static void store_global(uint64_t *__restrict__ g, const T *x, uint32_t t,
                                                         const DevTable &tab) {
+        const uint32_t lt = inv_slot<LOGC, LOGT, P, R>(t, 0, 0);
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j)
-                st_stream(g + inv_slot<LOGC, LOGT, P, R>(t, m, j), A::out_inv(x[(m << R) + j], tab));
+                st_stream(g + inv_slot<LOGC, LOGT, P, R>(0, m, j) + lt, A::out_inv(x[(m << R) + j], tab));
     }
 
+    // Runs rounds I .. NR-1.  hook() is called once, as round H starts
+    // (H: the first round with scalar twiddle loads, else the last round):
+    // vector loads it issues are then not waited for by later twiddle loads.
+    template <class H = NoHook>
     __device__ __forceinline__ static void run(T *x, T *lds, uint32_t t, uint32_t n, uint32_t b,
-                                               const DevTable &tab) {
+                                               const DevTable &tab, const H &hook = H()) {
         if constexpr (I > 0) {
             __syncthreads();
             InvRounds<A, LOGC, LOGT, I - 1, LAST, PERM>::store(lds, x, t);
             __syncthreads();
             load(lds, x, t);
         }
+        constexpr int HR = G::first_scalar_inv() < G::NR ? G::first_scalar_inv() : G::NR - 1;
+        if constexpr (I == HR) {
+            __builtin_amdgcn_sched_barrier(0);
+            hook();
+            __builtin_amdgcn_sched_barrier(0);
+        }
         constexpr bool last_here = LAST && (I + 1 == G::NR);
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m) {
             uint32_t cl;
-            if constexpr (P >= LOGT) cl = (uint32_t)m >> (P - LOGT);  // wave-uniform
+            if constexpr (P >= LOGT) cl = (uint32_t)m >> (P - LOGT);  // workgroup-uniform
             else cl = (t + ((uint32_t)m << LOGT)) >> P;
+            if constexpr (P >= 6 && P < LOGT) cl = __builtin_amdgcn_readfirstlane(cl);  // as in FwdRounds
+            constexpr bool LANE = P < 6 && P < LOGT;
             if constexpr (PERM)
-                gs_round<A, R, last_here, true, true>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n,
-                                                      A::inv_reg(tab, G::LOGE), tab);
+                gs_round<A, R, last_here, true, true, LANE>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n,
+                                                            A::inv_reg(tab, G::LOGE), tab);
             else
-                gs_round<A, R, last_here>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n, A::inv(tab), tab);
+                gs_round<A, R, last_here, true, false, LANE>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n,
+                                                             A::inv(tab), tab);
         }
-        if constexpr (I + 1 < G::NR) InvRounds<A, LOGC, LOGT, I + 1, LAST, PERM>::run(x, lds, t, n, b, tab);
+        if constexpr (I + 1 < G::NR) InvRounds<A, LOGC, LOGT, I + 1, LAST, PERM>::run(x, lds, t, n, b, tab, hook);
     }
 };
 
