@@ -77,9 +77,23 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
             // (t, j) at j*T + t, so each of the E loads of a wave is one row
             const auto rs = table_rsrc(as_global(operand_of(a, (const A *)nullptr, LE)) + (uint64_t)limb * n);
             using W = typename A::Tw;
+            // Every load issued before the first product: left to itself the
+            // scheduler pairs each load with its product and waits for each.
+            // The streaming kernel (a hook) has no registers to spare for it.
+            if constexpr (std::is_same<H, NoHook>::value) {
+                W w[G::E];
 #pragma unroll
-            for (int j = 0; j < G::E; ++j)
-                x[j] = A::mul_operand(x[j], ld_lane<W>(rs, t * (uint32_t)sizeof(W), j * G::T * (uint32_t)sizeof(W)), tab);
+                for (int j = 0; j < G::E; ++j)
+                    w[j] = ld_lane<W>(rs, t * (uint32_t)sizeof(W), j * G::T * (uint32_t)sizeof(W));
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < G::E; ++j) x[j] = A::mul_operand(x[j], w[j], tab);
+            } else {
+#pragma unroll
+                for (int j = 0; j < G::E; ++j)
+                    x[j] = A::mul_operand(x[j], ld_lane<W>(rs, t * (uint32_t)sizeof(W), j * G::T * (uint32_t)sizeof(W)),
+                                          tab);
+            }
         } else {
             const auto op = as_global(operand_of(a, (const A *)nullptr, LE)) + (uint64_t)limb * n +
                             (uint64_t)b * G::C + ((uint64_t)t << LE);
@@ -153,12 +167,16 @@ __global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) vo
     }
 }
 
-// Waves per SIMD of the streaming kernel: as the chunk kernel's.  The next
-// polynomial's coefficients are held only through the last rounds, whose
-// twiddles sit in scalar registers, so they need no room of their own.
+// Waves per SIMD of the streaming kernel: as the chunk kernel's when there
+// is an inverse, whose last rounds (scalar twiddles) hold the next
+// polynomial's coefficients at little cost.  Forward-only modes take them in
+// beside the forward's output, which spills at 16 coefficients per thread
+// and 128 VGPRs: those run at two waves per SIMD.
 template <int LOGC, int LOGT, int MODE>
 struct StreamOccupancy {
-    static constexpr int value = Occupancy<LOGC, LOGT, MODE>::value;
+    static constexpr int base = Occupancy<LOGC, LOGT, MODE>::value;
+    static constexpr bool wide = (MODE & M_INV) == 0 && (LOGC - LOGT) >= 4;
+    static constexpr int value = wide ? (base > 2 ? 2 : base) : base;
 };
 
 // Persistent, software-pipelined form of the single-pass chunk kernel for

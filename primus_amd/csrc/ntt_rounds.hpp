@@ -61,17 +61,24 @@ struct FwdRounds {
     using T = typename A::T;
     static constexpr int S = G::fs(I), R = G::fr(I);
 
+    // LDS exchanges.  A slot is its thread part plus its register part, and
+    // the two never carry into each other's low four bits (the thread part's
+    // low bits are t mod 2^OB, the register part's a multiple of 2^OB below
+    // 16), so lds_pos of the sum is the sum of the lds_pos: one thread base
+    // per round and a constant offset per access.
     __device__ __forceinline__ static void store(T *lds, const T *x, uint32_t t) {
+        T *base = lds + lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, 0, 0));
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j) lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))] = x[(m << R) + j];
+            for (int j = 0; j < (1 << R); ++j) base[lds_pos(fwd_slot<LOGC, LOGT, S, R>(0, m, j))] = x[(m << R) + j];
     }
     __device__ __forceinline__ static void load(const T *lds, T *x, uint32_t t) {
+        const T *base = lds + lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, 0, 0));
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(fwd_slot<LOGC, LOGT, S, R>(t, m, j))];
+            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = base[lds_pos(fwd_slot<LOGC, LOGT, S, R>(0, m, j))];
     }
     // Global accesses split each slot into its thread part, fwd_slot(t, 0, 0),
     // and its register part, fwd_slot(0, m, j), a constant (the slot is their
@@ -136,17 +143,20 @@ struct InvRounds {
     using T = typename A::T;
     static constexpr int P = G::is(I), R = G::ir(I);
 
+    // thread base plus constant offsets, as in FwdRounds::store
     __device__ __forceinline__ static void store(T *lds, const T *x, uint32_t t) {
+        T *base = lds + lds_pos(inv_slot<LOGC, LOGT, P, R>(t, 0, 0));
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j) lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))] = x[(m << R) + j];
+            for (int j = 0; j < (1 << R); ++j) base[lds_pos(inv_slot<LOGC, LOGT, P, R>(0, m, j))] = x[(m << R) + j];
     }
     __device__ __forceinline__ static void load(const T *lds, T *x, uint32_t t) {
+        const T *base = lds + lds_pos(inv_slot<LOGC, LOGT, P, R>(t, 0, 0));
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = lds[lds_pos(inv_slot<LOGC, LOGT, P, R>(t, m, j))];
+            for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = base[lds_pos(inv_slot<LOGC, LOGT, P, R>(0, m, j))];
     }
     // thread part plus constant register part, as FwdRounds::load_global
     __device__ __forceinline__ static void store_global(uint64_t *__restrict__ g, const T *x, uint32_t t,
