@@ -37,11 +37,11 @@ __device__ __forceinline__ uint64_t gadget_digit(uint64_t adj, uint32_t d, uint3
 // acc += x * key (mod q), one term of DcrtPolynomial::add_mul_assign
 // (primus_poly/src/dcrt/mod.rs:94-114); canonical for IntArith, centred and
 // reduced every third term for F64Arith (|acc| <= q/2 + 1 + 3 * 0.744 q).
-__device__ __forceinline__ void ext_mac(uint64_t &acc, uint64_t x, gptr<Shoup> key, const DevTable &t, bool) {
-    acc = add_mod(acc, shoup_full(key->w, key->wq, x, t.q), t.q);
+__device__ __forceinline__ void ext_mac(uint64_t &acc, uint64_t x, const Shoup &key, const DevTable &t, bool) {
+    acc = add_mod(acc, shoup_full(key.w, key.wq, x, t.q), t.q);
 }
-__device__ __forceinline__ void ext_mac(double &acc, double x, gptr<ShoupF> key, const DevTable &t, bool reduce) {
-    acc += F64Arith::mulmod(x, key->w, key->wp, t.qd);
+__device__ __forceinline__ void ext_mac(double &acc, double x, const ShoupF &key, const DevTable &t, bool reduce) {
+    acc += F64Arith::mulmod(x, key.w, key.wp, t.qd);
     if (reduce) acc = F64Arith::reduce(acc, t);
 }
 __device__ __forceinline__ gptr<Shoup> ext_key(const ExtArgs &g, const IntArith *) { return as_global(g.key); }
@@ -53,6 +53,11 @@ __device__ __forceinline__ void opaque(DevTable &t) {
 }
 __device__ __forceinline__ void ext_finish(uint64_t &, const DevTable &) {}
 __device__ __forceinline__ void ext_finish(double &acc, const DevTable &t) { acc = F64Arith::reduce(acc, t); }
+
+// Registers whose key entries (two rows each) the multiply-accumulate loads
+// in one batch.  Two fit beside the accumulators at 4 waves per SIMD; four
+// spill there, and 3 waves per SIMD with eight measured slower.
+constexpr int kExtKeyBatch = 2;
 
 template <class A, int LOGN>
 __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(ExtArgs g) {
@@ -100,16 +105,32 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
             asm volatile("" : "+v"(tl));
             FwdRounds<A, LOGN, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, tl, 0, 0, td);
             // thread t now holds slots t*E + j; key rows are stored permuted so
-            // that (row, t*E + j) sits at row*N + j*T + t: coalesced loads.
-            // Loads are issued a quarter at a time to bound live registers.
+            // that (row, t*E + j) sits at row*N + j*T + t: coalesced loads,
+            // one lane offset (t) and a uniform offset per register.  The
+            // loads of KB registers' key entries are all issued before their
+            // products: left to itself the scheduler waits for each load in
+            // turn.
             const bool red = (++term % 3) == 0;
+            using W = typename A::Tw;
+            const auto krs = table_rsrc(key + ((uint64_t)(p * g.l + d) * 2) * n);
+            constexpr int KB = kExtKeyBatch < G::E ? kExtKeyBatch : G::E;
+            static_assert(G::E % KB == 0, "key batches tile the register group");
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int j = 0; j < G::E; ++j) {
-                const uint64_t base = ((uint64_t)(p * g.l + d) * 2) * n + (uint64_t)j * G::T + t;
-                ext_mac(acc[0][j], x[j], key + base, tab, red);
-                ext_mac(acc[1][j], x[j], key + base + n, tab, red);
-                if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            for (int j0 = 0; j0 < G::E; j0 += KB) {
+                W k0[KB], k1[KB];
+#pragma unroll
+                for (int jj = 0; jj < KB; ++jj) {
+                    k0[jj] = ld_lane<W>(krs, t * (uint32_t)sizeof(W), (j0 + jj) * G::T * (uint32_t)sizeof(W));
+                    k1[jj] = ld_lane<W>(krs, t * (uint32_t)sizeof(W), ((j0 + jj) * G::T + n) * (uint32_t)sizeof(W));
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int jj = 0; jj < KB; ++jj) {
+                    ext_mac(acc[0][j0 + jj], x[j0 + jj], k0[jj], tab, red);
+                    ext_mac(acc[1][j0 + jj], x[j0 + jj], k1[jj], tab, red);
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
@@ -163,11 +184,10 @@ hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64
 
 __device__ __forceinline__ uint64_t lift_small(int v, uint64_t q) { return v >= 0 ? (uint64_t)v : q - (uint64_t)(-v); }
 
-// e + Delta*m (mod q): sample_gaussian_values_inplace then add_mul_factor_assign
-// (primus_fhe_core/src/secret_key/rlwe.rs:245-260).
-__device__ __forceinline__ uint64_t rlwe_x(const RlweArgs &a, const RlweItem &it, uint64_t k, const DevTable &t) {
-    const uint64_t e = lift_small(a.e[it.small + k], t.q);
-    return add_mod(e, shoup_full(it.delta, it.delta_q, a.m[it.small + k], t.q), t.q);
+// e + Delta*m (mod q) from the small values e and m: sample_gaussian_values_inplace
+// then add_mul_factor_assign (primus_fhe_core/src/secret_key/rlwe.rs:245-260).
+__device__ __forceinline__ uint64_t rlwe_x(const RlweItem &it, int8_t e, uint8_t m, const DevTable &t) {
+    return add_mod(lift_small(e, t.q), shoup_full(it.delta, it.delta_q, m, t.q), t.q);
 }
 
 // b_hat = x_hat + a_hat (.) s_hat over one chunk of C = 2^LOGC slots starting
@@ -186,9 +206,12 @@ __device__ __forceinline__ void rlwe_mac_rows(const RlweArgs &a, const RlweItem 
     for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = IntArith::canon_fwd(sx[j], t);
     __syncthreads();
 #pragma unroll
+    for (int i = 0; i < G::E; ++i) as[i] = ld_stream(a.a + it.row + base + tid + ((uint32_t)i << LOGT));
+    __builtin_amdgcn_sched_barrier(0);  // every row load issued before the first product
+#pragma unroll
     for (int i = 0; i < G::E; ++i) {
         const uint32_t k = tid + ((uint32_t)i << LOGT);
-        as[i] = barrett_mul(ld_stream(a.a + it.row + base + k), lds[lds_pos(k)], t.q, t.br0, t.br1);
+        as[i] = barrett_mul(as[i], lds[lds_pos(k)], t.q, t.br0, t.br1);
     }
     __syncthreads();
 #pragma unroll
@@ -209,16 +232,27 @@ __device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &i
     using G = Geo<LOGN, LOGT>;
     constexpr int R0 = G::fr(0);
     uint64_t sx[G::E], x[G::E];
+    // the small values' loads all go out before any of them is used
+    int8_t sv[G::E], ev[G::E];
+    uint8_t mv[G::E];
+    const uint32_t lt = fwd_slot<LOGN, LOGT, 0, R0>(tid, 0, 0);
 #pragma unroll
     for (int m = 0; m < (G::E >> R0); ++m)
 #pragma unroll
         for (int j = 0; j < (1 << R0); ++j) {
-            const uint32_t k = fwd_slot<LOGN, LOGT, 0, R0>(tid, m, j);
-            sx[(m << R0) + j] = lift_small(a.s[it.small + k], t.q);
-            x[(m << R0) + j] = rlwe_x(a, it, k, t);
+            const uint64_t k = it.small + lt + fwd_slot<LOGN, LOGT, 0, R0>(0, m, j);
+            sv[(m << R0) + j] = a.s[k];
+            ev[(m << R0) + j] = a.e[k];
+            mv[(m << R0) + j] = a.m[k];
         }
-    FwdRounds<IntArith, LOGN, LOGT, 0, false>::run(sx, lds, tid, 0, 0, t);
-    FwdRounds<IntArith, LOGN, LOGT, 0, false>::run(x, lds, tid, 0, 0, t);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < G::E; ++i) {
+        sx[i] = lift_small(sv[i], t.q);
+        x[i] = rlwe_x(it, ev[i], mv[i], t);
+    }
+    FwdRounds<IntArith, LOGN, LOGT, 0, true>::run(sx, lds, tid, 0, 0, t);
+    FwdRounds<IntArith, LOGN, LOGT, 0, true>::run(x, lds, tid, 0, 0, t);
     rlwe_mac_rows<LOGN, LOGT>(a, it, 0, x, sx, lds, tid, t);
 }
 
@@ -233,11 +267,20 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
     for (uint32_t cc = 0; cc < C / kRlweThreads; ++cc) {
         const uint32_t col = tid + cc * kRlweThreads;
         uint64_t v[1 << R], w[1 << R];
+        int8_t sv[1 << R], ev[1 << R];
+        uint8_t mv[1 << R];
 #pragma unroll
         for (int i = 0; i < (1 << R); ++i) {
-            const uint64_t k = (uint64_t)i * C + col;
-            v[i] = lift_small(a.s[it.small + k], t.q);
-            w[i] = rlwe_x(a, it, k, t);
+            const uint64_t k = it.small + (uint64_t)i * C + col;
+            sv[i] = a.s[k];
+            ev[i] = a.e[k];
+            mv[i] = a.m[k];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // as in rlwe_single
+#pragma unroll
+        for (int i = 0; i < (1 << R); ++i) {
+            v[i] = lift_small(sv[i], t.q);
+            w[i] = rlwe_x(it, ev[i], mv[i], t);
         }
         ct_round<IntArith, R, false>(v, 0, 0, IntArith::fwd(t), t);
         ct_round<IntArith, R, false>(w, 0, 0, IntArith::fwd(t), t);
@@ -251,10 +294,10 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
     for (uint32_t b = 0; b < (1u << R); ++b) {
         asm volatile("" : "+v"(tid));  // keep slot addresses out of loop-invariant code motion
         uint64_t sx[G::E], x[G::E];
-        FwdRounds<IntArith, LOGC, LOGT, 0, false>::load_global(scr + (uint64_t)b * C, sx, tid, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0, false>::run(sx, lds, tid, R, b, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0, false>::load_global(a.out + it.row + (uint64_t)b * C, x, tid, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0, false>::run(x, lds, tid, R, b, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0, true>::load_global(scr + (uint64_t)b * C, sx, tid, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0, true>::run(sx, lds, tid, R, b, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0, true>::load_global(a.out + it.row + (uint64_t)b * C, x, tid, t);
+        FwdRounds<IntArith, LOGC, LOGT, 0, true>::run(x, lds, tid, R, b, t);
         rlwe_mac_rows<LOGC, LOGT>(a, it, (uint64_t)b * C, x, sx, lds, tid, t);
     }
 }
