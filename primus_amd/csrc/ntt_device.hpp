@@ -292,17 +292,24 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
         const uint32_t lane = PERM ? c : (c << u);
         const uint32_t base = (1u << (S + u)) + lane;
         const uint32_t step = PERM ? (1u << S) : 1u;
+        Tw w[1 << (R - 1)];
 #pragma unroll
         for (int sub = 0; sub < (1 << u); ++sub) {
-            Tw w;
             if constexpr (LANE)
-                w = ld_lane<Tw>(rs, lane * (uint32_t)sizeof(Tw), ((1u << (S + u)) + sub * step) * (uint32_t)sizeof(Tw));
+                w[sub] = ld_lane<Tw>(rs, lane * (uint32_t)sizeof(Tw), ((1u << (S + u)) + sub * step) * (uint32_t)sizeof(Tw));
             else
-                w = ld(tw + base + sub * step);
+                w[sub] = ld(tw + base + sub * step);
+        }
+        // A stage's vector loads all go out before its first butterfly; under
+        // register pressure the scheduler would otherwise wait for each load
+        // in turn.
+        if constexpr (FENCE && LANE) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int sub = 0; sub < (1 << u); ++sub) {
 #pragma unroll
             for (int jj = 0; jj < half; ++jj) {
                 const int j0 = sub * 2 * half + jj;
-                A::ct(x[j0], x[j0 + half], w, t);
+                A::ct(x[j0], x[j0 + half], w[sub], t);
             }
         }
         // one stage at a time: hoisting every stage's twiddles to the top of
@@ -333,17 +340,21 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
             const uint32_t seg = 1u + n - (n >> (P + u));
             const uint32_t base = seg + lane;
             const uint32_t step = PERM ? (n >> (P + R)) : 1u;
+            Tw w[1 << (R - 1)];
 #pragma unroll
             for (int sub = 0; sub < (1 << (R - u - 1)); ++sub) {
-                Tw w;
                 if constexpr (LANE)
-                    w = ld_lane<Tw>(rs, lane * (uint32_t)sizeof(Tw), (seg + sub * step) * (uint32_t)sizeof(Tw));
+                    w[sub] = ld_lane<Tw>(rs, lane * (uint32_t)sizeof(Tw), (seg + sub * step) * (uint32_t)sizeof(Tw));
                 else
-                    w = ld(itw + base + sub * step);
+                    w[sub] = ld(itw + base + sub * step);
+            }
+            if constexpr (FENCE && LANE) __builtin_amdgcn_sched_barrier(0);  // as in ct_round
+#pragma unroll
+            for (int sub = 0; sub < (1 << (R - u - 1)); ++sub) {
 #pragma unroll
                 for (int jj = 0; jj < half; ++jj) {
                     const int j0 = sub * 2 * half + jj;
-                    A::gs(x[j0], x[j0 + half], w, P + u, t);
+                    A::gs(x[j0], x[j0 + half], w[sub], P + u, t);
                 }
             }
         }

@@ -123,7 +123,8 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
 }
 
 template <class A, int LOGC, int LOGT, int MODE>
-__global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) void chunk_kernel(KArgs a) {
+__global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value))
+__attribute__((amdgpu_waves_per_eu(1, Occupancy<LOGC, LOGT, MODE>::value))) void chunk_kernel(KArgs a) {
     using G = Geo<LOGC, LOGT>;
     using T = typename A::T;
     constexpr int LE = G::LOGE;
@@ -167,16 +168,13 @@ __global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value)) vo
     }
 }
 
-// Waves per SIMD of the streaming kernel: as the chunk kernel's when there
-// is an inverse, whose last rounds (scalar twiddles) hold the next
-// polynomial's coefficients at little cost.  Forward-only modes take them in
-// beside the forward's output, which spills at 16 coefficients per thread
-// and 128 VGPRs: those run at two waves per SIMD.
+// Waves per SIMD of the streaming kernel: as the chunk kernel's, but two at
+// 16 coefficients per thread, where the next polynomial's coefficients beside
+// the current one's spill at 128 VGPRs.
 template <int LOGC, int LOGT, int MODE>
 struct StreamOccupancy {
     static constexpr int base = Occupancy<LOGC, LOGT, MODE>::value;
-    static constexpr bool wide = (MODE & M_INV) == 0 && (LOGC - LOGT) >= 4;
-    static constexpr int value = wide ? (base > 2 ? 2 : base) : base;
+    static constexpr int value = (LOGC - LOGT) >= 4 ? (base > 2 ? 2 : base) : base;
 };
 
 // Persistent, software-pipelined form of the single-pass chunk kernel for
