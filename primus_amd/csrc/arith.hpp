@@ -51,6 +51,29 @@ __device__ __forceinline__ uint64_t barrett_reduce128(uint64_t lo, uint64_t hi, 
     return sub_if_ge(lo - qq * q, q);
 }
 
+// The same quotient for any (hi:lo) < q * 2^64 (hi < q): the estimate is then
+// at most three below the true quotient (one from the ratio's truncation,
+// two from the dropped low partial products), so up to three subtractions
+// make the remainder canonical.
+__device__ __forceinline__ uint64_t barrett_reduce128_wide(uint64_t lo, uint64_t hi, uint64_t q, uint64_t r0,
+                                                           uint64_t r1) {
+    uint64_t ah = mulhi64(lo, r0);
+    uint64_t b_lo = lo * r1;
+    uint64_t b_hi = mulhi64(lo, r1);
+    b_lo += ah;
+    b_hi += (b_lo < ah);
+    uint64_t c_lo = hi * r0;
+    uint64_t c_hi = mulhi64(hi, r0);
+    uint64_t d = hi * r1;
+    uint64_t s = b_lo + c_lo;
+    uint64_t bch = b_hi + c_hi + (s < b_lo);
+    uint64_t qq = d + bch;
+    uint64_t r = lo - qq * q;
+    r = sub_if_ge(r, q);
+    r = sub_if_ge(r, q);
+    return sub_if_ge(r, q);
+}
+
 __device__ __forceinline__ uint64_t barrett_mul(uint64_t a, uint64_t b, uint64_t q, uint64_t r0,
                                                 uint64_t r1) {
     return barrett_reduce128(a * b, mulhi64(a, b), q, r0, r1);

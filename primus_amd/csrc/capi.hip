@@ -786,6 +786,33 @@ int ph_base_converter_new(const uint64_t *ibase, uint32_t in_count, const uint64
             m[j * I + i] = punct;
         }
     }
+    // binary64 form when every modulus is below 2^50: centred pairs as for the
+    // twiddles (mkf), appended to the same image (16-byte entries first)
+    bool f64 = true;
+    for (size_t i = 0; i < I; ++i) f64 = f64 && q[i] < (1ull << 50);
+    for (size_t j = 0; j < J; ++j) f64 = f64 && p[j] < (1ull << 50);
+    const size_t int_words = img.size() + (img.size() & 1);  // keep the pairs 16-byte aligned
+    size_t off_inv_f = 0, off_m_f = 0, off_qmp_f = 0, off_pd = 0, off_pinv = 0;
+    if (f64) {
+        std::vector<ShoupF> pairs;
+        for (size_t i = 0; i < I; ++i) pairs.push_back(mkf(Shoup{inv[i], inv_q[i]}, q[i]));
+        for (size_t j = 0; j < J; ++j)
+            for (size_t i = 0; i < I; ++i)
+                pairs.push_back(mkf(Shoup{m[j * I + i], h_shoup_quotient(m[j * I + i], p[j])}, p[j]));
+        for (size_t j = 0; j < J; ++j) pairs.push_back(mkf(Shoup{q_mod_p[j], h_shoup_quotient(q_mod_p[j], p[j])}, p[j]));
+        std::vector<double> scal;
+        for (size_t j = 0; j < J; ++j) scal.push_back((double)p[j]);
+        for (size_t j = 0; j < J; ++j) scal.push_back(1.0 / (double)p[j]);
+        // (the host pointers q, p, m, ... into img are not used past this point)
+        img.resize(int_words + pairs.size() * 2 + scal.size());
+        std::memcpy(img.data() + int_words, pairs.data(), pairs.size() * sizeof(ShoupF));
+        std::memcpy(img.data() + int_words + pairs.size() * 2, scal.data(), scal.size() * sizeof(double));
+        off_inv_f = int_words;
+        off_m_f = off_inv_f + 2 * I;
+        off_qmp_f = off_m_f + 2 * J * I;
+        off_pd = int_words + pairs.size() * 2;
+        off_pinv = off_pd + J;
+    }
     ph_base_converter *c = new (std::nothrow) ph_base_converter();
     if (!c) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
     c->in_count = in_count;
@@ -796,9 +823,41 @@ int ph_base_converter_new(const uint64_t *ibase, uint32_t in_count, const uint64
         return rc;
     }
     const uint64_t *base = (const uint64_t *)c->d_mem;
-    auto at = [&](const void *h) { return base + ((const uint64_t *)h - img.data()); };
-    c->dev = BaseConvDev{in_count, out_count, at(q), at(q_r0), at(q_r1), at(inv), at(inv_q),
-                         (const double *)at(qd), at(p), at(p_r0), at(p_r1), at(m), at(q_mod_p)};
+    size_t o = 0;
+    auto next = [&](size_t words) {
+        const uint64_t *ptr = base + o;
+        o += words;
+        return ptr;
+    };
+    c->dev.in_count = in_count;
+    c->dev.out_count = out_count;
+    c->dev.q = next(I);
+    c->dev.q_r0 = next(I);
+    c->dev.q_r1 = next(I);
+    c->dev.inv = next(I);
+    c->dev.inv_q = next(I);
+    c->dev.qd = (const double *)next(I);
+    c->dev.p = next(J);
+    c->dev.p_r0 = next(J);
+    c->dev.p_r1 = next(J);
+    c->dev.m = next(J * I);
+    c->dev.q_mod_p = next(J);
+    {
+        // the sum of k products y_i (Q/q_i mod p) stays below p * 2^64 while
+        // k * max q_i <= 2^64: barrett_reduce128_wide's range
+        uint64_t qmax = 0;
+        for (size_t i = 0; i < I; ++i) qmax = std::max(qmax, ibase[i]);
+        const u128 k = ((u128)1 << 64) / qmax;
+        c->dev.lazy_terms = (uint32_t)std::min<u128>(k, (u128)I);
+    }
+    c->dev.f64 = f64 ? 1u : 0u;
+    if (f64) {
+        c->dev.inv_f = (const ShoupF *)(base + off_inv_f);
+        c->dev.m_f = (const ShoupF *)(base + off_m_f);
+        c->dev.qmp_f = (const ShoupF *)(base + off_qmp_f);
+        c->dev.pd = (const double *)(base + off_pd);
+        c->dev.pinv = (const double *)(base + off_pinv);
+    }
     *out = c;
     return PH_OK;
 }

@@ -6,6 +6,8 @@
 
 namespace ph {
 
+struct ShoupF;
+
 constexpr uint32_t kMaxRnsModuli = 32;
 
 // BaseConverter (primus_rns/src/converter.rs:11-46) on the device: every
@@ -18,6 +20,15 @@ struct BaseConvDev {
     const uint64_t *p, *p_r0, *p_r1;  // output moduli and their Barrett ratios
     const uint64_t *m;                // [out][in]: (Q/q_i) mod p_j
     const uint64_t *q_mod_p;          // [out]: Q mod p_j
+    uint32_t lazy_terms;              // products summed in 128 bits per reduction: floor(2^64 / max q_i)
+    // The binary64 form (F64Arith, ntt_device.hpp), present when every input
+    // and output modulus is below 2^50; f64 != 0 selects it.
+    uint32_t f64;
+    const ShoupF *inv_f;  // [(Q/q_i)^-1]_{q_i}, centred pairs {w, w/q}
+    const ShoupF *m_f;    // [out][in]: (Q/q_i) mod p_j, centred pairs
+    const ShoupF *qmp_f;  // [out]: Q mod p_j, centred pairs
+    const double *pd;     // [out]: p_j
+    const double *pinv;   // [out]: 1/p_j
 };
 
 // `polys` CRT polynomials of 2^log_n coefficients: in [polys][in_count][N],
