@@ -16,6 +16,10 @@ CASES = [
     (P50, P59),
     ([2147473409, 4294957057, 1152921504606830593, 12289, 17], [576460752303421441, 97]),
     ([P59[0]], [P50[0], 65537]),  # one input modulus: the factor is one, inputs are reduced
+    # six moduli just below 2^62: the integer kernel sums at most 4 products per
+    # reduction (floor(2^64 / max q)), so each dot product is reduced in two parts
+    ([4611686018427387847, 4611686018427387817, 4611686018427387787, 4611686018427387761,
+      4611686018427387751, 4611686018427387737], [P59[0], P50[0]]),
 ]
 
 
