@@ -373,6 +373,6 @@ __device__ __forceinline__ void st_stream(uint64_t *p, uint64_t v) { __builtin_n
 
 // LDS slot of local index k: one pad word per 16 keeps the stride-16 and
 // stride-1 access patterns of the rounds free of bank conflicts.
-__device__ __forceinline__ uint32_t lds_pos(uint32_t k) { return k + (k >> 4); }
+__host__ __device__ __forceinline__ uint32_t lds_pos(uint32_t k) { return k + (k >> 4); }
 
 }  // namespace ph

@@ -35,7 +35,7 @@ struct Geo {
 
 // Local slot of register (m, j) in the forward round covering local stages [S, S+R).
 template <int LOGC, int LOGT, int S, int R>
-__device__ __forceinline__ uint32_t fwd_slot(uint32_t t, int m, int j) {
+__host__ __device__ __forceinline__ uint32_t fwd_slot(uint32_t t, int m, int j) {
     constexpr int OB = LOGC - S - R;
     const uint32_t gi = t + ((uint32_t)m << LOGT);
     const uint32_t c = gi >> OB;
@@ -45,7 +45,7 @@ __device__ __forceinline__ uint32_t fwd_slot(uint32_t t, int m, int j) {
 
 // Local slot of register (m, j) in the inverse round covering local stages [P, P+R).
 template <int LOGC, int LOGT, int P, int R>
-__device__ __forceinline__ uint32_t inv_slot(uint32_t t, int m, int j) {
+__host__ __device__ __forceinline__ uint32_t inv_slot(uint32_t t, int m, int j) {
     const uint32_t gi = t + ((uint32_t)m << LOGT);
     const uint32_t c = gi >> P;
     const uint32_t o = gi & ((1u << P) - 1u);
