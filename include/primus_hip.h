@@ -286,6 +286,22 @@ int ph_rlwe_plan_autotune(ph_rlwe_plan *plan, const uint64_t *d_a, const int8_t 
                           const uint8_t *d_m, uint64_t *d_out, int reps, uint32_t *grid, float *best_ms);
 /* Workgroups of the plan's launch. */
 uint32_t ph_rlwe_plan_grid(const ph_rlwe_plan *plan);
+/* Draws the encryption inputs on the device, laid out as for
+ * ph_rlwe_encrypt_device: s ternary uniform in {-1, 0, 1}, e the rounded
+ * discrete Gaussian of sigma 3.2 cut at |e| <= 19, m uniform in [0, 256) and
+ * a_hat uniform mod q_i.  Replaces the host sampling of
+ * NttRlweSecretKey::encrypt_inplace_with_embedding
+ * (primus_fhe_core/src/secret_key/rlwe.rs:227-268, samplers in
+ * primus_distr/src/common.rs:56-121 and primus_distr/src/discrete_gaussian/mod.rs)
+ * with the distributions of the batch workload.  Every value is a function of
+ * (seed, request index, what is drawn, element) through Philox4x32-10, so the
+ * result is the same however the launch is arranged. */
+int ph_rlwe_sample_device(const ph_rlwe_ctx *ctx, const ph_rlwe_request *requests, uint32_t n_requests,
+                          uint64_t seed, uint64_t *d_a, int8_t *d_s, int8_t *d_e, uint8_t *d_m, void *stream);
+/* The same for a plan's requests: one kernel enqueue (capturable, like
+ * ph_rlwe_encrypt_plan_device). */
+int ph_rlwe_sample_plan_device(const ph_rlwe_plan *plan, uint64_t seed, uint64_t *d_a, int8_t *d_s, int8_t *d_e,
+                               uint8_t *d_m, void *stream);
 
 /* ---- serving: continuous batching (no reference counterpart; the reference is a
  * single-threaded library, this is the request loop around the batched forms) ---- */

@@ -336,3 +336,44 @@ class BaseConverter:
     def exact_convert(self, x):
         """[len(ibase)][n] residues -> [1][n] (obase holds one modulus)."""
         return self._run(lib().po_exact_convert_array, x, 1)
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 block of counter ctr (4 x u32) under key (2 x u32)."""
+    L = lib()
+    L.po_philox4x32_10.restype = None
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.po_philox4x32_10.argtypes = [u32p, u32p, u32p]
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    L.po_philox4x32_10(c.ctypes.data_as(u32p), k.ctypes.data_as(u32p), out.ctypes.data_as(u32p))
+    return out
+
+
+def gauss_thresholds():
+    """The 38 cumulative thresholds of the rounded Gaussian (sigma 3.2, |e| <= 19)."""
+    L = lib()
+    L.po_gauss_thresholds.restype = None
+    L.po_gauss_thresholds.argtypes = [_p64]
+    t = np.zeros(38, dtype=np.uint64)
+    L.po_gauss_thresholds(ptr(t))
+    return t
+
+
+def rlwe_sample(seed, request, log_n, moduli):
+    """One request's (s, e, m, a_hat [limbs][N]) drawn as po_rlwe_sample does."""
+    L = lib()
+    vp = ctypes.c_void_p
+    L.po_rlwe_sample.restype = None
+    L.po_rlwe_sample.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _p64,
+                                 vp, vp, vp, _p64]
+    n = 1 << log_n
+    mods = np.ascontiguousarray(moduli, dtype=np.uint64)
+    s = np.zeros(n, dtype=np.int8)
+    e = np.zeros(n, dtype=np.int8)
+    m = np.zeros(n, dtype=np.uint8)
+    a = np.zeros(len(moduli) * n, dtype=np.uint64)
+    L.po_rlwe_sample(seed, request, log_n, len(moduli), ptr(mods), s.ctypes.data_as(vp), e.ctypes.data_as(vp),
+                     m.ctypes.data_as(vp), ptr(a))
+    return s, e, m, a.reshape(len(moduli), n)

@@ -12,6 +12,7 @@
  */
 #include "primus_oracle.h"
 
+#include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -766,4 +767,77 @@ void po_exact_convert_array(const po_base_converter *c, const uint64_t *in, uint
         out[k] = s >= vq ? s - vq : s + p - vq;
     }
     free(y);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sampling of RLWE encryption inputs                                        */
+/* ------------------------------------------------------------------------ */
+
+void po_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t a = (uint64_t)0xD2511F53u * x0; /* Philox multipliers */
+        const uint64_t b = (uint64_t)0xCD9E8D57u * x2;
+        const uint32_t y0 = (uint32_t)(b >> 32) ^ x1 ^ k0;
+        const uint32_t y2 = (uint32_t)(a >> 32) ^ x3 ^ k1;
+        x1 = (uint32_t)b;
+        x3 = (uint32_t)a;
+        x0 = y0;
+        x2 = y2;
+        k0 += 0x9E3779B9u; /* Weyl key increments */
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = x0;
+    out[1] = x1;
+    out[2] = x2;
+    out[3] = x3;
+}
+
+void po_gauss_thresholds(uint64_t t[38]) {
+    const double sigma = 3.2;
+    double w[39], z = 0.0;
+    for (int k = -19; k <= 19; ++k) {
+        w[k + 19] = exp(-(double)(k * k) / (2.0 * sigma * sigma));
+        z += w[k + 19];
+    }
+    double cum = 0.0;
+    for (int i = 0; i < 38; ++i) {
+        cum += w[i];
+        t[i] = (uint64_t)ldexp(cum / z, 64);
+    }
+}
+
+static void draw(uint64_t seed, uint32_t request, uint32_t purpose, uint32_t k, uint64_t *lo, uint64_t *hi) {
+    const uint32_t ctr[4] = {k, request, purpose, 0u};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    po_philox4x32_10(ctr, key, o);
+    *lo = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
+    *hi = (uint64_t)o[2] | ((uint64_t)o[3] << 32);
+}
+
+void po_rlwe_sample(uint64_t seed, uint32_t request, uint32_t log_n, uint32_t limbs, const uint64_t *moduli,
+                    int8_t *s, int8_t *e, uint8_t *m, uint64_t *a_hat) {
+    const uint32_t n = 1u << log_n;
+    uint64_t t[38];
+    po_gauss_thresholds(t);
+    for (uint32_t k = 0; k < n; ++k) {
+        uint64_t lo, hi;
+        draw(seed, request, 0, k, &lo, &hi);
+        s[k] = (int8_t)((int)(((unsigned __int128)lo * 3u) >> 64) - 1); /* floor(3u / 2^64) - 1 */
+        draw(seed, request, 1, k, &lo, &hi);
+        int v = -19; /* inversion of the cumulative table */
+        while (v < 19 && lo >= t[v + 19]) ++v;
+        e[k] = (int8_t)v;
+        draw(seed, request, 2, k, &lo, &hi);
+        m[k] = (uint8_t)(lo & 255u);
+        for (uint32_t j = 0; j < limbs; ++j) {
+            draw(seed, request, 3 + j, k, &lo, &hi);
+            /* floor((hi:lo) * q / 2^128) */
+            const unsigned __int128 prod_hi = (unsigned __int128)hi * moduli[j];
+            const unsigned __int128 prod_lo = ((unsigned __int128)lo * moduli[j]) >> 64;
+            a_hat[(size_t)j * n + k] = (uint64_t)((prod_hi + prod_lo) >> 64);
+        }
+    }
 }

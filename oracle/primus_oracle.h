@@ -158,6 +158,20 @@ extern "C" {
  * primus_fhe_core/src/secret_key/rlwe.rs:227-268 */
 void po_rlwe_encrypt(const po_table *t, const int8_t *s, const int8_t *e, const uint8_t *m, uint64_t delta,
                      const uint64_t *a_hat, uint64_t *b_hat);
+
+/* Sampling of the encryption inputs from a counter-based generator (the batch
+ * workload's distributions; the reference samples on the host with rand's
+ * thread RNG, primus_distr/src/common.rs:56-121).  Philox4x32-10 as published
+ * by Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as easy as 1, 2,
+ * 3" (SC 2011). */
+void po_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* The 38 cumulative thresholds of the rounded Gaussian, sigma 3.2, |e| <= 19. */
+void po_gauss_thresholds(uint64_t t[38]);
+/* One request's s, e, m [N] and a_hat [limbs][N]: element k of what-is-drawn p
+ * (0 s, 1 e, 2 m, 3 + j a_hat limb j) comes from the block of counter
+ * (k, request, p, 0) under key (seed low, seed high). */
+void po_rlwe_sample(uint64_t seed, uint32_t request, uint32_t log_n, uint32_t limbs, const uint64_t *moduli,
+                    int8_t *s, int8_t *e, uint8_t *m, uint64_t *a_hat);
 #ifdef __cplusplus
 }
 #endif

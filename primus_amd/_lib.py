@@ -165,6 +165,8 @@ def _declare(L):
         "ph_rlwe_encrypt_plan_device": (i, [vp, vp, vp, vp, vp, vp, vp]),
         "ph_rlwe_plan_autotune": (i, [vp, vp, vp, vp, vp, vp, i, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float)]),
         "ph_rlwe_plan_grid": (u32, [vp]),
+        "ph_rlwe_sample_device": (i, [vp, vp, u32, u64, vp, vp, vp, vp, vp]),
+        "ph_rlwe_sample_plan_device": (i, [vp, u64, vp, vp, vp, vp, vp]),
         "ph_host_alloc": (i, [sz, pvp]),
         "ph_host_free": (i, [vp]),
         "ph_tuning_set": (i, [u32, i, i, i]),

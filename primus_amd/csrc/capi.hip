@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -1176,6 +1177,83 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
 }
 
+// Cumulative thresholds of the rounded Gaussian (sigma 3.2, |e| <= 19): the
+// weights exp(-k^2 / (2 sigma^2)) for k = -19 .. 19 summed in order, and
+// threshold i = floor(2^64 * (w_-19 + ... + w_(-19+i)) / sum).
+static void gauss_thresholds(uint64_t *t) {
+    const double sigma = 3.2;
+    double w[2 * kGaussCut + 1], z = 0.0;
+    for (int k = -kGaussCut; k <= kGaussCut; ++k) {
+        w[k + kGaussCut] = std::exp(-(double)(k * k) / (2.0 * sigma * sigma));
+        z += w[k + kGaussCut];
+    }
+    double cum = 0.0;
+    for (int i = 0; i < kGaussThresholds; ++i) {
+        cum += w[i];
+        t[i] = (uint64_t)std::ldexp(cum / z, 64);
+    }
+}
+
+// The sampling launch's chunks (SampleChunk) for a request list; the
+// cumulative Gaussian thresholds follow them in the same upload.
+static int sample_chunks(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs,
+                         std::vector<SampleChunk> &chunks) {
+    chunks.clear();
+    for (uint32_t r = 0; r < n_reqs; ++r) {
+        const ph_rlwe_request &q = reqs[r];
+        if (q.log_n < c->min_log_n || q.log_n > c->max_log_n || q.limbs == 0 || q.limbs > c->max_limbs)
+            return fail(PH_ERR_INVALID, "request outside the context's ring degrees or limb count");
+        const uint32_t n = 1u << q.log_n;
+        const uint32_t base = (q.log_n - c->min_log_n) * c->max_limbs;
+        for (uint32_t k0 = 0; k0 < n; k0 += kSampleChunk) {
+            const uint32_t cnt = std::min(kSampleChunk, n - k0);
+            for (uint32_t p : {(uint32_t)kSampleSecret, (uint32_t)kSampleError, (uint32_t)kSampleMessage})
+                chunks.push_back(SampleChunk{q.small_offset + k0, 0, r, p, k0, cnt});
+            for (uint32_t j = 0; j < q.limbs; ++j)
+                chunks.push_back(SampleChunk{q.a_offset + (uint64_t)j * n + k0, c->tables[base + j]->q, r,
+                                             (uint32_t)kSampleA + j, k0, cnt});
+        }
+    }
+    return PH_OK;
+}
+
+static hipError_t sample_launch(const char *d_mem, uint32_t n_chunks, uint64_t seed, uint64_t *d_a, int8_t *d_s,
+                                int8_t *d_e, uint8_t *d_m, hipStream_t s) {
+    SampleArgs a{};
+    a.chunks = (const SampleChunk *)d_mem;
+    a.n_chunks = n_chunks;
+    a.seed_lo = (uint32_t)seed;
+    a.seed_hi = (uint32_t)(seed >> 32);
+    a.gauss = (const uint64_t *)(d_mem + (size_t)n_chunks * sizeof(SampleChunk));
+    a.a = d_a;
+    a.s = d_s;
+    a.e = d_e;
+    a.m = d_m;
+    return launch_rlwe_sample(a, s);
+}
+
+int ph_rlwe_sample_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs, uint64_t seed,
+                          uint64_t *d_a, int8_t *d_s, int8_t *d_e, uint8_t *d_m, void *stream) {
+    if (!c || (!reqs && n_reqs)) return fail(PH_ERR_INVALID, "null argument");
+    if (n_reqs == 0) return PH_OK;
+    if (!d_a || !d_s || !d_e || !d_m) return fail(PH_ERR_INVALID, "null device buffer");
+    std::vector<SampleChunk> chunks;
+    int rc = sample_chunks(c, reqs, n_reqs, chunks);
+    if (rc != PH_OK) return rc;
+    uint64_t gauss[kGaussThresholds];
+    gauss_thresholds(gauss);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t cb = chunks.size() * sizeof(SampleChunk);
+    char *d_mem = nullptr;
+    PH_HIP(hipMallocAsync((void **)&d_mem, cb + sizeof(gauss), s));
+    PH_HIP(hipMemcpyAsync(d_mem, chunks.data(), cb, hipMemcpyHostToDevice, s));
+    PH_HIP(hipMemcpyAsync(d_mem + cb, gauss, sizeof(gauss), hipMemcpyHostToDevice, s));
+    PH_HIP(hipStreamSynchronize(s));  // the host chunk list may go away after this
+    hipError_t e = sample_launch(d_mem, (uint32_t)chunks.size(), seed, d_a, d_s, d_e, d_m, s);
+    (void)hipFreeAsync(d_mem, s);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE sampling kernel");
+}
+
 struct ph_rlwe_plan {
     const ph_rlwe_ctx *ctx = nullptr;
     RlweItem *d_items = nullptr;
@@ -1183,12 +1261,15 @@ struct ph_rlwe_plan {
     uint32_t grid = 0, max_grid = 0;
     uint64_t *scratch = nullptr;
     uint64_t scratch_words = 0;
+    char *d_sample = nullptr;  // the sampling launch's chunks, then the Gaussian thresholds
+    uint32_t n_chunks = 0;
 };
 
 void ph_rlwe_plan_free(ph_rlwe_plan *p) {
     if (!p) return;
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->scratch) (void)hipFree(p->scratch);
+    if (p->d_sample) (void)hipFree(p->d_sample);
     delete p;
 }
 
@@ -1212,12 +1293,29 @@ int ph_rlwe_plan_new(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t
     if (rc == PH_OK && p->scratch_words && hipMalloc((void **)&p->scratch, (size_t)p->max_grid * p->scratch_words * 8) !=
                                                hipSuccess)
         rc = fail(PH_ERR_NO_MEMORY, "RLWE scratch");
+    std::vector<SampleChunk> chunks;
+    if (rc == PH_OK) rc = sample_chunks(c, reqs, n_reqs, chunks);
+    if (rc == PH_OK) {
+        std::vector<char> img(chunks.size() * sizeof(SampleChunk) + kGaussThresholds * sizeof(uint64_t));
+        std::memcpy(img.data(), chunks.data(), chunks.size() * sizeof(SampleChunk));
+        gauss_thresholds((uint64_t *)(img.data() + chunks.size() * sizeof(SampleChunk)));
+        p->n_chunks = (uint32_t)chunks.size();
+        rc = upload((void **)&p->d_sample, img.data(), img.size());
+    }
     if (rc != PH_OK) {
         ph_rlwe_plan_free(p);
         return rc;
     }
     *out = p;
     return PH_OK;
+}
+
+int ph_rlwe_sample_plan_device(const ph_rlwe_plan *p, uint64_t seed, uint64_t *d_a, int8_t *d_s, int8_t *d_e,
+                               uint8_t *d_m, void *stream) {
+    if (!p) return fail(PH_ERR_INVALID, "null plan");
+    if (!d_a || !d_s || !d_e || !d_m) return fail(PH_ERR_INVALID, "null device buffer");
+    hipError_t e = sample_launch(p->d_sample, p->n_chunks, seed, d_a, d_s, d_e, d_m, (hipStream_t)stream);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE sampling kernel");
 }
 
 int ph_rlwe_encrypt_plan_device(const ph_rlwe_plan *p, const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e,

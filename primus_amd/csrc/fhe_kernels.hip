@@ -329,6 +329,37 @@ __global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs 
     }
 }
 
+// Inputs of RLWE encryption drawn on the device (cfg 3's distributions): s
+// ternary uniform, e the rounded Gaussian of sigma 3.2 cut at 19, m uniform
+// in [0, 256), a_hat uniform mod q_i.  Each element's values come from the
+// Philox block of counter (element, request, purpose, 0).
+__global__ __launch_bounds__(256) void rlwe_sample_kernel(SampleArgs a) {
+    const SampleChunk ch = a.chunks[blockIdx.x];
+    uint64_t gauss[kGaussThresholds];
+    if (ch.purpose == kSampleError) {
+#pragma unroll
+        for (int i = 0; i < kGaussThresholds; ++i) gauss[i] = a.gauss[i];
+    }
+    for (uint32_t i = threadIdx.x; i < ch.count; i += 256) {
+        uint32_t c[4] = {ch.k0 + i, ch.request, ch.purpose, 0u};
+        philox4x32_10(c, a.seed_lo, a.seed_hi);
+        const uint64_t lo = (uint64_t)c[0] | ((uint64_t)c[1] << 32);
+        const uint64_t hi = (uint64_t)c[2] | ((uint64_t)c[3] << 32);
+        switch (ch.purpose) {
+            case kSampleSecret: a.s[ch.dst + i] = (int8_t)sample_ternary(lo); break;
+            case kSampleError: a.e[ch.dst + i] = (int8_t)sample_gauss(lo, gauss); break;
+            case kSampleMessage: a.m[ch.dst + i] = (uint8_t)(c[0] & 255u); break;
+            default: a.a[ch.dst + i] = sample_mod(lo, hi, ch.q); break;
+        }
+    }
+}
+
+hipError_t launch_rlwe_sample(const SampleArgs &a, hipStream_t st) {
+    if (a.n_chunks == 0) return hipSuccess;
+    rlwe_sample_kernel<<<dim3(a.n_chunks), dim3(256), 0, st>>>(a);
+    return hipGetLastError();
+}
+
 hipError_t launch_rlwe_encrypt(const RlweArgs &a, uint32_t grid, hipStream_t s) {
     if (a.n_items == 0 || grid == 0) return hipSuccess;
     rlwe_encrypt_kernel<<<dim3(grid), dim3(kRlweThreads), 0, s>>>(a);

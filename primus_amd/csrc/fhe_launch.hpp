@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "ntt_device.hpp"
+#include "sample.hpp"
 
 namespace ph {
 
@@ -63,5 +64,30 @@ struct RlweArgs {
 };
 
 hipError_t launch_rlwe_encrypt(const RlweArgs &a, uint32_t grid, hipStream_t s);
+
+// Sampling of RLWE encryption inputs (sample.hpp): one workgroup per chunk of
+// up to kSampleChunk consecutive elements of one request's row for one
+// purpose.  dst: element offset in s / e / m, or word offset in a_hat.
+constexpr uint32_t kSampleChunk = 4096;
+struct SampleChunk {
+    uint64_t dst;
+    uint64_t q;          // the row's modulus (a_hat rows)
+    uint32_t request;    // counter word 1
+    uint32_t purpose;    // counter word 2: kSampleSecret, kSampleError, kSampleMessage, kSampleA + limb
+    uint32_t k0, count;  // first element (counter word 0) and element count
+};
+
+struct SampleArgs {
+    const SampleChunk *chunks;
+    uint32_t n_chunks;
+    uint32_t seed_lo, seed_hi;  // Philox key
+    const uint64_t *gauss;      // kGaussThresholds cumulative thresholds
+    uint64_t *a;
+    int8_t *s;
+    int8_t *e;
+    uint8_t *m;
+};
+
+hipError_t launch_rlwe_sample(const SampleArgs &a, hipStream_t st);
 
 }  // namespace ph

@@ -91,6 +91,12 @@ class RlweContext:
         _check(lib().ph_rlwe_encrypt_device(self._h, ctypes.cast(arr, ctypes.c_void_p), len(requests), d_a, d_s,
                                             d_e, d_m, d_out, _stream(stream)))
 
+    def sample_device(self, requests, seed, d_a, d_s, d_e, d_m, stream=None):
+        """Draws s, e, m and a_hat for the requests on the device (ph_rlwe_sample_device)."""
+        arr = (RlweRequest * len(requests))(*[RlweRequest(*r) for r in requests])
+        _check(lib().ph_rlwe_sample_device(self._h, ctypes.cast(arr, ctypes.c_void_p), len(requests), seed, d_a,
+                                           d_s, d_e, d_m, _stream(stream)))
+
     def plan(self, requests):
         """A prepared launch for this batch layout (RlwePlan)."""
         return RlwePlan(self, requests)
@@ -112,6 +118,10 @@ class RlwePlan:
 
     def encrypt_device(self, d_a, d_s, d_e, d_m, d_out, stream=None):
         _check(lib().ph_rlwe_encrypt_plan_device(self._h, d_a, d_s, d_e, d_m, d_out, _stream(stream)))
+
+    def sample_device(self, seed, d_a, d_s, d_e, d_m, stream=None):
+        """Draws the plan's s, e, m and a_hat on the device (ph_rlwe_sample_plan_device)."""
+        _check(lib().ph_rlwe_sample_plan_device(self._h, seed, d_a, d_s, d_e, d_m, _stream(stream)))
 
     def autotune(self, d_a, d_s, d_e, d_m, d_out, reps=5):
         """Times persistent grids of 1..4 workgroups per CU; returns (grid, ms)."""

@@ -140,3 +140,38 @@ def test_graph_ext_product():
     with torch.cuda.graph(g, stream=stream):
         key.product_device(_vp(din), _vp(dout), count, stream=stream.cuda_stream)
     _replay_and_check(torch, g, stream, [dout], [want])
+
+
+def test_graph_rlwe_sample_and_encrypt():
+    """Device sampling and encryption of an uneven batch, both from one plan,
+    captured together and replayed."""
+    torch = _torch()
+    import primus_amd as pa
+    from primus_amd.fhe import RlweContext, rlwe_layout
+
+    shapes = [(10, 2), (12, 3), (14, 1), (11, 4), (13, 2)]
+    grid = {ln: pa.ntt_primes_below(59, 2 << ln, 4) for ln in range(10, 15)}
+    ctx = RlweContext(10, 14, 4, grid)
+    reqs, a_words, small = rlwe_layout(shapes)
+    plan = ctx.plan(reqs)
+    dev = torch.device("cuda:0")
+    da = torch.zeros(a_words, dtype=torch.int64, device=dev)
+    dout = torch.zeros_like(da)
+    ds, de = (torch.zeros(small, dtype=torch.int8, device=dev) for _ in range(2))
+    dm = torch.zeros(small, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    seed = 0xFEEDF00D
+    plan.sample_device(seed, _vp(da), _vp(ds), _vp(de), _vp(dm), stream=stream.cuda_stream)
+    plan.encrypt_device(_vp(da), _vp(ds), _vp(de), _vp(dm), _vp(dout), stream=stream.cuda_stream)
+    stream.synchronize()
+    want = [t.cpu().numpy().copy() for t in (da, dout)]
+    ln, lb, ao, so = reqs[1]
+    mods = grid[ln][:lb]
+    s_, e_, m_, a_ = O.rlwe_sample(seed, 1, ln, mods)
+    np.testing.assert_array_equal(want[0].view(np.uint64)[ao:ao + lb * (1 << ln)], a_.ravel())
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=stream):
+        plan.sample_device(seed, _vp(da), _vp(ds), _vp(de), _vp(dm), stream=stream.cuda_stream)
+        plan.encrypt_device(_vp(da), _vp(ds), _vp(de), _vp(dm), _vp(dout), stream=stream.cuda_stream)
+    _replay_and_check(torch, g, stream, [da, dout], [w.view(np.uint64) for w in want])
