@@ -329,10 +329,11 @@ __global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs 
     }
 }
 
-// Inputs of RLWE encryption drawn on the device (cfg 3's distributions): s
-// ternary uniform, e the rounded Gaussian of sigma 3.2 cut at 19, m uniform
-// in [0, 256), a_hat uniform mod q_i.  Each element's values come from the
-// Philox block of counter (element, request, purpose, 0).
+// Inputs of RLWE encryption drawn on the device, with the distributions of
+// the uneven encryption batch: s ternary uniform, e the rounded Gaussian of
+// sigma 3.2 cut at 19, m uniform in [0, 256), a_hat uniform mod q_i.  Each
+// element's values come from the Philox block of counter (element, request,
+// purpose, 0).
 __global__ __launch_bounds__(256) void rlwe_sample_kernel(SampleArgs a) {
     const SampleChunk ch = a.chunks[blockIdx.x];
     uint64_t gauss[kGaussThresholds];
