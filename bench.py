@@ -424,10 +424,11 @@ def serving_main(args):
                 "throughput_per_s": count / elapsed, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_ms": float(np.percentile(lat, 99) * 1e3), "batches": after["batches"] - before["batches"],
                 "launches": after["launches"] - before["launches"],
-                "peak_inflight_bytes": after["peak_inflight_bytes"]}
+                "peak_inflight_bytes": after["peak_inflight_bytes"],
+                "host_s": {k: after[k] - before[k] for k in ("submit_s", "dispatch_s", "slot_wait_s", "device_wait_s")}}
 
     batched = sv.Server(grid, 10, 16, 16, ext_key=key, hbm_budget_bytes=budget, max_batch=args.serving_max_batch,
-                        streams=4)
+                        streams=args.serving_streams)
     plain = sv.Server(grid, 10, 16, 16, ext_key=key, hbm_budget_bytes=budget, max_batch=1, streams=1,
                       per_poly_launches=True)
     # parity gate: the first requests of each kind against the oracle, through both servers
@@ -474,7 +475,8 @@ def serving_main(args):
                        "host buffers in and out (PCIe included; " +
                        ("pageable, staged by the server" if args.serving_pageable else "page-locked, copied directly") +
                        "); 64 GiB HBM admission budget",
-           "pool_distinct_requests": len(pool), "max_batch": args.serving_max_batch, "host_link": link,
+           "pool_distinct_requests": len(pool), "max_batch": args.serving_max_batch,
+           "stream_slots": args.serving_streams, "host_link": link,
            "mean_request_bytes": {"h2d": float(np.mean([request_bytes(r)[0] for r in pool])),
                                   "d2h": float(np.mean([request_bytes(r)[1] for r in pool]))},
            "rates": []}
@@ -519,6 +521,8 @@ def main():
     ap.add_argument("--serving-plain-requests", type=int, default=3000)
     ap.add_argument("--serving-pool", type=int, default=256)
     ap.add_argument("--serving-max-batch", type=int, default=256)
+    ap.add_argument("--serving-streams", type=int, default=4,
+                    help="stream slots of the batched server (batches in flight)")
     ap.add_argument("--serving-pageable", action="store_true",
                     help="requests in ordinary (pageable) host memory, staged by the server")
     args = ap.parse_args()
@@ -528,6 +532,11 @@ def main():
         return 2
 
     if args.serving:
+        # one hardware queue per server stream (slots, input copies, result
+        # copies): HIP's default of 4 makes streams share queues, and a copy
+        # then waits behind another stream's event wait.  Read when HIP starts,
+        # which has not happened yet in this process.
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(8, args.serving_streams + 4)))
         return serving_main(args)
 
     import torch

@@ -171,6 +171,14 @@ int ph_dcrt_fwd_mul_inv_device(const ph_dcrt_table *table, uint64_t *d_polys, si
  * (primus_poly/src/poly/mul.rs:111-142). */
 int ph_dcrt_negacyclic_mul_device(const ph_dcrt_table *table, uint64_t *d_a, const uint64_t *d_b,
                                   size_t count, void *stream);
+/* The same product over limb-polynomials of uneven limb counts (no reference
+ * counterpart: a batched form of the call above): row i of d_a and d_b is
+ * reduced by the table's modulus d_limb_of[i] (< the table's limb count)
+ * instead of i % L, so requests of one ring degree and any limb counts share
+ * one launch.  d_limb_of is device memory, one byte per row; a row whose
+ * index is L or more is left unchanged. */
+int ph_dcrt_negacyclic_mul_rows_device(const ph_dcrt_table *table, uint64_t *d_a, const uint64_t *d_b,
+                                       const uint8_t *d_limb_of, size_t count, void *stream);
 /* NttPolynomial::mul_inplace (primus_poly/src/ntt/mul.rs:105-119): d_out = d_a * d_b. */
 int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *table, const uint64_t *d_a, const uint64_t *d_b,
                                  uint64_t *d_out, size_t count, void *stream);
@@ -325,7 +333,12 @@ typedef struct {
 typedef struct {
     uint64_t hbm_budget_bytes; /* admission limit on the device memory of in-flight requests */
     uint32_t max_batch;        /* requests per batch; 1 = the plain one-launch-per-request path */
-    uint32_t streams;          /* batches in flight at once */
+    uint32_t streams;          /* batches in flight at once: one stream each, plus one stream
+                                * for all input copies and one for all result copies.  Each
+                                * wants a hardware queue of its own (HIP's
+                                * GPU_MAX_HW_QUEUES >= streams + 2, set before HIP starts):
+                                * streams sharing a queue wait behind each other's event
+                                * waits. */
     uint32_t min_log_n, max_log_n, max_limbs;
     const uint64_t *moduli;    /* [(max_log_n - min_log_n + 1)][max_limbs] limb moduli per ring degree */
     const ph_ext_key *ext_key; /* key of the external-product requests (NULL: none accepted) */
@@ -335,6 +348,10 @@ typedef struct {
 
 typedef struct {
     uint64_t completed, batches, launches, peak_inflight_bytes;
+    /* host seconds, summed: inside ph_server_submit; the dispatcher enqueueing
+     * batches, and waiting for a free slot; the completion thread waiting for
+     * batches to finish on the device */
+    double submit_s, dispatch_s, slot_wait_s, device_wait_s;
 } ph_server_stats;
 
 typedef struct ph_server ph_server;

@@ -140,6 +140,7 @@ def _declare(L):
         "ph_operand_free": (None, [vp]),
         "ph_dcrt_fwd_mul_inv_device": (i, [vp, vp, sz, vp, vp]),
         "ph_dcrt_negacyclic_mul_device": (i, [vp, vp, vp, sz, vp]),
+        "ph_dcrt_negacyclic_mul_rows_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_pointwise_mul_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_pointwise_mul_add_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_automorphism_device": (i, [vp, vp, vp, sz, u64, vp]),

@@ -19,6 +19,7 @@
 
 #include "../../include/primus_hip.h"
 #include "arith.hpp"
+#include "capi_internal.hpp"
 #include "fhe_launch.hpp"
 #include "ntt_launch.hpp"
 #include "rns_launch.hpp"
@@ -366,10 +367,15 @@ int tuning_of(uint32_t log_n, Op op, bool f64) {
     return g_tuning[log_n][(int)op][f64 ? 1 : 0].load(std::memory_order_relaxed);
 }
 
+// limb_of: per limb-polynomial table index (poly-multiply), or null for i % L.
+// caller_scratch: device memory of scratch_bytes(op, log_n, count) or more to
+// use instead of an allocation, or null.
 int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, uint64_t *d_data,
-           const uint64_t *d_data2, const ph_operand *operand, uint64_t count, hipStream_t s) {
+           const uint64_t *d_data2, const ph_operand *operand, uint64_t count, hipStream_t s,
+           const uint8_t *limb_of = nullptr, void *caller_scratch = nullptr) {
     if (count == 0) return PH_OK;
     KArgs a{};
+    a.limb_of = limb_of;
     a.data = d_data;
     a.data2 = d_data2;
     a.operand = operand ? operand->d : nullptr;
@@ -378,10 +384,11 @@ int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, 
     if (op == Op::FwdMulInv && f64 && (!a.operand_f[0] || !a.operand_f[1])) f64 = false;
     a.tabs = d_tabs;
     a.L = L;
-    void *scratch = nullptr;
-    if (needs_scratch(op, log_n)) PH_HIP(hipMallocAsync(&scratch, (count << log_n) * 8, s));
+    void *scratch = caller_scratch;
+    const bool own = needs_scratch(op, log_n) && !scratch;
+    if (own) PH_HIP(hipMallocAsync(&scratch, (count << log_n) * 8, s));
     hipError_t e = launch_transform(op, a, log_n, count, scratch, f64, tuning_of(log_n, op, f64), s);
-    if (scratch) (void)hipFreeAsync(scratch, s);
+    if (own) (void)hipFreeAsync(scratch, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return PH_OK;
 }
@@ -689,6 +696,13 @@ int ph_dcrt_negacyclic_mul_device(const ph_dcrt_table *d, uint64_t *d_a, const u
                                   void *stream) {
     if (!d || ((!d_a || !d_b) && count)) return fail(PH_ERR_INVALID, "null argument");
     return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, (hipStream_t)stream);
+}
+
+int ph_dcrt_negacyclic_mul_rows_device(const ph_dcrt_table *d, uint64_t *d_a, const uint64_t *d_b,
+                                       const uint8_t *d_limb_of, size_t count, void *stream) {
+    if (!d || ((!d_a || !d_b || !d_limb_of) && count)) return fail(PH_ERR_INVALID, "null argument");
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, (hipStream_t)stream,
+                  d_limb_of);
 }
 
 int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *d, const uint64_t *d_a, const uint64_t *d_b,
@@ -1141,6 +1155,33 @@ static int rlwe_items(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_
     return PH_OK;
 }
 
+// Persistent grid of an RLWE launch over n_items items: two resident
+// workgroups per compute unit (the kernel's occupancy), fewer for a short list.
+static uint32_t rlwe_grid(const ph_rlwe_ctx *c, uint64_t n_items) {
+    return (uint32_t)std::min<uint64_t>(n_items, (uint64_t)std::max(c->cus, 1) * 2);
+}
+
+// Scratch words per workgroup: only the two-pass ring degrees use scratch,
+// one polynomial of the largest ring degree.
+static uint64_t rlwe_scratch_words(uint32_t max_log_n) { return max_log_n > 12 ? 1ull << max_log_n : 0; }
+
+static hipError_t rlwe_launch(const ph_rlwe_ctx *c, const RlweItem *d_items, uint64_t n_items, uint32_t max_log_n,
+                              const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m,
+                              uint64_t *d_out, uint64_t *scratch, hipStream_t s) {
+    RlweArgs a{};
+    a.items = d_items;
+    a.n_items = (uint32_t)n_items;
+    a.tabs = c->d_tabs;
+    a.a = d_a;
+    a.out = d_out;
+    a.s = d_s;
+    a.e = d_e;
+    a.m = d_m;
+    a.scratch = scratch;
+    a.scratch_words = rlwe_scratch_words(max_log_n);
+    return launch_rlwe_encrypt(a, rlwe_grid(c, n_items), s);
+}
+
 int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs, const uint64_t *d_a,
                            const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m, uint64_t *d_out, void *stream) {
     if (!c || (!reqs && n_reqs)) return fail(PH_ERR_INVALID, "null argument");
@@ -1150,28 +1191,16 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     std::vector<RlweItem> items;
     int rc = rlwe_items(c, reqs, n_reqs, items);
     if (rc != PH_OK) return rc;
-    // persistent: two resident workgroups per compute unit (the kernel's occupancy)
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(items.size(), (uint64_t)std::max(c->cus, 1) * 2);
     RlweItem *d_items = nullptr;
     uint64_t *scratch = nullptr;
     PH_HIP(hipMallocAsync((void **)&d_items, items.size() * sizeof(RlweItem), s));
     PH_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(RlweItem), hipMemcpyHostToDevice, s));
-    // only the two-pass ring degrees use scratch: one largest ring degree per workgroup
-    const uint64_t scratch_words = items.front().log_n > 12 ? 1ull << items.front().log_n : 0;
-    if (scratch_words) PH_HIP(hipMallocAsync((void **)&scratch, (size_t)grid * scratch_words * 8, s));
+    const uint32_t max_log_n = items.front().log_n;
+    const uint64_t scratch_words = rlwe_scratch_words(max_log_n);
+    if (scratch_words)
+        PH_HIP(hipMallocAsync((void **)&scratch, (size_t)rlwe_grid(c, items.size()) * scratch_words * 8, s));
     PH_HIP(hipStreamSynchronize(s));  // the host item list may go away after this
-    RlweArgs a{};
-    a.items = d_items;
-    a.n_items = (uint32_t)items.size();
-    a.tabs = c->d_tabs;
-    a.a = d_a;
-    a.out = d_out;
-    a.s = d_s;
-    a.e = d_e;
-    a.m = d_m;
-    a.scratch = scratch;
-    a.scratch_words = scratch_words;
-    hipError_t e = launch_rlwe_encrypt(a, grid, s);
+    hipError_t e = rlwe_launch(c, d_items, items.size(), max_log_n, d_a, d_s, d_e, d_m, d_out, scratch, s);
     if (scratch) (void)hipFreeAsync(scratch, s);
     (void)hipFreeAsync(d_items, s);
     return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
@@ -1530,3 +1559,55 @@ done:
 }
 
 }  // extern "C"
+
+// Library-internal forms (capi_internal.hpp), C++ linkage.
+namespace ph {
+
+RlweStageSize rlwe_stage_size(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs) {
+    RlweStageSize z{0, 0};
+    if (!c || (!reqs && n_reqs)) return z;
+    uint64_t n_items = 0;
+    uint32_t max_log_n = 0;
+    for (uint32_t r = 0; r < n_reqs; ++r) {
+        const ph_rlwe_request &q = reqs[r];
+        if (q.log_n < c->min_log_n || q.log_n > c->max_log_n || q.limbs == 0 || q.limbs > c->max_limbs) return z;
+        n_items += q.limbs;
+        max_log_n = std::max(max_log_n, q.log_n);
+    }
+    z.item_bytes = n_items * sizeof(RlweItem);
+    z.scratch_bytes = rlwe_grid(c, n_items) * rlwe_scratch_words(max_log_n) * 8;
+    return z;
+}
+
+int rlwe_encrypt_staged(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs, const uint64_t *d_a,
+                        const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m, uint64_t *d_out, hipStream_t s,
+                        void *host_items, void *d_items, void *d_scratch, const RlweStageSize &capacity) {
+    if (!c || (!reqs && n_reqs)) return fail(PH_ERR_INVALID, "null argument");
+    if (n_reqs == 0) return PH_OK;
+    if (!d_a || !d_s || !d_e || !d_m || !d_out || !host_items || !d_items) return fail(PH_ERR_INVALID, "null buffer");
+    std::vector<RlweItem> items;
+    int rc = rlwe_items(c, reqs, n_reqs, items);
+    if (rc != PH_OK) return rc;
+    const RlweStageSize need = rlwe_stage_size(c, reqs, n_reqs);
+    if (need.item_bytes > capacity.item_bytes || need.scratch_bytes > capacity.scratch_bytes ||
+        (need.scratch_bytes && !d_scratch))
+        return fail(PH_ERR_INVALID, "staging buffers smaller than the request list needs");
+    std::memcpy(host_items, items.data(), need.item_bytes);
+    PH_HIP(hipMemcpyAsync(d_items, host_items, need.item_bytes, hipMemcpyHostToDevice, s));
+    hipError_t e = rlwe_launch(c, (const RlweItem *)d_items, items.size(), items.front().log_n, d_a, d_s, d_e, d_m,
+                               d_out, need.scratch_bytes ? (uint64_t *)d_scratch : nullptr, s);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
+}
+
+uint64_t negacyclic_mul_scratch_bytes(const ph_dcrt_table *d, uint64_t count) {
+    return d && needs_scratch(Op::PolyMul, d->log_n) ? (count << d->log_n) * 8 : 0;
+}
+
+int negacyclic_mul_rows_staged(const ph_dcrt_table *d, uint64_t *d_a, const uint64_t *d_b, const uint8_t *d_limb_of,
+                               uint64_t count, void *d_scratch, hipStream_t s) {
+    if (!d || ((!d_a || !d_b || !d_limb_of) && count)) return fail(PH_ERR_INVALID, "null argument");
+    if (negacyclic_mul_scratch_bytes(d, count) && !d_scratch) return fail(PH_ERR_INVALID, "null scratch");
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, s, d_limb_of, d_scratch);
+}
+
+}  // namespace ph

@@ -133,7 +133,13 @@ __attribute__((amdgpu_waves_per_eu(1, Occupancy<LOGC, LOGT, MODE>::value))) void
     const uint64_t blk = blockIdx.x;
     const uint64_t poly = blk / a.blocks_per_poly;
     const uint32_t b = (uint32_t)(blk - poly * a.blocks_per_poly);
-    const uint32_t limb = (uint32_t)(poly % a.L);
+    uint32_t limb;
+    if constexpr ((MODE & M_MUL_POLY) != 0) {
+        limb = a.limb_of ? a.limb_of[poly] : (uint32_t)(poly % a.L);
+        if (limb >= a.L) return;  // a row with no table is left as it is (whole workgroup)
+    } else {
+        limb = (uint32_t)(poly % a.L);
+    }
     const DevTable tab = a.tabs[limb];
     const uint32_t n = 1u << tab.log_n;
     const uint32_t st0 = tab.log_n - LOGC;
@@ -227,7 +233,8 @@ __global__ __launch_bounds__(256) void column_kernel(KArgs a) {
     const uint32_t logc = a.tabs[0].log_n - LOGR;
     const uint64_t poly = gid >> logc;
     const uint32_t j = (uint32_t)(gid & ((1u << logc) - 1u));
-    const uint32_t limb = (uint32_t)(poly % a.L);
+    const uint32_t limb = a.limb_of ? a.limb_of[poly] : (uint32_t)(poly % a.L);
+    if (limb >= a.L) return;  // a row with no table is left as it is (no barriers here)
     const DevTable tab = a.tabs[limb];
     const uint32_t n = 1u << tab.log_n;
     const uint64_t off = poly * n + j;

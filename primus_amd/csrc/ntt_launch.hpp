@@ -21,6 +21,8 @@ struct KArgs {
     const ShoupF *operand_f[2]; // the same operand as centred binary64 pairs (q < 2^50), in the
                                 // register order of 8 / 16 coefficients per thread
     const DevTable *tabs;    // L device tables; limb-polynomial i uses tabs[i % L]
+    const uint8_t *limb_of;  // when set (poly-multiply only): limb-polynomial i uses
+                             // tabs[limb_of[i]], for batches of uneven limb counts
     uint32_t L;
     uint32_t blocks_per_poly;
     uint64_t count;          // limb-polynomials of the launch (persistent kernels)

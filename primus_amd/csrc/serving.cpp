@@ -2,14 +2,17 @@
 //
 // A dispatcher thread turns the pending queue into batches: it admits
 // requests in arrival order while their device footprint fits the HBM budget
-// and a stream slot is free, groups them by kind and shape (one launch per
-// poly-multiply shape, one persistent launch for all RLWE encryptions, one for
-// all external products), stages inputs through pinned memory and enqueues
-// copies and kernels on the slot's stream.  A completion thread retires
-// batches in order, copies results to the callers' buffers and frees the
-// budget.  Requests that arrive while batches are on the GPU join the next
-// batch as soon as a slot and memory free up; batches on different slots
-// overlap on the device.  Tables and keys are built once and stay resident.
+// and a stream slot is free, groups them by kind and ring degree (one launch
+// per poly-multiply ring degree whatever the limb counts, one persistent
+// launch for all RLWE encryptions, one for all external products), and
+// enqueues the batch as three stages: input copies on the server's
+// host-to-device stream, kernels on the slot's stream, result copies on the
+// device-to-host stream, handed over by events.  A completion thread retires
+// batches in order, copies staged results to the callers' buffers and frees
+// the budget.  Requests that arrive while batches are on the GPU join the
+// next batch as soon as a slot and memory free up; one batch's compute
+// overlaps the next batch's inputs and the previous batch's results.  Tables
+// and keys are built once and stay resident.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +31,7 @@
 #include <vector>
 
 #include "../../include/primus_hip.h"
+#include "capi_internal.hpp"
 #include "copy_launch.hpp"
 
 namespace {
@@ -61,11 +65,18 @@ struct Batch {
 };
 
 struct Slot {
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the batch's compute, allocation and completion
+    // stage hand-offs: arena allocated -> inputs copied -> compute done ->
+    // results copied (a slot holds one batch at a time, so they are reused)
+    hipEvent_t ev_alloc = nullptr, ev_in = nullptr, ev_comp = nullptr, ev_out = nullptr;
     void *pinned = nullptr;
     uint64_t pinned_cap = 0;
     ph::CopySpan *spans = nullptr;  // page-locked: read by the copy kernels
     uint64_t spans_cap = 0;
+    void *items = nullptr;  // page-locked: the RLWE launch's item list
+    uint64_t items_cap = 0;
+    uint8_t *maps = nullptr;  // page-locked: the products' row -> limb map
+    uint64_t maps_cap = 0;
     bool busy = false;
 };
 
@@ -165,6 +176,14 @@ struct ph_server {
     std::map<std::pair<uint32_t, uint32_t>, ph_dcrt_table *> single;  // (log_n, limb) -> one-limb view
     ph_rlwe_ctx *rlwe = nullptr;
     uint32_t rlwe_min = 0, rlwe_max = 0, rlwe_limbs = 0;
+    // batch arenas: a pool of the server's own that keeps up to the HBM budget
+    // mapped between batches instead of returning it at every synchronisation
+    hipMemPool_t pool = nullptr;
+    // every batch's input copies run on h2d and its result copies on d2h, in
+    // batch order: each direction of the host link carries one copy at a time
+    // at its full rate, and the two directions overlap each other and the
+    // slots' compute
+    hipStream_t h2d = nullptr, d2h = nullptr;
     std::vector<Slot> slots;
 
     std::mutex mu;
@@ -221,7 +240,7 @@ uint64_t request_bytes(const ph_serve_request &r) {
     const uint64_t n = 1ull << r.log_n;
     switch (r.kind) {
         case PH_REQ_POLY_MUL: return 3 * round_up(r.limbs * n * 8);
-        case PH_REQ_RLWE_ENCRYPT: return 2 * round_up(r.limbs * n * 8) + 3 * round_up(n);
+        case PH_REQ_RLWE_ENCRYPT: return 2 * round_up(r.limbs * n * 8) + 3 * round_up(n) + 64ull * r.limbs;
         case PH_REQ_EXT_PRODUCT: return 2 * round_up(2 * n * 8);
     }
     return 0;
@@ -252,9 +271,12 @@ int validate(const ph_server *s, const ph_serve_request &r) {
 // per compute unit) when log_n > 12, none otherwise.
 uint64_t rlwe_scratch_bytes(uint32_t log_n) {
     if (log_n <= 12) return 0;
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) cus = 256;
-    return (uint64_t)std::max(cus, 1) * 2 * (1ull << log_n) * 8;
+    static const int cus = [] {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) n = 256;
+        return std::max(n, 1);
+    }();
+    return (uint64_t)cus * 2 * (1ull << log_n) * 8;
 }
 
 int ensure_spans(Slot &sl, uint64_t count) {
@@ -275,6 +297,28 @@ void add_spans(std::vector<ph::CopySpan> &v, uint64_t src, uint64_t dst, uint64_
         v.push_back({src + o, dst + o, std::min<uint64_t>(ph::kCopySpanMax, bytes - o)});
 }
 
+int ensure_items(Slot &sl, uint64_t bytes) {
+    if (bytes <= sl.items_cap) return PH_OK;
+    if (sl.items) (void)hipHostFree(sl.items);
+    sl.items = nullptr;
+    sl.items_cap = 0;
+    const uint64_t cap = std::max<uint64_t>(bytes, 64 << 10);
+    if (hipHostMalloc(&sl.items, cap, hipHostMallocDefault) != hipSuccess) return PH_ERR_NO_MEMORY;
+    sl.items_cap = cap;
+    return PH_OK;
+}
+
+int ensure_maps(Slot &sl, uint64_t bytes) {
+    if (bytes <= sl.maps_cap) return PH_OK;
+    if (sl.maps) (void)hipHostFree(sl.maps);
+    sl.maps = nullptr;
+    sl.maps_cap = 0;
+    const uint64_t cap = std::max<uint64_t>(bytes, 16 << 10);
+    if (hipHostMalloc((void **)&sl.maps, cap, hipHostMallocDefault) != hipSuccess) return PH_ERR_NO_MEMORY;
+    sl.maps_cap = cap;
+    return PH_OK;
+}
+
 int ensure_pinned(Slot &sl, uint64_t bytes) {
     if (sl.pinned_cap >= bytes) return PH_OK;
     if (sl.pinned) (void)hipHostFree(sl.pinned);
@@ -289,11 +333,14 @@ int ensure_pinned(Slot &sl, uint64_t bytes) {
 // Enqueues one batch on its slot's stream.  Called by the dispatcher without
 // the lock held; the batch's tickets are not touched by anyone else meanwhile.
 //
-// Arena layout (staging mirrors it byte for byte): one [count][limbs][N] block
-// pair per poly-multiply shape (a, then b; the product replaces a), the RLWE
-// requests' a_hat / b_hat / s / e / m regions with shared per-request offsets,
-// and the external products' input and output blocks.  Each group is one
-// launch.
+// Arena layout (staging mirrors it byte for byte): per poly-multiply ring
+// degree an a block and a b block of limb-polynomial rows (each request's
+// limbs consecutive; the product replaces a) and the two-pass scratch, one
+// byte per row naming its limb (all degrees together), the RLWE requests'
+// a_hat / b_hat / s / e / m regions with shared per-request offsets and the
+// launch's item list and scratch, and the external products' input and output
+// blocks.  Each group is one launch: the products of one ring degree share it
+// whatever their limb counts.
 int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
     Slot &sl = s->slots[b.slot];
     hipStream_t st = sl.stream;
@@ -303,30 +350,40 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
         off += round_up(bytes);
         return o;
     };
-    // poly-multiply groups
-    std::map<std::pair<uint32_t, uint32_t>, std::vector<size_t>> groups;
+    // poly-multiply groups, one per ring degree
+    std::map<uint32_t, std::vector<size_t>> groups;
     std::vector<size_t> rlwe, ext;
     for (size_t i = 0; i < tk.size(); ++i) {
         const ph_serve_request &r = tk[i]->req;
-        if (r.kind == PH_REQ_POLY_MUL) groups[{r.log_n, r.limbs}].push_back(i);
+        if (r.kind == PH_REQ_POLY_MUL) groups[r.log_n].push_back(i);
         else if (r.kind == PH_REQ_RLWE_ENCRYPT) rlwe.push_back(i);
         else ext.push_back(i);
     }
-    struct GroupAt { uint64_t a, b, count; ph_dcrt_table *t; uint32_t log_n; };
+    struct GroupAt { uint64_t a, b, scratch, rows, row0; ph_dcrt_table *t; uint32_t log_n; };
     std::vector<GroupAt> gat;
+    std::vector<uint8_t> limb_of;  // every group's rows, in group order
     for (auto &g : groups) {
-        const uint64_t row = (uint64_t)g.first.second << g.first.first;  // words per request
-        GroupAt x{take(row * 8 * g.second.size()), take(row * 8 * g.second.size()), g.second.size(), s->dcrt[g.first],
-                  g.first.first};
-        for (size_t k = 0; k < g.second.size(); ++k) {
-            Part &p = b.parts[g.second[k]];
-            p.in0 = x.a + k * row * 8;
-            p.in1 = x.b + k * row * 8;
+        const uint32_t log_n = g.first;
+        uint64_t rows = 0;
+        for (size_t i : g.second) rows += tk[i]->req.limbs;
+        ph_dcrt_table *t = s->dcrt[{log_n, s->cfg.max_limbs}];
+        const uint64_t row_bytes = 8ull << log_n;
+        GroupAt x{take(rows * row_bytes), take(rows * row_bytes), take(ph::negacyclic_mul_scratch_bytes(t, rows)), rows,
+                  limb_of.size(), t, log_n};
+        uint64_t r0 = 0;
+        for (size_t i : g.second) {
+            Part &p = b.parts[i];
+            const uint32_t limbs = tk[i]->req.limbs;
+            p.in0 = x.a + r0 * row_bytes;
+            p.in1 = x.b + r0 * row_bytes;
             p.out = p.in0;
-            p.out_bytes = row * 8;
+            p.out_bytes = limbs * row_bytes;
+            for (uint32_t j = 0; j < limbs; ++j) limb_of.push_back((uint8_t)j);
+            r0 += limbs;
         }
         gat.push_back(x);
     }
+    const uint64_t lmap = take(limb_of.size());
     // RLWE regions
     uint64_t a_words = 0, small = 0;
     std::vector<ph_rlwe_request> rq;
@@ -342,6 +399,12 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
         small += 1ull << r.log_n;
     }
     const uint64_t ra = take(a_words * 8), rout = take(a_words * 8), rs = take(small), re = take(small), rm = take(small);
+    // the RLWE launch's item list and scratch, staged in the arena: no
+    // allocation or host synchronisation inside the launch
+    const ph::RlweStageSize rstage = rlwe.empty() ? ph::RlweStageSize{0, 0}
+                                                  : ph::rlwe_stage_size(s->rlwe, rq.data(), (uint32_t)rq.size());
+    if (!rlwe.empty() && rstage.item_bytes == 0) return PH_ERR_INVALID;
+    const uint64_t ritems = take(rstage.item_bytes), rscratch = take(rstage.scratch_bytes);
     for (size_t k = 0; k < rlwe.size(); ++k) {
         Part &p = b.parts[rlwe[k]];
         p.in0 = ra + rq[k].a_offset * 8;
@@ -388,10 +451,32 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
     }
     s->copy_in.run(jobs);
     void *arena = nullptr;
-    if (hipMallocAsync(&arena, std::max<uint64_t>(off, kAlign), st) != hipSuccess) return PH_ERR_NO_MEMORY;
+    if (!rlwe.empty() && ensure_items(sl, rstage.item_bytes) != PH_OK) return PH_ERR_NO_MEMORY;
+    if (!limb_of.empty() && !s->cfg.per_poly_launches && ensure_maps(sl, limb_of.size()) != PH_OK)
+        return PH_ERR_NO_MEMORY;
+    if (hipMallocFromPoolAsync(&arena, std::max<uint64_t>(off, kAlign), s->pool, st) != hipSuccess)
+        return PH_ERR_NO_MEMORY;
     char *dev = (char *)arena;
     int rc = PH_OK;
-    if (staged && hipMemcpyAsync(dev, host, off, hipMemcpyHostToDevice, st) != hipSuccess) rc = PH_ERR_DEVICE;
+    const hipStream_t sin = s->h2d, sout = s->d2h;
+    if (hipEventRecord(sl.ev_alloc, st) != hipSuccess || hipStreamWaitEvent(sin, sl.ev_alloc, 0) != hipSuccess)
+        rc = PH_ERR_DEVICE;
+    // staged inputs: the arena's input regions only (not the results, the
+    // scratch or the item list), each one copy
+    if (staged) {
+        std::vector<std::pair<uint64_t, uint64_t>> in_regions;  // (offset, bytes)
+        for (const GroupAt &g : gat) in_regions.push_back({g.a, g.scratch - g.a});  // a and b blocks
+        if (!rlwe.empty()) {
+            in_regions.push_back({ra, rout - ra});     // a_hat
+            in_regions.push_back({rs, ritems - rs});   // s, e, m
+        }
+        if (!ext.empty()) in_regions.push_back({xin, xout - xin});
+        for (const auto &rg : in_regions) {
+            if (rc != PH_OK || rg.second == 0) continue;
+            if (hipMemcpyAsync(dev + rg.first, host + rg.first, rg.second, hipMemcpyHostToDevice, sin) != hipSuccess)
+                rc = PH_ERR_DEVICE;
+        }
+    }
     // page-locked requests: every copy of the batch, in and out, as spans of
     // two copy-kernel launches (inputs now, results after the compute)
     std::vector<ph::CopySpan> spans_in, spans_out;
@@ -405,29 +490,38 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
         else {
             std::memcpy(sl.spans, spans_in.data(), spans_in.size() * sizeof(ph::CopySpan));
             std::memcpy(sl.spans + spans_in.size(), spans_out.data(), spans_out.size() * sizeof(ph::CopySpan));
-            if (ph::launch_copy_spans(sl.spans, (uint32_t)spans_in.size(), st) != hipSuccess) rc = PH_ERR_DEVICE;
+            if (ph::launch_copy_spans(sl.spans, (uint32_t)spans_in.size(), sin) != hipSuccess) rc = PH_ERR_DEVICE;
         }
     }
+    // the row -> limb map, after the staging copy of the whole arena
+    if (rc == PH_OK && !limb_of.empty() && !s->cfg.per_poly_launches) {
+        std::memcpy(sl.maps, limb_of.data(), limb_of.size());
+        if (hipMemcpyAsync(dev + lmap, sl.maps, limb_of.size(), hipMemcpyHostToDevice, sin) != hipSuccess)
+            rc = PH_ERR_DEVICE;
+    }
+    if (rc == PH_OK && (hipEventRecord(sl.ev_in, sin) != hipSuccess || hipStreamWaitEvent(st, sl.ev_in, 0) != hipSuccess))
+        rc = PH_ERR_DEVICE;
     for (const GroupAt &g : gat) {
         if (rc != PH_OK) break;
-        const uint32_t L = (uint32_t)ph_dcrt_moduli_count(g.t);
+        const uint64_t row_bytes = 8ull << g.log_n;
         if (s->cfg.per_poly_launches) {
-            const uint32_t log_n = g.log_n;
-            const uint64_t row = (1ull << log_n) * 8;
-            for (uint64_t k = 0; k < g.count * L && rc == PH_OK; ++k) {
-                rc = ph_dcrt_negacyclic_mul_device(s->single[{log_n, (uint32_t)(k % L)}], (uint64_t *)(dev + g.a + k * row),
-                                                   (const uint64_t *)(dev + g.b + k * row), 1, st);
+            for (uint64_t k = 0; k < g.rows && rc == PH_OK; ++k) {
+                rc = ph_dcrt_negacyclic_mul_device(s->single[{g.log_n, (uint32_t)limb_of[g.row0 + k]}],
+                                                   (uint64_t *)(dev + g.a + k * row_bytes),
+                                                   (const uint64_t *)(dev + g.b + k * row_bytes), 1, st);
                 ++s->stats.launches;
             }
             continue;
         }
-        rc = ph_dcrt_negacyclic_mul_device(g.t, (uint64_t *)(dev + g.a), (const uint64_t *)(dev + g.b), g.count * L, st);
+        rc = ph::negacyclic_mul_rows_staged(g.t, (uint64_t *)(dev + g.a), (const uint64_t *)(dev + g.b),
+                                            (const uint8_t *)(dev + lmap + g.row0), g.rows, dev + g.scratch, st);
         ++s->stats.launches;
     }
     if (rc == PH_OK && !rlwe.empty()) {
-        rc = ph_rlwe_encrypt_device(s->rlwe, rq.data(), (uint32_t)rq.size(), (const uint64_t *)(dev + ra),
-                                    (const int8_t *)(dev + rs), (const int8_t *)(dev + re), (const uint8_t *)(dev + rm),
-                                    (uint64_t *)(dev + rout), st);
+        rc = ph::rlwe_encrypt_staged(s->rlwe, rq.data(), (uint32_t)rq.size(), (const uint64_t *)(dev + ra),
+                                     (const int8_t *)(dev + rs), (const int8_t *)(dev + re),
+                                     (const uint8_t *)(dev + rm), (uint64_t *)(dev + rout), st, sl.items,
+                                     dev + ritems, dev + rscratch, rstage);
         ++s->stats.launches;
     }
     if (rc == PH_OK && !ext.empty()) {
@@ -449,14 +543,24 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
         for (size_t i = 0; i < tk.size(); ++i)
             if (!tk[i]->direct) regions.push_back({b.parts[i].out, b.parts[i].out_bytes});
     }
+    if (rc == PH_OK &&
+        (hipEventRecord(sl.ev_comp, st) != hipSuccess || hipStreamWaitEvent(sout, sl.ev_comp, 0) != hipSuccess))
+        rc = PH_ERR_DEVICE;
     for (const auto &rg : regions) {
         if (rc != PH_OK || rg.second == 0) continue;
-        if (hipMemcpyAsync(host + rg.first, dev + rg.first, rg.second, hipMemcpyDeviceToHost, st) != hipSuccess)
+        if (hipMemcpyAsync(host + rg.first, dev + rg.first, rg.second, hipMemcpyDeviceToHost, sout) != hipSuccess)
             rc = PH_ERR_DEVICE;
     }
     if (rc == PH_OK && !spans_out.empty() &&
-        ph::launch_copy_spans(sl.spans + spans_in.size(), (uint32_t)spans_out.size(), st) != hipSuccess)
+        ph::launch_copy_spans(sl.spans + spans_in.size(), (uint32_t)spans_out.size(), sout) != hipSuccess)
         rc = PH_ERR_DEVICE;
+    // the arena goes back (and the batch completes) once the results are out;
+    // on an error the slot stream still waits for whatever the copy streams
+    // were given
+    if (hipEventRecord(sl.ev_out, sout) != hipSuccess || hipStreamWaitEvent(st, sl.ev_out, 0) != hipSuccess)
+        rc = rc == PH_OK ? PH_ERR_DEVICE : rc;
+    if (hipEventRecord(sl.ev_in, sin) != hipSuccess || hipStreamWaitEvent(st, sl.ev_in, 0) != hipSuccess)
+        rc = rc == PH_OK ? PH_ERR_DEVICE : rc;
     (void)hipFreeAsync(arena, st);
     if (hipEventRecord(b.done, st) != hipSuccess && rc == PH_OK) rc = PH_ERR_DEVICE;
     return rc;
@@ -469,11 +573,13 @@ void dispatcher_loop(ph_server *s) {
         if (s->pending.empty() && s->stopping) return;
         // a free slot
         int slot = -1;
+        const auto w0 = Clock::now();
         s->cv_slot.wait(lk, [&] {
             for (size_t i = 0; i < s->slots.size(); ++i)
                 if (!s->slots[i].busy) return true;
             return false;
         });
+        s->stats.slot_wait_s += std::chrono::duration<double>(Clock::now() - w0).count();
         for (size_t i = 0; i < s->slots.size(); ++i)
             if (!s->slots[i].busy) {
                 slot = (int)i;
@@ -514,8 +620,11 @@ void dispatcher_loop(ph_server *s) {
         s->stats.peak_inflight_bytes = std::max(s->stats.peak_inflight_bytes, s->inflight_bytes);
         (void)hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
         lk.unlock();
+        const auto d0 = Clock::now();
         b.status = launch_batch(s, b, tk);
+        const double took = std::chrono::duration<double>(Clock::now() - d0).count();
         lk.lock();
+        s->stats.dispatch_s += took;
         s->inflight.push_back(std::move(b));
         ++s->stats.batches;
         s->cv_done.notify_all();  // wakes the completer
@@ -541,11 +650,14 @@ void completer_loop(ph_server *s) {
         }
         lk.unlock();
         int st = b.status;
+        const auto e0 = Clock::now();
         if (hipEventSynchronize(ev) != hipSuccess && st == PH_OK) st = PH_ERR_DEVICE;
+        const double waited = std::chrono::duration<double>(Clock::now() - e0).count();
         // results from the pinned staging area to the callers' buffers
         if (st == PH_OK) s->copy_out.run(jobs);
         const auto now = Clock::now();
         lk.lock();
+        s->stats.device_wait_s += waited;
         for (const Part &p : b.parts) {
             Ticket &t = s->tickets[p.ticket];
             t.done_at = now;
@@ -569,9 +681,16 @@ void destroy(ph_server *s) {
     if (s->rlwe) ph_rlwe_ctx_free(s->rlwe);
     for (Slot &sl : s->slots) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        for (hipEvent_t e : {sl.ev_alloc, sl.ev_in, sl.ev_comp, sl.ev_out})
+            if (e) (void)hipEventDestroy(e);
         if (sl.pinned) (void)hipHostFree(sl.pinned);
         if (sl.spans) (void)hipHostFree(sl.spans);
+        if (sl.items) (void)hipHostFree(sl.items);
+        if (sl.maps) (void)hipHostFree(sl.maps);
     }
+    if (s->h2d) (void)hipStreamDestroy(s->h2d);
+    if (s->d2h) (void)hipStreamDestroy(s->d2h);
+    if (s->pool) (void)hipMemPoolDestroy(s->pool);
     delete s;
 }
 
@@ -620,9 +739,35 @@ int ph_server_new(const ph_server_config *cfg, ph_server **out) {
                 sub.push_back(s->moduli[(size_t)(ln - cfg->min_log_n) * cfg->max_limbs + i]);
         rc = ph_rlwe_ctx_new(s->rlwe_min, s->rlwe_max, s->rlwe_limbs, sub.data(), &s->rlwe);
     }
+    if (rc == PH_OK) {
+        int dev = 0;
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        if (hipGetDevice(&dev) != hipSuccess) rc = PH_ERR_DEVICE;
+        props.location.id = dev;
+        if (rc == PH_OK && hipMemPoolCreate(&s->pool, &props) != hipSuccess) {
+            s->pool = nullptr;
+            rc = PH_ERR_DEVICE;
+        }
+        uint64_t keep = cfg->hbm_budget_bytes;
+        if (rc == PH_OK && hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
+            rc = PH_ERR_DEVICE;
+        // memory freed on one slot's stream is reused by another only once the
+        // free has completed, never by making that stream wait for it (which
+        // would serialise the slots)
+        int no = 0;
+        if (rc == PH_OK && hipMemPoolSetAttribute(s->pool, hipMemPoolReuseAllowInternalDependencies, &no) != hipSuccess)
+            rc = PH_ERR_DEVICE;
+    }
     s->slots.resize(cfg->streams);
-    for (Slot &sl : s->slots)
+    for (Slot &sl : s->slots) {
         if (rc == PH_OK && hipStreamCreate(&sl.stream) != hipSuccess) rc = PH_ERR_DEVICE;
+        for (hipEvent_t *e : {&sl.ev_alloc, &sl.ev_in, &sl.ev_comp, &sl.ev_out})
+            if (rc == PH_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = PH_ERR_DEVICE;
+    }
+    if (rc == PH_OK && (hipStreamCreate(&s->h2d) != hipSuccess || hipStreamCreate(&s->d2h) != hipSuccess))
+        rc = PH_ERR_DEVICE;
     if (rc != PH_OK) {
         destroy(s);
         return rc;
@@ -636,6 +781,7 @@ int ph_server_new(const ph_server_config *cfg, ph_server **out) {
 
 int ph_server_submit(ph_server *s, const ph_serve_request *req, uint64_t *ticket) {
     if (!s || !req) return PH_ERR_INVALID;
+    const auto t0 = Clock::now();
     int rc = validate(s, *req);
     if (rc != PH_OK) return rc;
     const uint64_t bytes = request_bytes(*req);
@@ -653,6 +799,7 @@ int ph_server_submit(ph_server *s, const ph_serve_request *req, uint64_t *ticket
     const uint64_t id = s->tickets.size() - 1;
     s->pending.push_back(id);
     if (ticket) *ticket = id;
+    s->stats.submit_s += std::chrono::duration<double>(Clock::now() - t0).count();
     s->cv_pending.notify_one();
     return PH_OK;
 }

@@ -305,6 +305,12 @@ def negacyclic_mul_device(table, da, db, count, stream=None):
     _check(lib().ph_dcrt_negacyclic_mul_device(table.handle, da, db, count, _stream(stream)))
 
 
+def negacyclic_mul_rows_device(table, da, db, d_limb_of, count, stream=None):
+    """negacyclic_mul_device over rows of uneven limbs: row i uses the table's
+    modulus d_limb_of[i] (device bytes) instead of i % L."""
+    _check(lib().ph_dcrt_negacyclic_mul_rows_device(table.handle, da, db, d_limb_of, count, _stream(stream)))
+
+
 def pointwise_mul_device(table, da, db, dout, count, stream=None):
     _check(lib().ph_dcrt_pointwise_mul_device(table.handle, da, db, dout, count, _stream(stream)))
 

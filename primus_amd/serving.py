@@ -31,7 +31,8 @@ class ServerConfig(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("launches", ctypes.c_uint64),
-                ("peak_inflight_bytes", ctypes.c_uint64)]
+                ("peak_inflight_bytes", ctypes.c_uint64), ("submit_s", ctypes.c_double),
+                ("dispatch_s", ctypes.c_double), ("slot_wait_s", ctypes.c_double), ("device_wait_s", ctypes.c_double)]
 
 
 def _declare():

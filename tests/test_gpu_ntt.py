@@ -195,6 +195,41 @@ def test_negacyclic_batch(log_n, L, count, bits):
     np.testing.assert_array_equal(db.download((count, n)), b)
 
 
+@pytest.mark.parametrize("bits", [59, 50])
+@pytest.mark.parametrize("log_n", [10, 12, 13, 14, 16])
+def test_negacyclic_rows_uneven_limbs(log_n, bits):
+    """Requests of one ring degree with uneven limb counts in one launch: row i
+    uses limb limb_of[i] of a 4-limb table; a row naming no limb (7) stays as it
+    was, and b is left intact."""
+    n, L = 1 << log_n, 4
+    moduli = pa.ntt_primes_below(bits, 2 * n, L)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    limbs = [3, 1, 4, 2] if log_n <= 13 else [2, 1]
+    limb_of = np.array([j for k in limbs for j in range(k)] + [7], dtype=np.uint8)
+    rows = limb_of.size
+    print("negacyclic rows", log_n, bits, rows, flush=True)
+    rng = np.random.default_rng(100 + log_n)
+    qs = [moduli[j] if j < L else moduli[0] for j in limb_of]
+    a = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in qs])
+    b = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in qs])
+    da, db = pa.DeviceBuffer.from_array(a), pa.DeviceBuffer.from_array(b)
+    padded = np.zeros((rows + 7) // 8 * 8, dtype=np.uint8)
+    padded[:rows] = limb_of
+    dm = pa.DeviceBuffer.from_array(padded.view(np.uint64))
+    pa.negacyclic_mul_rows_device(tab, da.ptr, db.ptr, dm.ptr, rows)
+    pa.synchronize()
+    got = da.download((rows, n))
+    tables = [O.Table(log_n, q) for q in moduli]
+    for i, j in enumerate(limb_of):
+        if j >= L:
+            np.testing.assert_array_equal(got[i], a[i], err_msg="row %d without a limb" % i)
+            continue
+        t = tables[j]
+        want = t.inverse_transform(O.pointwise_mul(t.transform(a[i]), t.transform(b[i]), t.q))
+        np.testing.assert_array_equal(got[i], want, err_msg="row %d limb %d" % (i, j))
+    np.testing.assert_array_equal(db.download((rows, n)), b)
+
+
 @pytest.mark.parametrize("bits", [62, 50, 40])
 def test_device_forward_inverse_batch(bits):
     log_n, L, count = 11, 3, 12
