@@ -11,7 +11,7 @@
 // and every request a copy keeps outstanding waits in the memory system's
 // queues, in front of the requests of kernels running on other streams.  One
 // workgroup per span (hundreds per batch) kept megabytes in flight and held
-// the other streams' kernels back for the whole copy; kCopyGroups workgroups
+// the other streams' kernels back for the whole copy; a few dozen workgroups
 // with two loads per lane in flight keep 8 KiB each, enough to fill the link.
 #include "copy_launch.hpp"
 
@@ -45,9 +45,10 @@ __global__ __launch_bounds__(256) void copy_spans_kernel(const CopySpan *spans, 
     }
 }
 
-hipError_t launch_copy_spans(const CopySpan *spans, uint32_t count, hipStream_t s) {
+hipError_t launch_copy_spans(const CopySpan *spans, uint32_t count, uint32_t groups, hipStream_t s) {
     if (count == 0) return hipSuccess;
-    copy_spans_kernel<<<dim3(count < kCopyGroups ? count : kCopyGroups), dim3(256), 0, s>>>(spans, count);
+    if (groups == 0) return hipErrorInvalidValue;
+    copy_spans_kernel<<<dim3(count < groups ? count : groups), dim3(256), 0, s>>>(spans, count);
     return hipGetLastError();
 }
 

@@ -17,12 +17,17 @@ struct CopySpan {
 // workgroups share them evenly (a span is one workgroup's unit of work).
 constexpr uint64_t kCopySpanMax = 256u << 10;
 
-// Workgroups of one span-copy launch (fewer when there are fewer spans):
-// 64 keep one direction of the host link at its full rate (57 GB/s measured
-// on MI355X, the same as a DMA copy) and both directions near 44 GB/s each.
-constexpr uint32_t kCopyGroups = 64;
+// Workgroups of one span-copy launch (fewer when there are fewer spans).
+// Inputs: 64 keep one direction of the host link at its full rate (57 GB/s
+// measured on MI355X, the same as a DMA copy), and near 44 GB/s while the
+// other direction runs too.  Results: 16, enough for the serving mix's result
+// traffic (about 0.6 of its input bytes) with fewer stores outstanding while
+// other streams compute.
+constexpr uint32_t kCopyGroupsIn = 64;
+constexpr uint32_t kCopyGroupsOut = 16;
 
-// spans: `count` spans in memory the device can read (page-locked host memory).
-hipError_t launch_copy_spans(const CopySpan *spans, uint32_t count, hipStream_t s);
+// spans: `count` spans in memory the device can read (page-locked host
+// memory), copied by at most `groups` workgroups.
+hipError_t launch_copy_spans(const CopySpan *spans, uint32_t count, uint32_t groups, hipStream_t s);
 
 }  // namespace ph

@@ -490,7 +490,8 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
         else {
             std::memcpy(sl.spans, spans_in.data(), spans_in.size() * sizeof(ph::CopySpan));
             std::memcpy(sl.spans + spans_in.size(), spans_out.data(), spans_out.size() * sizeof(ph::CopySpan));
-            if (ph::launch_copy_spans(sl.spans, (uint32_t)spans_in.size(), sin) != hipSuccess) rc = PH_ERR_DEVICE;
+            if (ph::launch_copy_spans(sl.spans, (uint32_t)spans_in.size(), ph::kCopyGroupsIn, sin) != hipSuccess)
+                rc = PH_ERR_DEVICE;
         }
     }
     // the row -> limb map, after the staging copy of the whole arena
@@ -552,7 +553,8 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
             rc = PH_ERR_DEVICE;
     }
     if (rc == PH_OK && !spans_out.empty() &&
-        ph::launch_copy_spans(sl.spans + spans_in.size(), (uint32_t)spans_out.size(), sout) != hipSuccess)
+        ph::launch_copy_spans(sl.spans + spans_in.size(), (uint32_t)spans_out.size(), ph::kCopyGroupsOut, sout) !=
+            hipSuccess)
         rc = PH_ERR_DEVICE;
     // the arena goes back (and the batch completes) once the results are out;
     // on an error the slot stream still waits for whatever the copy streams
