@@ -58,3 +58,15 @@ def test_null_arguments():
     assert L.ph_transform_slice(None, None, 0) == 5
     assert L.ph_dcrt_table_new(10, None, 0, ctypes.byref(out)) == 5
     assert L.ph_ntt_table_poly_length(None) == 0
+
+
+def test_null_arguments_batched_forms():
+    """Batched entry points reject a null table or buffer before touching the
+    device (status 5, PH_ERR_INVALID); a zero count with null buffers is a no-op
+    only once the table is there, so these all fail on the table."""
+    L = pa.lib()
+    dummy = ctypes.c_void_p(0x1000)  # never dereferenced: the table check fails first
+    assert L.ph_dcrt_negacyclic_mul_rows_device(None, dummy, dummy, dummy, 1, None) == 5
+    assert L.ph_dcrt_negacyclic_mul_device(None, dummy, dummy, 1, None) == 5
+    assert L.ph_dcrt_fwd_mul_inv_device(None, dummy, 1, None, None) == 5
+    assert L.ph_rlwe_encrypt_device(None, None, 0, None, None, None, None, None, None) == 5
