@@ -102,6 +102,22 @@ int ph_transform_slice(const ph_ntt_table *table, uint64_t *poly, size_t len);
 int ph_lazy_inverse_transform_slice(const ph_ntt_table *table, uint64_t *values, size_t len);
 /* NttTable::inverse_transform_slice: [0,2q) -> [0,q) (primitive.rs:288-296) */
 int ph_inverse_transform_slice(const ph_ntt_table *table, uint64_t *values, size_t len);
+/* HexlNttTable::compute_forward (primus_ntt/src/ntt/hexl/mod.rs:447-540), the
+ * entry the HEXL table's NttTable methods call (hexl/mod.rs:321-338).  The
+ * HEXL table computes the transform of UintNttTable on the same roots, and
+ * the reference asserts equal outputs (primus_ntt/tests/hexl.rs), so this
+ * runs the kernels above on a table made by ph_ntt_table_new.
+ * input_mod_factor 1, 2 or 4: operand in [0, factor*q).  output_mod_factor 1:
+ * [0,q), bit-identical to HEXL's; 4: a value in [0,4q) congruent to it (this
+ * table's lazy value, not necessarily HEXL's own lazy bit pattern).  Other
+ * factors are PH_ERR_INVALID (the reference debug_asserts them). */
+int ph_hexl_compute_forward(const ph_ntt_table *table, uint64_t *operand, size_t len, uint64_t input_mod_factor,
+                            uint64_t output_mod_factor);
+/* HexlNttTable::compute_inverse (hexl/mod.rs:542-635): input_mod_factor 1 or 2,
+ * output_mod_factor 1 ([0,q), bit-identical to HEXL's) or 2 (in [0,2q); this
+ * table's lazy inverse is already in [0,q)). */
+int ph_hexl_compute_inverse(const ph_ntt_table *table, uint64_t *operand, size_t len, uint64_t input_mod_factor,
+                            uint64_t output_mod_factor);
 /* NttTable::transform_monomial (primitive.rs:298-347) */
 int ph_transform_monomial(const ph_ntt_table *table, uint64_t coeff, size_t degree, uint64_t *values,
                           size_t len);

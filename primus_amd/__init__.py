@@ -12,6 +12,7 @@ from .ntt import (  # noqa: F401
     NttError,
     NttTableErr,
     Operand,
+    HexlNttTable,
     UintCrtNttTable,
     UintNttTable,
     automorphism_device,

@@ -141,6 +141,8 @@ def _declare(L):
         "ph_dcrt_fwd_mul_inv_device": (i, [vp, vp, sz, vp, vp]),
         "ph_dcrt_negacyclic_mul_device": (i, [vp, vp, vp, sz, vp]),
         "ph_dcrt_negacyclic_mul_rows_device": (i, [vp, vp, vp, vp, sz, vp]),
+        "ph_hexl_compute_forward": (i, [vp, vp, sz, ctypes.c_uint64, ctypes.c_uint64]),
+        "ph_hexl_compute_inverse": (i, [vp, vp, sz, ctypes.c_uint64, ctypes.c_uint64]),
         "ph_dcrt_pointwise_mul_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_pointwise_mul_add_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_automorphism_device": (i, [vp, vp, vp, sz, u64, vp]),

@@ -507,6 +507,24 @@ int ph_inverse_transform_slice(const ph_ntt_table *t, uint64_t *values, size_t l
     return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, t->f64, values, len);
 }
 
+int ph_hexl_compute_forward(const ph_ntt_table *t, uint64_t *operand, size_t len, uint64_t input_mod_factor,
+                            uint64_t output_mod_factor) {
+    if (input_mod_factor != 1 && input_mod_factor != 2 && input_mod_factor != 4)
+        return fail(PH_ERR_INVALID, "input_mod_factor must be 1, 2 or 4");
+    if (output_mod_factor != 1 && output_mod_factor != 4) return fail(PH_ERR_INVALID, "output_mod_factor must be 1 or 4");
+    // both forward kernels take [0,4q)
+    return output_mod_factor == 1 ? ph_transform_slice(t, operand, len) : ph_lazy_transform_slice(t, operand, len);
+}
+
+int ph_hexl_compute_inverse(const ph_ntt_table *t, uint64_t *operand, size_t len, uint64_t input_mod_factor,
+                            uint64_t output_mod_factor) {
+    if (input_mod_factor != 1 && input_mod_factor != 2) return fail(PH_ERR_INVALID, "input_mod_factor must be 1 or 2");
+    if (output_mod_factor != 1 && output_mod_factor != 2) return fail(PH_ERR_INVALID, "output_mod_factor must be 1 or 2");
+    // the inverse takes [0,2q) and returns [0,q) either way
+    return output_mod_factor == 1 ? ph_inverse_transform_slice(t, operand, len)
+                                  : ph_lazy_inverse_transform_slice(t, operand, len);
+}
+
 int ph_transform_monomial(const ph_ntt_table *t, uint64_t coeff, size_t degree, uint64_t *values,
                           size_t len) {
     return monomial(t, coeff, degree, values, len, 0);

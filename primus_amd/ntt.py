@@ -159,6 +159,35 @@ class UintNttTable:
         return values
 
 
+class HexlNttTable(UintNttTable):
+    """HexlNttTable (primus_ntt/src/ntt/hexl/mod.rs): the NttTable interface plus
+    compute_forward / compute_inverse with HEXL's range factors, on the same
+    device tables and kernels.  Full-range outputs and monomials are identical
+    to UintNttTable's (the reference asserts it in primus_ntt/tests/hexl.rs);
+    lazy outputs lie in HEXL's ranges without being HEXL's own bit patterns."""
+
+    def compute_forward(self, operand, input_mod_factor, output_mod_factor):
+        _check(lib().ph_hexl_compute_forward(self._h, _slice_ptr(operand), operand.size, input_mod_factor,
+                                             output_mod_factor))
+
+    def compute_inverse(self, operand, input_mod_factor, output_mod_factor):
+        _check(lib().ph_hexl_compute_inverse(self._h, _slice_ptr(operand), operand.size, input_mod_factor,
+                                             output_mod_factor))
+
+    # the reference's NttTable methods of the HEXL table (hexl/mod.rs:321-338)
+    def lazy_transform_slice(self, poly):
+        self.compute_forward(poly, 1, 4)
+
+    def transform_slice(self, poly):
+        self.compute_forward(poly, 1, 1)
+
+    def lazy_inverse_transform_slice(self, values):
+        self.compute_inverse(values, 1, 2)
+
+    def inverse_transform_slice(self, values):
+        self.compute_inverse(values, 1, 1)
+
+
 class UintCrtNttTable:
     """DcrtTable: one UintNttTable per modulus over limb-major [L][N] data."""
 

@@ -70,3 +70,14 @@ def test_null_arguments_batched_forms():
     assert L.ph_dcrt_negacyclic_mul_device(None, dummy, dummy, 1, None) == 5
     assert L.ph_dcrt_fwd_mul_inv_device(None, dummy, 1, None, None) == 5
     assert L.ph_rlwe_encrypt_device(None, None, 0, None, None, None, None, None, None) == 5
+
+
+def test_hexl_factor_checks_on_host():
+    """HEXL's range factors are checked before the table (no device needed)."""
+    L = pa.lib()
+    assert L.ph_hexl_compute_forward(None, None, 0, 3, 1) == 5
+    assert L.ph_hexl_compute_forward(None, None, 0, 1, 2) == 5
+    assert L.ph_hexl_compute_inverse(None, None, 0, 4, 1) == 5
+    assert L.ph_hexl_compute_inverse(None, None, 0, 1, 4) == 5
+    # valid factors reach the table check
+    assert L.ph_hexl_compute_forward(None, None, 0, 4, 4) == 5
