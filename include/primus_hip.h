@@ -261,6 +261,27 @@ int ph_ext_product_device(const ph_ext_key *key, const uint64_t *d_in, uint64_t 
 /* The same on host memory, synchronous. */
 int ph_ext_product(const ph_ext_key *key, const uint64_t *in, uint64_t *out, size_t count, int ntt_output);
 
+/* The external product over several limbs: CrtGlwe::mul_dcrt_ggsw_inplace
+ * (primus_lattice/src/glwe/crt.rs:201-229) with k = 1.  The key is a DCRT GGSW
+ * [2][l][2][L][N] in NTT form over the table's L limbs (1 <= L <= 4), values
+ * below their limb's modulus, with the gadget basis
+ * BigUintApproxSignedBasis::new(Q, log_basis, Some(l)) over Q = q_0 ... q_{L-1}
+ * (primus_decompose/src/big_integer/basis.rs:34-195).  The table must outlive
+ * the key. */
+typedef struct ph_ext_key_rns ph_ext_key_rns;
+int ph_ext_key_rns_new(const ph_dcrt_table *table, uint32_t log_basis, uint32_t decompose_length,
+                       const uint64_t *key_ntt, ph_ext_key_rns **out);
+void ph_ext_key_rns_free(ph_ext_key_rns *key);
+/* d_in [count][2][L][N] residues, d_out [count][2][L][N]: per input polynomial
+ * each coefficient's residues composed into one integer below Q
+ * (primus_rns/src/base.rs:396-450), its gadget digits, each digit wrapped
+ * into every limb (base.rs:201-236), transformed, and multiply-accumulated
+ * with the key (glwe/dcrt.rs:170-247); then, unless ntt_output, the inverse
+ * transform of the accumulators.  A digit pass, the batched transforms and a
+ * multiply-accumulate pass, on the stream; scratch by hipMallocAsync. */
+int ph_ext_product_rns_device(const ph_ext_key_rns *key, const uint64_t *d_in, uint64_t *d_out, size_t count,
+                              int ntt_output, void *stream);
+
 /* ---- RLWE encryption, uneven batches ---- */
 
 typedef struct ph_rlwe_ctx ph_rlwe_ctx; /* device-resident tables for a grid of (ring degree, limb) */

@@ -266,6 +266,109 @@ def ext_product(table, basis, key, rlwe, ntt_output=False):
     return out.reshape(2, table.n)
 
 
+class RnsBasis:
+    """BigUintApproxSignedBasis over Q = the product of several limb moduli
+    (primus_decompose/src/big_integer/basis.rs:34-195) with the RNS base's
+    composition (primus_rns/src/base.rs:356-450), on Python integers: the
+    multi-limb form of Basis/po_basis (which restates one u64 limb)."""
+
+    def __init__(self, moduli, log_basis, l):
+        self.moduli = [int(q) for q in moduli]
+        Q = 1
+        for q in self.moduli:
+            Q *= q
+        self.Q = Q
+        bits = Q.bit_length()
+        if log_basis <= 0 or log_basis >= 64 or l <= 0 or bits // log_basis < l:
+            raise ValueError("gadget basis does not fit the modulus")  # basis.rs:44-59
+        self.log_basis, self.l, self.bits = log_basis, l, bits
+        self.drop_bits = bits - l * log_basis
+        self.basis = 1 << log_basis
+        bm1 = self.basis - 1
+        self.carry_mask = 2 if log_basis == 1 else (self.basis | (self.basis >> 1))  # :75-79
+        # split_value (:81-128)
+        if log_basis == 1:
+            if self.drop_bits == 0:
+                split = None
+            else:
+                v = 0
+                for _ in range(l + 1):
+                    v = (v << 1) | 1
+                v <<= self.drop_bits - 1
+                split = v if v < Q else None
+        else:
+            v = 0
+            for _ in range(l):
+                v = (v << log_basis) | (bm1 >> 1)
+            v = (((v << 1) | 1) << (self.drop_bits - 1)) if self.drop_bits > 0 else v + 1
+            split = v if v < Q else None
+        self.split = split
+        self.pow2_sub_modulus = (1 << bits) - Q  # :134-144
+        # compose constants (base.rs:33-70): Q/q_i and (Q/q_i)^-1 mod q_i
+        self.punctured = [Q // q for q in self.moduli]
+        self.inv_punctured = [pow(Q // q % q, -1, q) for q in self.moduli]
+
+    def compose(self, residues):
+        """[L][N] residues -> N Python integers in [0, Q) (compose_inplace, base.rs:396-420:
+        one product per limb added with a conditional subtraction of Q)."""
+        r = np.asarray(residues, dtype=np.uint64)
+        out = []
+        for k in range(r.shape[1]):
+            value = 0
+            for i, q in enumerate(self.moduli):
+                value += (int(r[i, k]) * self.inv_punctured[i] % q) * self.punctured[i]
+                if value >= self.Q:
+                    value -= self.Q
+            out.append(value)
+        return out
+
+    def decompose(self, residues):
+        """[L][N] residues -> [l][L][N] wrapped digit residues: init_value_carry
+        (basis.rs:307-331), the digits with their carries (common.rs:141-152,
+        184-201) and wrapping_decompose_small_values_inplace (base.rs:201-236)."""
+        r = np.asarray(residues, dtype=np.uint64)
+        L, n = r.shape
+        out = np.zeros((self.l, L, n), dtype=np.uint64)
+        half = (self.basis + 1) // 2
+        bm1 = self.basis - 1
+        for k, value in enumerate(self.compose(r)):
+            if self.split is not None and value >= self.split:
+                value += self.pow2_sub_modulus
+            carry = (value >> (self.drop_bits - 1)) & 1 if self.drop_bits > 0 else 0
+            for d in range(self.l):
+                temp = ((value >> (self.drop_bits + d * self.log_basis)) & bm1) + carry
+                carry = 1 if temp & self.carry_mask else 0
+                digit = temp & bm1
+                for i, q in enumerate(self.moduli):
+                    out[d, i, k] = digit if (self.basis == 2 or digit < half) else q - self.basis + digit
+        return out
+
+
+def ext_product_rns(tables, basis, key, rlwe, ntt_output=False):
+    """External product of one CRT GLWE (k = 1) [2][L][N] with a DCRT GGSW key
+    [2][l][2][L][N] in NTT form (CrtGlwe::mul_dcrt_ggsw_inplace, primus_lattice/src/glwe/crt.rs:201-229,
+    through add_dcrt_glev_mul_crt_poly_assign, glwe/dcrt.rs:170-247): for each input
+    polynomial p and digit d, the digit's residues are transformed and
+    multiply-accumulated into both output polynomials.  The result is in NTT
+    form; ntt_output=False applies the inverse transform per limb."""
+    key = np.asarray(key, dtype=np.uint64)
+    x = np.asarray(rlwe, dtype=np.uint64)
+    L, n = len(tables), tables[0].n
+    acc = np.zeros((2, L, n), dtype=np.uint64)
+    for p in range(2):
+        digits = basis.decompose(x[p])
+        for d in range(basis.l):
+            for i, t in enumerate(tables):
+                dh = t.transform(digits[d, i])
+                for c in range(2):
+                    acc[c, i] = pointwise_mul_add(acc[c, i], dh, key[p, d, c, i], t.q)
+    if not ntt_output:
+        for c in range(2):
+            for i, t in enumerate(tables):
+                acc[c, i] = t.inverse_transform(acc[c, i])
+    return acc
+
+
 def rlwe_encrypt(table, s, e, m, delta, a_hat):
     """b_hat = NTT(e + delta*m) + a_hat (.) NTT(s) for one limb (po_rlwe_encrypt)."""
     L = lib()

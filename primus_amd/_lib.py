@@ -142,6 +142,9 @@ def _declare(L):
         "ph_dcrt_negacyclic_mul_device": (i, [vp, vp, vp, sz, vp]),
         "ph_dcrt_negacyclic_mul_rows_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_hexl_compute_forward": (i, [vp, vp, sz, ctypes.c_uint64, ctypes.c_uint64]),
+        "ph_ext_key_rns_new": (i, [vp, ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.POINTER(vp)]),
+        "ph_ext_key_rns_free": (None, [vp]),
+        "ph_ext_product_rns_device": (i, [vp, vp, vp, sz, ctypes.c_int, vp]),
         "ph_hexl_compute_inverse": (i, [vp, vp, sz, ctypes.c_uint64, ctypes.c_uint64]),
         "ph_dcrt_pointwise_mul_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_pointwise_mul_add_device": (i, [vp, vp, vp, vp, sz, vp]),

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <atomic>
 #include <cstdio>
@@ -1044,6 +1045,218 @@ int ph_ext_key_new(const ph_ntt_table *t, uint32_t log_basis, uint32_t decompose
 }
 
 uint32_t ph_ext_key_log_n(const ph_ext_key *key) { return key ? key->table->log_n : 0; }
+
+// ---------------------------------------------------------------------------
+// External product over several limbs
+// ---------------------------------------------------------------------------
+
+namespace {
+// Little-endian kExtRnsWords-word integers on the host, for the basis constants.
+using Words = std::array<uint64_t, kExtRnsWords>;
+
+Words words_mul_small(const Words &v, uint64_t m, bool *overflow) {
+    Words r{};
+    u128 carry = 0;
+    for (uint32_t i = 0; i < kExtRnsWords; ++i) {
+        const u128 t = (u128)v[i] * m + carry;
+        r[i] = (uint64_t)t;
+        carry = t >> 64;
+    }
+    if (overflow) *overflow = carry != 0;
+    return r;
+}
+
+uint64_t words_mod_small(const Words &v, uint64_t q) {
+    u128 r = 0;
+    for (int i = kExtRnsWords - 1; i >= 0; --i) r = ((r << 64) | v[i]) % q;
+    return (uint64_t)r;
+}
+
+uint32_t words_bits(const Words &v) {
+    for (int i = kExtRnsWords - 1; i >= 0; --i)
+        if (v[i]) return 64u * i + 64u - (uint32_t)__builtin_clzll(v[i]);
+    return 0;
+}
+
+bool words_less(const Words &a, const Words &b) {
+    for (int i = kExtRnsWords - 1; i >= 0; --i)
+        if (a[i] != b[i]) return a[i] < b[i];
+    return false;
+}
+
+// v << s for s < 64 (bits shifted out of the top word are lost)
+Words words_shl(const Words &v, uint32_t s) {
+    if (s == 0) return v;
+    Words r{};
+    for (int i = kExtRnsWords - 1; i >= 0; --i) r[i] = (v[i] << s) | (i ? v[i - 1] >> (64 - s) : 0);
+    return r;
+}
+
+Words words_shl_any(Words v, uint32_t s) {
+    while (s >= 63) {
+        v = words_shl(v, 63);
+        s -= 63;
+    }
+    return words_shl(v, s);
+}
+}  // namespace
+
+struct ph_ext_key_rns {
+    const ph_dcrt_table *table;  // not owned
+    ExtRnsArgs proto;
+    uint64_t *d_key = nullptr;
+};
+
+int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t decompose_length, const uint64_t *key_ntt,
+                       ph_ext_key_rns **out) {
+    if (!d || !key_ntt || !out) return fail(PH_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (d->L == 0 || d->L > kExtRnsMaxLimbs) return fail(PH_ERR_UNSUPPORTED, "1 to 4 limbs");
+    ExtRnsArgs g{};
+    g.L = d->L;
+    g.log_n = d->log_n;
+    // Q and the punctured products Q/q_i (primus_rns/src/base.rs:33-70)
+    Words Q{};
+    Q[0] = 1;
+    for (uint32_t i = 0; i < d->L; ++i) Q = words_mul_small(Q, d->tables[i]->q, nullptr);
+    for (uint32_t i = 0; i < d->L; ++i) {
+        const uint64_t q = d->tables[i]->q;
+        Words p{};
+        p[0] = 1;
+        for (uint32_t j = 0; j < d->L; ++j)
+            if (j != i) p = words_mul_small(p, d->tables[j]->q, nullptr);
+        const uint64_t inv = inv_mod(words_mod_small(p, q), q);
+        if (inv == 0) return fail(PH_ERR_INVALID, "limb moduli are not coprime");
+        g.q[i] = q;
+        g.inv[i] = inv;
+        g.inv_q[i] = h_shoup_quotient(inv, q);
+        for (uint32_t w = 0; w < kExtRnsWords; ++w) g.punct[i][w] = p[w];
+    }
+    // BigUintApproxSignedBasis::new(Q, log_basis, Some(l)) (primus_decompose/src/big_integer/basis.rs:34-195)
+    const uint32_t bits = words_bits(Q);
+    if (log_basis == 0 || log_basis >= 64 || decompose_length == 0 || bits / log_basis < decompose_length)
+        return fail(PH_ERR_INVALID, "gadget basis does not fit the modulus");
+    g.l = decompose_length;
+    g.log_basis = log_basis;
+    g.drop_bits = bits - decompose_length * log_basis;
+    g.basis = 1ull << log_basis;
+    g.basis_minus_one = g.basis - 1;
+    g.carry_mask = log_basis == 1 ? 2ull : (g.basis | (g.basis >> 1));
+    g.half = (g.basis + 1) / 2;
+    Words split{};
+    bool has_split;
+    if (log_basis == 1) {
+        if (g.drop_bits == 0) {
+            has_split = false;
+        } else {
+            for (uint32_t i = 0; i <= decompose_length; ++i) {
+                split = words_shl(split, 1);
+                split[0] |= 1;
+            }
+            split = words_shl_any(split, g.drop_bits - 1);
+            has_split = words_less(split, Q);
+        }
+    } else {
+        for (uint32_t i = 0; i < decompose_length; ++i) {
+            split = words_shl(split, log_basis);
+            split[0] |= g.basis_minus_one >> 1;
+        }
+        if (g.drop_bits > 0) {
+            split = words_shl(split, 1);
+            split[0] |= 1;
+            split = words_shl_any(split, g.drop_bits - 1);
+        } else {
+            u128 c = 1;  // split + 1
+            for (uint32_t w = 0; w < kExtRnsWords; ++w) {
+                const u128 t = (u128)split[w] + c;
+                split[w] = (uint64_t)t;
+                c = t >> 64;
+            }
+        }
+        has_split = words_less(split, Q);
+    }
+    g.has_split = has_split;
+    // 2^bits - Q: the two's complement of Q in kExtRnsWords words, reduced mod 2^bits
+    Words pow2_sub{};
+    u128 c = 1;
+    for (uint32_t w = 0; w < kExtRnsWords; ++w) {
+        const u128 t = (u128)(~Q[w]) + c;
+        pow2_sub[w] = (uint64_t)t;
+        c = t >> 64;
+    }
+    for (uint32_t w = 0; w < kExtRnsWords; ++w) {
+        const uint32_t lo = 64 * w;
+        if (bits <= lo) pow2_sub[w] = 0;
+        else if (bits < lo + 64) pow2_sub[w] &= (1ull << (bits - lo)) - 1;
+    }
+    for (uint32_t w = 0; w < kExtRnsWords; ++w) {
+        g.Q[w] = Q[w];
+        g.split[w] = split[w];
+        g.pow2_sub[w] = pow2_sub[w];
+    }
+    // key [2][l][2][L][N], canonical per limb
+    const uint64_t n = 1ull << d->log_n, words = 4ull * decompose_length * d->L * n;
+    for (uint64_t r = 0; r < 4ull * decompose_length * d->L; ++r) {
+        const uint64_t q = d->tables[r % d->L]->q;
+        for (uint64_t k = 0; k < n; ++k)
+            if (key_ntt[r * n + k] >= q) return fail(PH_ERR_INVALID, "key values must be below their limb modulus");
+    }
+    ph_ext_key_rns *key = new (std::nothrow) ph_ext_key_rns();
+    if (!key) return fail(PH_ERR_NO_MEMORY, "key");
+    key->table = d;
+    if (hipMalloc(&key->d_key, words * 8) != hipSuccess ||
+        hipMemcpy(key->d_key, key_ntt, words * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        if (key->d_key) (void)hipFree(key->d_key);
+        delete key;
+        return fail(PH_ERR_DEVICE, "key upload");
+    }
+    key->proto = g;
+    *out = key;
+    return PH_OK;
+}
+
+void ph_ext_key_rns_free(ph_ext_key_rns *key) {
+    if (!key) return;
+    if (key->d_key) (void)hipFree(key->d_key);
+    delete key;
+}
+
+int ph_ext_product_rns_device(const ph_ext_key_rns *key, const uint64_t *d_in, uint64_t *d_out, size_t count,
+                              int ntt_output, void *stream) {
+    if (!key || ((!d_in || !d_out) && count)) return fail(PH_ERR_INVALID, "null argument");
+    if (count == 0) return PH_OK;
+    const ph_dcrt_table *d = key->table;
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t n = 1ull << d->log_n, L = d->L, l = key->proto.l;
+    uint64_t *digits = nullptr;
+    PH_HIP(hipMallocAsync((void **)&digits, count * 2 * l * L * n * 8, s));
+    ExtRnsArgs a = key->proto;
+    a.in = d_in;
+    a.digits = digits;
+    a.count = count;
+    hipError_t e = launch_ext_rns_digits(a, s);
+    int rc = e == hipSuccess ? PH_OK : hip_fail(e, "digit kernel");
+    // the digits' forward transforms: row r of [count][2][l][L][N] is limb r % L
+    if (rc == PH_OK) rc = run_op(Op::FwdCanon, d->d_tabs, d->L, d->log_n, d->f64, digits, nullptr, nullptr,
+                                 count * 2 * l * L, s);
+    if (rc == PH_OK) {
+        ExtRnsMacArgs m{};
+        m.digits = digits;
+        m.key = key->d_key;
+        m.out = d_out;
+        m.tabs = d->d_tabs;
+        m.count = count;
+        m.L = d->L;
+        m.l = (uint32_t)l;
+        m.log_n = d->log_n;
+        e = launch_ext_rns_mac(m, s);
+        if (e != hipSuccess) rc = hip_fail(e, "multiply-accumulate kernel");
+    }
+    if (rc == PH_OK && !ntt_output)
+        rc = run_op(Op::Inv, d->d_tabs, d->L, d->log_n, d->f64, d_out, nullptr, nullptr, count * 2 * L, s);
+    (void)hipFreeAsync(digits, s);
+    return rc;
+}
 
 void ph_ext_key_free(ph_ext_key *key) {
     if (!key) return;

@@ -179,6 +179,141 @@ hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64
 }
 
 // ---------------------------------------------------------------------------
+// External product over several limbs
+// ---------------------------------------------------------------------------
+
+// Word i of a kExtRnsWords-word value held in registers (i uniform or not).
+__device__ __forceinline__ uint64_t word_at(const uint64_t (&v)[kExtRnsWords], uint32_t i) {
+    return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
+
+__device__ __forceinline__ bool words_ge(const uint64_t (&v)[kExtRnsWords], const uint64_t (&w)[kExtRnsWords]) {
+#pragma unroll
+    for (int i = kExtRnsWords - 1; i >= 0; --i)
+        if (v[i] != w[i]) return v[i] > w[i];
+    return true;
+}
+
+__device__ __forceinline__ void words_sub(uint64_t (&v)[kExtRnsWords], const uint64_t (&w)[kExtRnsWords]) {
+    uint64_t borrow = 0;
+#pragma unroll
+    for (int i = 0; i < (int)kExtRnsWords; ++i) {
+        const uint64_t d = v[i] - w[i];
+        const uint64_t b1 = v[i] < w[i];
+        v[i] = d - borrow;
+        borrow = b1 | (d < borrow);
+    }
+}
+
+__device__ __forceinline__ void words_add(uint64_t (&v)[kExtRnsWords], const uint64_t (&w)[kExtRnsWords]) {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < (int)kExtRnsWords; ++i) {
+        const uint64_t s1 = v[i] + w[i];
+        const uint64_t c1 = s1 < w[i];
+        v[i] = s1 + carry;
+        carry = c1 | (v[i] < carry);
+    }
+}
+
+// One thread per (product, input polynomial, coefficient):
+// compose_inplace (primus_rns/src/base.rs:396-420) -- per limb the Shoup
+// product r_i * (Q/q_i)^-1 mod q_i times Q/q_i added, Q subtracted on a carry
+// or when the sum reaches Q --, init_value_carry (primus_decompose/src/
+// big_integer/basis.rs:307-331), the digits with their carries
+// (common.rs:141-152), and wrapping_decompose_small_values_inplace
+// (base.rs:201-236) into every limb.
+__global__ __launch_bounds__(256) void ext_rns_digits_kernel(ExtRnsArgs a) {
+    const uint64_t n = 1ull << a.log_n;
+    const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (gid >= a.count * 2 * n) return;
+    const uint64_t poly = gid >> a.log_n;  // product * 2 + input polynomial
+    const uint64_t k = gid & (n - 1);
+    const uint64_t *src = a.in + poly * a.L * n + k;
+    uint64_t v[kExtRnsWords] = {0, 0, 0, 0};
+    uint64_t Q[kExtRnsWords];
+#pragma unroll
+    for (int w = 0; w < (int)kExtRnsWords; ++w) Q[w] = a.Q[w];
+    for (uint32_t i = 0; i < a.L; ++i) {
+        const uint64_t prod = shoup_full(a.inv[i], a.inv_q[i], src[(uint64_t)i * n], a.q[i]);
+        uint64_t carry = 0;
+#pragma unroll
+        for (int w = 0; w < (int)kExtRnsWords; ++w) {
+            // prod * punct + v + carry < 2^128: the high word takes both carries
+            const uint64_t lo = prod * a.punct[i][w], hi = mulhi64(prod, a.punct[i][w]);
+            const uint64_t s1 = v[w] + lo;
+            uint64_t c = s1 < lo;
+            const uint64_t s2 = s1 + carry;
+            c += s2 < carry;
+            v[w] = s2;
+            carry = hi + c;
+        }
+        if (carry != 0 || words_ge(v, Q)) words_sub(v, Q);
+    }
+    if (a.has_split) {
+        uint64_t sp[kExtRnsWords], ps[kExtRnsWords];
+#pragma unroll
+        for (int w = 0; w < (int)kExtRnsWords; ++w) {
+            sp[w] = a.split[w];
+            ps[w] = a.pow2_sub[w];
+        }
+        if (words_ge(v, sp)) words_add(v, ps);
+    }
+    uint64_t carry = 0;
+    if (a.drop_bits > 0) carry = (word_at(v, (a.drop_bits - 1) >> 6) >> ((a.drop_bits - 1) & 63)) & 1;
+    uint64_t *dst = a.digits + poly * a.l * a.L * n + k;
+    for (uint32_t d = 0; d < a.l; ++d) {
+        const uint32_t sh = a.drop_bits + d * a.log_basis;
+        const uint32_t wi = sh >> 6, off = sh & 63;
+        uint64_t bits = word_at(v, wi) >> off;
+        if (off + a.log_basis > 64 && wi + 1 < kExtRnsWords) bits |= word_at(v, wi + 1) << (64 - off);
+        const uint64_t temp = (bits & a.basis_minus_one) + carry;
+        carry = (temp & a.carry_mask) != 0;
+        const uint64_t digit = temp & a.basis_minus_one;
+        const bool low = a.basis == 2 || digit < a.half;
+        for (uint32_t i = 0; i < a.L; ++i) dst[((uint64_t)d * a.L + i) * n] = low ? digit : a.q[i] - a.basis + digit;
+    }
+}
+
+// One thread per output coefficient: sum over the input polynomials p and
+// digits d of digit_hat * key[p][d][c] mod q_i (add_dcrt_glwe_mul_dcrt_polynomial_assign,
+// primus_lattice/src/glwe/dcrt.rs:101-119), canonical.
+__global__ __launch_bounds__(256) void ext_rns_mac_kernel(ExtRnsMacArgs a) {
+    const uint64_t n = 1ull << a.log_n;
+    const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (gid >= a.count * 2 * a.L * n) return;
+    const uint64_t k = gid & (n - 1);
+    const uint64_t row = gid >> a.log_n;  // (product * 2 + c) * L + i
+    const uint32_t i = (uint32_t)(row % a.L);
+    const uint64_t pc = row / a.L;
+    const uint32_t c = (uint32_t)(pc & 1);
+    const uint64_t prod = pc >> 1;
+    const uint64_t q = a.tabs[i].q, r0 = a.tabs[i].br0, r1 = a.tabs[i].br1;
+    uint64_t acc = 0;
+    for (uint32_t p = 0; p < 2; ++p)
+        for (uint32_t d = 0; d < a.l; ++d) {
+            const uint64_t x = a.digits[(((prod * 2 + p) * a.l + d) * a.L + i) * n + k];
+            const uint64_t w = a.key[((((uint64_t)p * a.l + d) * 2 + c) * a.L + i) * n + k];
+            acc = add_mod(acc, barrett_mul(x, w, q, r0, r1), q);
+        }
+    a.out[gid] = acc;
+}
+
+hipError_t launch_ext_rns_digits(const ExtRnsArgs &a, hipStream_t s) {
+    const uint64_t threads = a.count * 2 * (1ull << a.log_n);
+    if (threads == 0) return hipSuccess;
+    ext_rns_digits_kernel<<<dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_rns_mac(const ExtRnsMacArgs &a, hipStream_t s) {
+    const uint64_t threads = a.count * 2 * a.L * (1ull << a.log_n);
+    if (threads == 0) return hipSuccess;
+    ext_rns_mac_kernel<<<dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // RLWE encryption
 // ---------------------------------------------------------------------------
 

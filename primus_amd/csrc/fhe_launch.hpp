@@ -34,6 +34,39 @@ struct ExtArgs {
 
 hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64_t count, hipStream_t s);
 
+// External product over several limbs (CrtGlwe::mul_dcrt_ggsw_inplace): the
+// gadget basis is over Q = q_0 ... q_{L-1}, at most kExtRnsMaxLimbs limbs, so
+// Q and every constant below fit kExtRnsWords little-endian words.
+constexpr uint32_t kExtRnsMaxLimbs = 4;
+constexpr uint32_t kExtRnsWords = 4;
+
+// The digit pass: each coefficient's L residues composed into one integer
+// below Q, its gadget digits, and each digit's residue per limb.
+struct ExtRnsArgs {
+    const uint64_t *in;  // [count][2][L][N] residues
+    uint64_t *digits;    // [count][2][l][L][N] digit residues
+    uint64_t q[kExtRnsMaxLimbs];
+    uint64_t inv[kExtRnsMaxLimbs], inv_q[kExtRnsMaxLimbs];  // (Q/q_i)^-1 mod q_i as Shoup pairs
+    uint64_t punct[kExtRnsMaxLimbs][kExtRnsWords];          // Q/q_i
+    uint64_t Q[kExtRnsWords], split[kExtRnsWords], pow2_sub[kExtRnsWords];
+    uint64_t basis, basis_minus_one, carry_mask, half;
+    uint64_t count;
+    uint32_t L, l, log_basis, drop_bits, has_split, log_n;
+};
+
+// The multiply-accumulate of the transformed digits with the key.
+struct ExtRnsMacArgs {
+    const uint64_t *digits;  // [count][2][l][L][N], NTT form
+    const uint64_t *key;     // [2][l][2][L][N], NTT form, canonical
+    uint64_t *out;           // [count][2][L][N]
+    const DevTable *tabs;    // L tables
+    uint64_t count;
+    uint32_t L, l, log_n;
+};
+
+hipError_t launch_ext_rns_digits(const ExtRnsArgs &a, hipStream_t s);
+hipError_t launch_ext_rns_mac(const ExtRnsMacArgs &a, hipStream_t s);
+
 // RLWE encryption: ring degrees 2^10 .. 2^16 in one persistent launch.
 constexpr uint32_t kRlweMinLogN = 10;
 constexpr uint32_t kRlweMaxLogN = 16;

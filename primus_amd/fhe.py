@@ -52,6 +52,35 @@ class ExtKey:
         _check(lib().ph_ext_product_device(self._h, d_in, d_out, count, int(bool(ntt_output)), _stream(stream)))
 
 
+class ExtKeyRns:
+    """Device-resident DCRT GGSW key [2][l][2][L][N] (NTT form) over a
+    UintCrtNttTable's L limbs, with the gadget basis over their product Q
+    (CrtGlwe::mul_dcrt_ggsw_inplace, primus_lattice/src/glwe/crt.rs:201-229)."""
+
+    def __init__(self, table, log_basis, decompose_length, key_ntt):
+        k = np.ascontiguousarray(key_ntt, dtype=np.uint64)
+        self.L, self.n = table.moduli_count(), table.poly_length()
+        assert k.size == 4 * decompose_length * self.L * self.n
+        h = ctypes.c_void_p()
+        _check(lib().ph_ext_key_rns_new(table.handle, log_basis, decompose_length, k.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.byref(h)))
+        self._h = h
+        self.table = table  # keeps the table alive
+        self.l = decompose_length
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().ph_ext_key_rns_free(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def product_device(self, d_in, d_out, count, ntt_output=False, stream=None):
+        """[count][2][L][N] residues in, [count][2][L][N] out (device pointers)."""
+        _check(lib().ph_ext_product_rns_device(self._h, d_in, d_out, count, int(bool(ntt_output)), _stream(stream)))
+
+
 class RlweRequest(ctypes.Structure):
     """ph_rlwe_request: one request of an uneven encryption batch."""
 
