@@ -301,6 +301,37 @@ def secondary_ext_product(pa, timer, sp):
             "rotating_buffers": rot, "arith": "f64"}
 
 
+def secondary_ext_product_rns(pa, timer, sp):
+    """External product over 2 limbs of 59-bit primes (Q of 118 bits): 1024
+    products, N=1024, base 2^8, l=3 (the top 24 bits of Q), one digit pass,
+    the batched digit transforms, one multiply-accumulate pass, the inverse."""
+    from primus_amd.fhe import ExtKeyRns
+
+    log_n, L, l, count = 10, 2, 3, 1024
+    n = 1 << log_n
+    moduli = pa.ntt_primes_below(59, 2 * n, L)
+    table = pa.UintCrtNttTable.new(log_n, moduli)
+    rng = np.random.default_rng(SEED + 8)
+    qcol = np.array(moduli, dtype=np.uint64).reshape(-1, 1)
+    key = ExtKeyRns(table, 8, l, np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol
+                                           for _ in range(4 * l)]).reshape(2, l, 2, L, n))
+    x = np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol for _ in range(2 * count)])
+    rot = 4  # 4 x 64 MiB in + out
+    pairs = [(pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(count * 2 * L * n)) for _ in range(rot)]
+
+    def step(i):
+        din, dout = pairs[i % rot]
+        key.product_device(din.ptr, dout.ptr, count, stream=sp)
+
+    per, tot = timer.run(step, 200, 10)
+    ms = tot / 200
+    return {"workload": "1024 external products over 2 limbs (59-bit primes), N=1024, B=2^8, l=3: digit pass, "
+                        "digit transforms, multiply-accumulate, inverse",
+            "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+            "ntt_per_s": (2 * l + 2) * L * count / (ms * 1e-3),
+            "algorithmic_gbps": 4 * L * n * 8 * count / (ms * 1e-3) / 1e9, "rotating_buffers": rot}
+
+
 def secondary_automorphism(pa, timer, sp):
     """NTT-domain automorphism x -> x^5 on the headline batch shape (1024 limb-polys, N=4096)."""
     moduli = pa.ntt_primes_below(50, 2 * N, LIMBS)
@@ -734,6 +765,7 @@ def main():
             "plain_n1024": secondary_plain_products(pa, timer, sp),
             "rlwe_uneven": secondary_rlwe(pa, timer, sp),
             "ext_product": secondary_ext_product(pa, timer, sp),
+            "ext_product_rns": secondary_ext_product_rns(pa, timer, sp),
             "automorphism": secondary_automorphism(pa, timer, sp),
             "base_conversion": secondary_base_conversion(pa, timer, sp),
         }
