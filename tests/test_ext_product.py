@@ -10,7 +10,7 @@ import pytest
 
 from oracle import oracle as O
 
-CFG4_Q = 4294957057  # largest prime < 2^32 with q = 1 mod 2048
+EXT_Q32 = 4294957057  # largest prime < 2^32 with q = 1 mod 2048
 
 
 def _basis_reference(q, log_basis, l):
@@ -30,7 +30,7 @@ def _basis_reference(q, log_basis, l):
     return drop, split, (1 << bits) - q
 
 
-@pytest.mark.parametrize("q,log_basis,l", [(CFG4_Q, 8, 3), (CFG4_Q, 8, 4), (CFG4_Q, 4, 5), (132120577, 7, 3),
+@pytest.mark.parametrize("q,log_basis,l", [(EXT_Q32, 8, 3), (EXT_Q32, 8, 4), (EXT_Q32, 4, 5), (132120577, 7, 3),
                                            (1125899906826241, 10, 4), (576460752303421441, 20, 2),
                                            (12289, 1, 13), (12289, 1, 10)])
 def test_basis_constants(q, log_basis, l):
@@ -43,7 +43,7 @@ def test_basis_constants(q, log_basis, l):
     assert b.carry_mask == (2 if log_basis == 1 else (1 << log_basis) | (1 << (log_basis - 1)))
 
 
-@pytest.mark.parametrize("q,log_basis,l", [(CFG4_Q, 8, 3), (CFG4_Q, 4, 6), (132120577, 7, 3),
+@pytest.mark.parametrize("q,log_basis,l", [(EXT_Q32, 8, 3), (EXT_Q32, 4, 6), (132120577, 7, 3),
                                            (1125899906826241, 10, 4), (576460752303421441, 20, 2)])
 def test_decomposition_reconstructs(q, log_basis, l):
     """sum_l s_l 2^(drop + l*B) = centred(v) + delta with |delta| <= 2^(drop-1), digits |s_l| <= 2^(B-1)."""
@@ -79,7 +79,7 @@ def _negacyclic(a, b, q):
     return np.array([x % q for x in c], dtype=np.uint64)
 
 
-@pytest.mark.parametrize("q,log_n,log_basis,l", [(12289, 4, 4, 3), (132120577, 5, 7, 3), (CFG4_Q, 5, 8, 3)])
+@pytest.mark.parametrize("q,log_n,log_basis,l", [(12289, 4, 4, 3), (132120577, 5, 7, 3), (EXT_Q32, 5, 8, 3)])
 def test_oracle_ext_product_schoolbook(q, log_n, log_basis, l):
     n = 1 << log_n
     t = O.Table(log_n, q)
@@ -106,8 +106,8 @@ def test_oracle_ext_product_schoolbook(q, log_n, log_basis, l):
 @pytest.mark.gpu
 @pytest.mark.parametrize("integer_only", [False, True])
 @pytest.mark.parametrize("q,log_n,log_basis,l,count", [
-    (CFG4_Q, 10, 8, 3, 2048),                 # the configuration's full batch
-    (CFG4_Q, 10, 8, 4, 7),
+    (EXT_Q32, 10, 8, 3, 2048),                 # the full batch of 2048 products
+    (EXT_Q32, 10, 8, 4, 7),
     (132120577, 8, 7, 3, 5),
     (1125899906826241, 12, 10, 4, 3),
     (576460752303421441, 10, 20, 2, 4),       # > 2^50: integer path either way
@@ -141,7 +141,7 @@ def test_gpu_ext_product_device_batch_properties():
     import primus_amd as pa
     from primus_amd.fhe import ExtKey
 
-    q, log_n, log_basis, l, count = CFG4_Q, 10, 8, 3, 2048
+    q, log_n, log_basis, l, count = EXT_Q32, 10, 8, 3, 2048
     n = 1 << log_n
     tab = pa.UintNttTable.new(log_n, q)
     rng = np.random.default_rng(44)

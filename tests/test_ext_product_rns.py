@@ -87,7 +87,7 @@ def test_rns_ext_product_one_limb_equals_c_oracle():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("L,bits,log_b,l,log_n,count", [
-    (1, 32, 8, 3, 10, 3),     # one limb: the cfg 4 modulus
+    (1, 32, 8, 3, 10, 3),     # one limb: the 32-bit external-product modulus
     (2, 59, 8, 3, 10, 3),     # approximate digits, top 24 bits
     (2, 59, 8, 14, 10, 2),    # every digit
     (3, 50, 16, 9, 11, 2),    # binary64 tables

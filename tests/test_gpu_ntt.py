@@ -13,7 +13,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-CFG2_PRIMES = [1125899906826241, 1125899906629633, 1125899906424833, 1125899906260993]
+BATCH_PRIMES_50 = [1125899906826241, 1125899906629633, 1125899906424833, 1125899906260993]
 
 
 def _rand(q, shape, seed):
@@ -129,11 +129,11 @@ def test_slice_length_checked():
 
 def test_dcrt_slices():
     log_n = 12
-    tab = pa.UintCrtNttTable.new(log_n, CFG2_PRIMES)
+    tab = pa.UintCrtNttTable.new(log_n, BATCH_PRIMES_50)
     assert tab.moduli_count() == 4 and tab.crt_poly_length() == 4 << log_n
-    x = np.concatenate([_rand(q, 1 << log_n, i) for i, q in enumerate(CFG2_PRIMES)])
+    x = np.concatenate([_rand(q, 1 << log_n, i) for i, q in enumerate(BATCH_PRIMES_50)])
     want = np.concatenate([O.Table(log_n, q).transform(x[i << log_n:(i + 1) << log_n])
-                           for i, q in enumerate(CFG2_PRIMES)])
+                           for i, q in enumerate(BATCH_PRIMES_50)])
     y = x.copy()
     tab.transform_slice(y)
     np.testing.assert_array_equal(y, want)
