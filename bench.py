@@ -639,11 +639,11 @@ def main():
                           "launches": K + args.warmup, "calibration_copy_bytes": src.numel() * 8}))
         return 0
 
-    # pipelined: consecutive batches on 2 and 3 streams (ROTATE is a multiple of
-    # both, so a buffer is only ever reused on the stream that last wrote it)
+    # pipelined: consecutive batches on 2, 3 and 4 streams (ROTATE is a multiple
+    # of each, so a buffer is only ever reused on the stream that last wrote it)
     pipelined = {}
-    extra = [torch.cuda.Stream(device=dev) for _ in range(2)]
-    for S in (2, 3):
+    extra = [torch.cuda.Stream(device=dev) for _ in range(3)]
+    for S in (2, 3, 4):
         streams = [stream] + extra[:S - 1]
         steps = [launcher(table, op, ctypes.c_void_p(s.cuda_stream)) for s in streams]
         pipelined[S] = timer.run_pipelined(steps, streams, K, args.warmup)
