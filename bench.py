@@ -567,7 +567,7 @@ def main():
         # copies): HIP's default of 4 makes streams share queues, and a copy
         # then waits behind another stream's event wait.  Read when HIP starts,
         # which has not happened yet in this process.
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(8, args.serving_streams + 4)))
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(8, args.serving_streams + 4))))
         return serving_main(args)
 
     import torch
