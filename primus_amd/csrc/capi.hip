@@ -1104,7 +1104,7 @@ Words words_shl_any(Words v, uint32_t s) {
 struct ph_ext_key_rns {
     const ph_dcrt_table *table;  // not owned
     ExtRnsArgs proto;
-    uint64_t *d_key = nullptr;
+    Shoup *d_key = nullptr;  // the key with Shoup quotients
 };
 
 int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t decompose_length, const uint64_t *key_ntt,
@@ -1201,11 +1201,16 @@ int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t deco
         for (uint64_t k = 0; k < n; ++k)
             if (key_ntt[r * n + k] >= q) return fail(PH_ERR_INVALID, "key values must be below their limb modulus");
     }
+    std::vector<Shoup> hk(words);
+    for (uint64_t r = 0; r < 4ull * decompose_length * d->L; ++r) {
+        const uint64_t q = d->tables[r % d->L]->q;
+        for (uint64_t k = 0; k < n; ++k) hk[r * n + k] = mk(key_ntt[r * n + k], q);
+    }
     ph_ext_key_rns *key = new (std::nothrow) ph_ext_key_rns();
     if (!key) return fail(PH_ERR_NO_MEMORY, "key");
     key->table = d;
-    if (hipMalloc(&key->d_key, words * 8) != hipSuccess ||
-        hipMemcpy(key->d_key, key_ntt, words * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMalloc(&key->d_key, words * sizeof(Shoup)) != hipSuccess ||
+        hipMemcpy(key->d_key, hk.data(), words * sizeof(Shoup), hipMemcpyHostToDevice) != hipSuccess) {
         if (key->d_key) (void)hipFree(key->d_key);
         delete key;
         return fail(PH_ERR_DEVICE, "key upload");

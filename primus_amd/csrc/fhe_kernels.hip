@@ -288,13 +288,13 @@ __global__ __launch_bounds__(256) void ext_rns_mac_kernel(ExtRnsMacArgs a) {
     const uint64_t pc = row / a.L;
     const uint32_t c = (uint32_t)(pc & 1);
     const uint64_t prod = pc >> 1;
-    const uint64_t q = a.tabs[i].q, r0 = a.tabs[i].br0, r1 = a.tabs[i].br1;
+    const uint64_t q = a.tabs[i].q;
     uint64_t acc = 0;
     for (uint32_t p = 0; p < 2; ++p)
         for (uint32_t d = 0; d < a.l; ++d) {
             const uint64_t x = a.digits[(((prod * 2 + p) * a.l + d) * a.L + i) * n + k];
-            const uint64_t w = a.key[((((uint64_t)p * a.l + d) * 2 + c) * a.L + i) * n + k];
-            acc = add_mod(acc, barrett_mul(x, w, q, r0, r1), q);
+            const Shoup w = a.key[((((uint64_t)p * a.l + d) * 2 + c) * a.L + i) * n + k];
+            acc = add_mod(acc, shoup_full(w.w, w.wq, x, q), q);
         }
     a.out[gid] = acc;
 }

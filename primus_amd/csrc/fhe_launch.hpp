@@ -57,7 +57,7 @@ struct ExtRnsArgs {
 // The multiply-accumulate of the transformed digits with the key.
 struct ExtRnsMacArgs {
     const uint64_t *digits;  // [count][2][l][L][N], NTT form
-    const uint64_t *key;     // [2][l][2][L][N], NTT form, canonical
+    const Shoup *key;        // [2][l][2][L][N], NTT form, canonical, with Shoup quotients
     uint64_t *out;           // [count][2][L][N]
     const DevTable *tabs;    // L tables
     uint64_t count;
