@@ -275,28 +275,30 @@ __global__ __launch_bounds__(256) void ext_rns_digits_kernel(ExtRnsArgs a) {
     }
 }
 
-// One thread per output coefficient: sum over the input polynomials p and
-// digits d of digit_hat * key[p][d][c] mod q_i (add_dcrt_glwe_mul_dcrt_polynomial_assign,
-// primus_lattice/src/glwe/dcrt.rs:101-119), canonical.
+// One thread per (product, limb, coefficient), both output polynomials c: sum
+// over the input polynomials p and digits d of digit_hat * key[p][d][c] mod q_i
+// (add_dcrt_glwe_mul_dcrt_polynomial_assign, primus_lattice/src/glwe/dcrt.rs:101-119),
+// canonical; each digit is read once for both.
 __global__ __launch_bounds__(256) void ext_rns_mac_kernel(ExtRnsMacArgs a) {
     const uint64_t n = 1ull << a.log_n;
     const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    if (gid >= a.count * 2 * a.L * n) return;
+    if (gid >= a.count * a.L * n) return;
     const uint64_t k = gid & (n - 1);
-    const uint64_t row = gid >> a.log_n;  // (product * 2 + c) * L + i
+    const uint64_t row = gid >> a.log_n;  // product * L + i
     const uint32_t i = (uint32_t)(row % a.L);
-    const uint64_t pc = row / a.L;
-    const uint32_t c = (uint32_t)(pc & 1);
-    const uint64_t prod = pc >> 1;
+    const uint64_t prod = row / a.L;
     const uint64_t q = a.tabs[i].q;
-    uint64_t acc = 0;
+    uint64_t acc0 = 0, acc1 = 0;
     for (uint32_t p = 0; p < 2; ++p)
         for (uint32_t d = 0; d < a.l; ++d) {
             const uint64_t x = a.digits[(((prod * 2 + p) * a.l + d) * a.L + i) * n + k];
-            const Shoup w = a.key[((((uint64_t)p * a.l + d) * 2 + c) * a.L + i) * n + k];
-            acc = add_mod(acc, shoup_full(w.w, w.wq, x, q), q);
+            const uint64_t kr = (((uint64_t)p * a.l + d) * 2) * a.L + i;  // key row of c = 0
+            const Shoup w0 = a.key[kr * n + k], w1 = a.key[(kr + a.L) * n + k];
+            acc0 = add_mod(acc0, shoup_full(w0.w, w0.wq, x, q), q);
+            acc1 = add_mod(acc1, shoup_full(w1.w, w1.wq, x, q), q);
         }
-    a.out[gid] = acc;
+    a.out[((prod * 2) * a.L + i) * n + k] = acc0;
+    a.out[((prod * 2 + 1) * a.L + i) * n + k] = acc1;
 }
 
 hipError_t launch_ext_rns_digits(const ExtRnsArgs &a, hipStream_t s) {
@@ -307,7 +309,7 @@ hipError_t launch_ext_rns_digits(const ExtRnsArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_ext_rns_mac(const ExtRnsMacArgs &a, hipStream_t s) {
-    const uint64_t threads = a.count * 2 * a.L * (1ull << a.log_n);
+    const uint64_t threads = a.count * a.L * (1ull << a.log_n);
     if (threads == 0) return hipSuccess;
     ext_rns_mac_kernel<<<dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s>>>(a);
     return hipGetLastError();
