@@ -318,6 +318,11 @@ def secondary_ext_product_rns(pa, timer, sp):
     x = np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol for _ in range(2 * count)])
     rot = 4  # 4 x 64 MiB in + out
     pairs = [(pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(count * 2 * L * n)) for _ in range(rot)]
+    # the launch shapes of its digit transforms and inverse transforms, tuned on
+    # this batch's polynomial counts (outputs do not depend on them)
+    from primus_amd import tuning
+    shapes = {op: tuning.autotune(table, op, cnt, reps=5)[0]
+              for op, cnt in (("forward", count * 2 * l * L), ("inverse", count * 2 * L))}
 
     def step(i):
         din, dout = pairs[i % rot]
@@ -329,7 +334,8 @@ def secondary_ext_product_rns(pa, timer, sp):
                         "digit transforms, multiply-accumulate, inverse",
             "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
             "ntt_per_s": (2 * l + 2) * L * count / (ms * 1e-3),
-            "algorithmic_gbps": 4 * L * n * 8 * count / (ms * 1e-3) / 1e9, "rotating_buffers": rot}
+            "algorithmic_gbps": 4 * L * n * 8 * count / (ms * 1e-3) / 1e9, "rotating_buffers": rot,
+            "transform_shapes": {op: tuning.describe(v) for op, v in shapes.items()}}
 
 
 def secondary_automorphism(pa, timer, sp):
