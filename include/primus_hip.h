@@ -49,6 +49,7 @@ extern "C" {
 #define PH_ERR_DEVICE 6            /* a HIP call failed */
 #define PH_ERR_UNSUPPORTED 7       /* ring degree outside 2^1 .. 2^17 */
 #define PH_ERR_NO_MEMORY 8         /* host or device allocation failed */
+#define PH_ERR_CANCELLED 9         /* serving: the request was cancelled before it was launched */
 
 /* Table creation flags.  By default canonical-output transforms over moduli
  * q < 2^50 run on exact binary64 FMA arithmetic (full-rate on gfx950) and all
@@ -384,7 +385,7 @@ typedef struct {
 } ph_server_config;
 
 typedef struct {
-    uint64_t completed, batches, launches, peak_inflight_bytes;
+    uint64_t completed, batches, launches, peak_inflight_bytes, cancelled;
     /* host seconds, summed: inside ph_server_submit; the dispatcher enqueueing
      * batches, and waiting for a free slot; the completion thread waiting for
      * batches to finish on the device */
@@ -395,7 +396,9 @@ typedef struct ph_server ph_server;
 
 /* Builds the resident tables and starts the dispatcher and completion threads. */
 int ph_server_new(const ph_server_config *config, ph_server **out);
-/* Drains outstanding work and stops. */
+/* Stops taking requests, launches every pending one, waits until all
+ * in-flight batches have completed and their results are in the callers'
+ * buffers, then frees the server. */
 void ph_server_free(ph_server *server);
 /* Queues a request; host buffers must stay valid until it completes.  Rejects
  * malformed requests (PH_ERR_INVALID) and ones larger than the budget
@@ -406,6 +409,17 @@ int ph_server_wait(ph_server *server, uint64_t ticket, int *status);
 /* Seconds from submission to completion of a finished ticket. */
 int ph_server_latency(ph_server *server, uint64_t ticket, double *seconds);
 int ph_server_drain(ph_server *server);
+/* Cancels a request that is still queued: its ticket completes at once with
+ * PH_ERR_CANCELLED and its buffers are never touched.  PH_ERR_INVALID if it
+ * is already in a batch or finished. */
+int ph_server_cancel(ph_server *server, uint64_t ticket);
+/* paused != 0: the dispatcher forms no new batch (submissions queue, batches
+ * already launched complete); 0 resumes.  Freeing the server resumes it. */
+int ph_server_pause(ph_server *server, int paused);
+/* Failure injection for tests of error handling: the next `batches` batches
+ * are retired with PH_ERR_DEVICE without launching anything, as if a HIP call
+ * of theirs had failed; later batches run normally. */
+int ph_server_inject_fault(ph_server *server, uint32_t batches);
 int ph_server_stats_get(ph_server *server, ph_server_stats *out);
 /* Open-loop load: submits requests[i] at Poisson arrival times of the given
  * rate, waits for all, reports per-request latency and the elapsed time. */

@@ -281,6 +281,9 @@ class RnsBasis:
         bits = Q.bit_length()
         if log_basis <= 0 or log_basis >= 64 or l <= 0 or bits // log_basis < l:
             raise ValueError("gadget basis does not fit the modulus")  # basis.rs:44-59
+        if any(q <= (1 << log_basis) for q in self.moduli):
+            # digits d >= B/2 become q_i - B + d (primus_rns/src/base.rs:209-213)
+            raise ValueError("every limb modulus must exceed the gadget basis")
         self.log_basis, self.l, self.bits = log_basis, l, bits
         self.drop_bits = bits - l * log_basis
         self.basis = 1 << log_basis

@@ -445,6 +445,7 @@ const char *ph_status_string(int status) {
         case PH_ERR_DEVICE: return "device error";
         case PH_ERR_UNSUPPORTED: return "unsupported ring degree";
         case PH_ERR_NO_MEMORY: return "out of memory";
+        case PH_ERR_CANCELLED: return "cancelled";
     }
     return "unknown status";
 }
@@ -1136,6 +1137,12 @@ int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t deco
     const uint32_t bits = words_bits(Q);
     if (log_basis == 0 || log_basis >= 64 || decompose_length == 0 || bits / log_basis < decompose_length)
         return fail(PH_ERR_INVALID, "gadget basis does not fit the modulus");
+    // wrapping_decompose_small_values_inplace maps a digit d >= B/2 to
+    // q_i - B + d, which needs every limb modulus above the basis
+    // (primus_rns/src/base.rs:209-213)
+    for (uint32_t i = 0; i < d->L; ++i)
+        if (d->tables[i]->q <= (1ull << log_basis))
+            return fail(PH_ERR_INVALID, "every limb modulus must exceed the gadget basis 2^log_basis");
     g.l = decompose_length;
     g.log_basis = log_basis;
     g.drop_bits = bits - decompose_length * log_basis;
@@ -1712,6 +1719,8 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
     ph_operand op_dev{};
     float best = 1e30f;
     int best_shape = 0;
+    // the shape in force before the trials: put back if they fail part-way
+    const int prior_shape = g_tuning[d->log_n][op][f64 ? 1 : 0].load();
     // 16 / 8 coefficients per thread, each as one workgroup per polynomial
     // and as the persistent streaming kernel (forward-first operations)
     const int candidates[4] = {4, 3, 4 | kShapeStream, 3 | kShapeStream};
@@ -1781,6 +1790,7 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
     if (chosen) *chosen = best_shape;
     if (best_ms) *best_ms = best;
 done:
+    if (rc != PH_OK) g_tuning[d->log_n][op][f64 ? 1 : 0].store(prior_shape);
     if (op_dev.d) (void)hipFree(op_dev.d);
     for (ShoupF *p : op_dev.d_f)
         if (p) (void)hipFree(p);

@@ -194,7 +194,15 @@ struct ph_server {
     std::deque<Ticket> tickets;
     std::deque<Batch> inflight;
     uint64_t inflight_bytes = 0;
+    // batches taken off the pending queue whose launch is being enqueued
+    // (outside the lock) and that are not in `inflight` yet
+    uint32_t launching = 0;
     bool stopping = false;
+    bool dispatcher_done = false;  // set by the dispatcher as it exits
+    // host-forced failures: the next fault_batches batches are completed with
+    // PH_ERR_DEVICE without being launched
+    uint32_t fault_batches = 0;
+    bool paused = false;  // the dispatcher takes no new batch while set
     std::thread dispatcher, completer;
 
     ph_server_stats stats{};
@@ -571,8 +579,12 @@ int launch_batch(ph_server *s, Batch &b, const std::vector<Ticket *> &tk) {
 void dispatcher_loop(ph_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        s->cv_pending.wait(lk, [&] { return s->stopping || !s->pending.empty(); });
-        if (s->pending.empty() && s->stopping) return;
+        s->cv_pending.wait(lk, [&] { return s->stopping || (!s->pending.empty() && !s->paused); });
+        if (s->pending.empty() && s->stopping) {
+            s->dispatcher_done = true;
+            s->cv_done.notify_all();  // the completer exits once in-flight work is retired
+            return;
+        }
         // a free slot
         int slot = -1;
         const auto w0 = Clock::now();
@@ -621,12 +633,18 @@ void dispatcher_loop(ph_server *s) {
         s->inflight_bytes += b.budget_bytes;
         s->stats.peak_inflight_bytes = std::max(s->stats.peak_inflight_bytes, s->inflight_bytes);
         (void)hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+        const bool fault = s->fault_batches > 0;
+        if (fault) --s->fault_batches;
+        ++s->launching;
         lk.unlock();
         const auto d0 = Clock::now();
-        b.status = launch_batch(s, b, tk);
+        // a forced fault enqueues nothing: the unrecorded event is complete
+        // and the batch retires with the error
+        b.status = fault ? PH_ERR_DEVICE : launch_batch(s, b, tk);
         const double took = std::chrono::duration<double>(Clock::now() - d0).count();
         lk.lock();
         s->stats.dispatch_s += took;
+        --s->launching;
         s->inflight.push_back(std::move(b));
         ++s->stats.batches;
         s->cv_done.notify_all();  // wakes the completer
@@ -636,9 +654,11 @@ void dispatcher_loop(ph_server *s) {
 void completer_loop(ph_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        s->cv_done.wait(lk, [&] { return !s->inflight.empty() || (s->stopping && s->pending.empty()); });
+        // exits only after the dispatcher has: a batch it is still enqueueing
+        // is in neither queue, and its results must still be retired
+        s->cv_done.wait(lk, [&] { return !s->inflight.empty() || s->dispatcher_done; });
         if (s->inflight.empty()) {
-            if (s->stopping && s->pending.empty()) return;
+            if (s->dispatcher_done) return;
             continue;
         }
         Batch &b = s->inflight.front();
@@ -827,7 +847,39 @@ int ph_server_latency(ph_server *s, uint64_t ticket, double *seconds) {
 int ph_server_drain(ph_server *s) {
     if (!s) return PH_ERR_INVALID;
     std::unique_lock<std::mutex> lk(s->mu);
-    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->inflight.empty(); });
+    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->launching == 0 && s->inflight.empty(); });
+    return PH_OK;
+}
+
+int ph_server_cancel(ph_server *s, uint64_t ticket) {
+    if (!s) return PH_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (ticket >= s->tickets.size()) return PH_ERR_INVALID;
+    auto it = std::find(s->pending.begin(), s->pending.end(), ticket);
+    if (it == s->pending.end()) return PH_ERR_INVALID;  // already in a batch, or finished
+    s->pending.erase(it);
+    Ticket &t = s->tickets[ticket];
+    t.status = PH_ERR_CANCELLED;
+    t.done_at = Clock::now();
+    ++s->stats.cancelled;
+    s->cv_done.notify_all();
+    return PH_OK;
+}
+
+int ph_server_pause(ph_server *s, int paused) {
+    if (!s) return PH_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->paused = paused != 0;
+    }
+    s->cv_pending.notify_all();
+    return PH_OK;
+}
+
+int ph_server_inject_fault(ph_server *s, uint32_t batches) {
+    if (!s) return PH_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->fault_batches = batches;
     return PH_OK;
 }
 
@@ -843,14 +895,14 @@ void ph_server_free(ph_server *s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stopping = true;
+        s->paused = false;
     }
     s->cv_pending.notify_all();
     s->cv_done.notify_all();
     s->cv_slot.notify_all();
+    // the dispatcher launches what is still pending and exits; the completer
+    // retires every in-flight batch after that, so all tickets complete
     if (s->dispatcher.joinable()) s->dispatcher.join();
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-    }
     s->cv_done.notify_all();
     if (s->completer.joinable()) s->completer.join();
     destroy(s);

@@ -18,7 +18,8 @@ class ExtKey:
     def __init__(self, table, log_basis, decompose_length, key_ntt):
         k = np.ascontiguousarray(key_ntt, dtype=np.uint64)
         n = table.poly_length()
-        assert k.size == 4 * decompose_length * n
+        if k.size != 4 * decompose_length * n:
+            raise ValueError("key must hold 2 x l x 2 x N values (%d), got %d" % (4 * decompose_length * n, k.size))
         h = ctypes.c_void_p()
         _check(lib().ph_ext_key_new(table.handle, log_basis, decompose_length, k.ctypes.data_as(_p64),
                                     ctypes.byref(h)))
@@ -60,7 +61,9 @@ class ExtKeyRns:
     def __init__(self, table, log_basis, decompose_length, key_ntt):
         k = np.ascontiguousarray(key_ntt, dtype=np.uint64)
         self.L, self.n = table.moduli_count(), table.poly_length()
-        assert k.size == 4 * decompose_length * self.L * self.n
+        want = 4 * decompose_length * self.L * self.n
+        if k.size != want:
+            raise ValueError("key must hold 2 x l x 2 x L x N values (%d), got %d" % (want, k.size))
         h = ctypes.c_void_p()
         _check(lib().ph_ext_key_rns_new(table.handle, log_basis, decompose_length, k.ctypes.data_as(ctypes.c_void_p),
                                         ctypes.byref(h)))

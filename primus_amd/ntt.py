@@ -271,7 +271,8 @@ class DeviceBuffer:
 
     def upload(self, a):
         a = np.ascontiguousarray(a, dtype=np.uint64)
-        assert a.size <= self.words
+        if a.size > self.words:
+            raise ValueError("upload of %d words into a buffer of %d" % (a.size, self.words))
         _check(lib().ph_memcpy_htod(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.size * 8))
 
     def download(self, shape=None):
@@ -296,7 +297,8 @@ class Operand:
 
     def __init__(self, table, values):
         vals = np.ascontiguousarray(values, dtype=np.uint64)
-        assert vals.size == table.crt_poly_length()
+        if vals.size != table.crt_poly_length():
+            raise ValueError("operand must hold L x N = %d values, got %d" % (table.crt_poly_length(), vals.size))
         h = ctypes.c_void_p()
         _check(lib().ph_operand_new(table.handle, vals.ctypes.data_as(_p64), ctypes.byref(h)))
         self._h = h

@@ -31,7 +31,7 @@ class ServerConfig(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("launches", ctypes.c_uint64),
-                ("peak_inflight_bytes", ctypes.c_uint64), ("submit_s", ctypes.c_double),
+                ("peak_inflight_bytes", ctypes.c_uint64), ("cancelled", ctypes.c_uint64), ("submit_s", ctypes.c_double),
                 ("dispatch_s", ctypes.c_double), ("slot_wait_s", ctypes.c_double), ("device_wait_s", ctypes.c_double)]
 
 
@@ -47,6 +47,9 @@ def _declare():
     L.ph_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
     L.ph_server_latency.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.ph_server_drain.argtypes = [_vp]
+    L.ph_server_cancel.argtypes = [_vp, ctypes.c_uint64]
+    L.ph_server_inject_fault.argtypes = [_vp, ctypes.c_uint32]
+    L.ph_server_pause.argtypes = [_vp, ctypes.c_int]
     L.ph_server_stats_get.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     L.ph_server_run_poisson.argtypes = [_vp, ctypes.POINTER(ServeRequest), ctypes.c_size_t, ctypes.c_double,
                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
@@ -56,7 +59,7 @@ def _declare():
     L.ph_host_free.argtypes = [_vp]
     L.ph_host_free.restype = ctypes.c_int
     for f in ("ph_server_submit", "ph_server_wait", "ph_server_latency", "ph_server_drain", "ph_server_stats_get",
-              "ph_server_run_poisson"):
+              "ph_server_run_poisson", "ph_server_cancel", "ph_server_inject_fault", "ph_server_pause"):
         getattr(L, f).restype = ctypes.c_int
     L._serving_declared = True
     return L
@@ -162,6 +165,22 @@ class Server:
 
     def drain(self):
         _check(lib().ph_server_drain(self._h))
+
+    def cancel(self, ticket):
+        """Cancels a still-queued request; True if it was cancelled (its status
+        is then PH_ERR_CANCELLED), False if it was already in a batch."""
+        rc = lib().ph_server_cancel(self._h, ticket)
+        if rc == 0:
+            self._live.pop(ticket, None)
+        return rc == 0
+
+    def pause(self, paused=True):
+        """While paused, submissions queue and no new batch is formed."""
+        _check(lib().ph_server_pause(self._h, int(bool(paused))))
+
+    def inject_fault(self, batches=1):
+        """The next `batches` batches complete with PH_ERR_DEVICE without running."""
+        _check(lib().ph_server_inject_fault(self._h, batches))
 
     def stats(self):
         st = ServerStats()

@@ -50,7 +50,8 @@ def test_rns_compose_is_crt(L, bits):
 
 
 @pytest.mark.parametrize("L,bits,log_b,l", [(2, 59, 8, 3), (2, 59, 8, 14), (3, 50, 16, 9), (4, 62, 20, 12),
-                                            (2, 31, 1, 61), (3, 40, 7, 2)])
+                                            (2, 31, 1, 61), (3, 40, 7, 2),
+                                            (2, 59, 2, 59), (2, 31, 1, 62)])  # drop_bits == 0
 def test_rns_digits_reconstruct(L, bits, log_b, l):
     moduli = _moduli(L, bits)
     b = O.RnsBasis(moduli, log_b, l)
@@ -94,6 +95,8 @@ def test_rns_ext_product_one_limb_equals_c_oracle():
     (4, 62, 20, 12, 10, 2),   # four words
     (2, 31, 1, 61, 8, 2),     # binary digits
     (2, 59, 8, 3, 14, 1),     # two-pass transforms
+    (2, 59, 2, 59, 8, 1),     # Q of 118 bits: every bit a digit bit (drop_bits 0, split + 1)
+    (2, 31, 1, 62, 8, 1),     # binary digits, drop_bits 0 (no split value)
 ])
 def test_gpu_ext_product_rns(L, bits, log_b, l, log_n, count):
     from primus_amd.fhe import ExtKeyRns
@@ -137,3 +140,16 @@ def test_gpu_ext_key_rns_rejects():
     bad[1, 2, 1, 1, 5] = moduli[1]
     with pytest.raises(ValueError):
         ExtKeyRns(t, 8, 3, bad)
+    with pytest.raises(ValueError):  # a short key array is refused before the library reads it
+        ExtKeyRns(t, 8, 3, np.zeros((2, 3, 2, 2, n - 1), dtype=np.uint64))
+    # 2^40 exceeds the 31-bit limb moduli: digits >= 2^39 have no residue q_i - B + d
+    small = pa.ntt_primes_below(31, 2048, 2)
+    with pytest.raises(ValueError):
+        ExtKeyRns(pa.UintCrtNttTable.new(10, small), 40, 1, np.zeros((2, 1, 2, 2, n), dtype=np.uint64))
+
+
+def test_rns_basis_rejects_basis_above_a_modulus():
+    small = pa.ntt_primes_below(31, 2048, 2)
+    with pytest.raises(ValueError):
+        O.RnsBasis(small, 40, 1)
+    O.RnsBasis(small, 30, 2)  # 2^30 is below both moduli

@@ -159,3 +159,111 @@ def test_server_rejects():
     ok = sv.Request(sv.EXT_PRODUCT, 10, 1, in0=np.zeros(2 << 10, dtype=np.uint64))
     assert srv.wait(srv.submit(ok)) == 0
     srv.close()
+
+
+def test_server_full_range_matches_oracle_and_alone():
+    """The serving workload's whole range: ring degrees up to 2^16 (two-pass
+    transforms, RLWE scratch) and 5-16 limbs in one mix, against the oracle
+    and against each request run alone."""
+    import primus_amd as pa
+    from primus_amd import serving as sv
+    from primus_amd.fhe import ExtKey
+    from test_rlwe import samples
+
+    grid = _grid(10, 16, 16)
+    rng = np.random.default_rng(123)
+    key_np = rng.integers(0, EXT_Q, size=(2, 3, 2, 1024), dtype=np.uint64)
+    key = ExtKey(pa.UintNttTable.new(10, EXT_Q), 8, 3, key_np)
+    shapes = [(sv.POLY_MUL, 15, 5), (sv.POLY_MUL, 16, 16), (sv.POLY_MUL, 16, 7), (sv.POLY_MUL, 15, 16),
+              (sv.RLWE_ENCRYPT, 16, 16), (sv.RLWE_ENCRYPT, 15, 12), (sv.RLWE_ENCRYPT, 16, 5),
+              (sv.POLY_MUL, 10, 9), (sv.RLWE_ENCRYPT, 11, 16), (sv.EXT_PRODUCT, 10, 1), (sv.POLY_MUL, 13, 14),
+              (sv.EXT_PRODUCT, 10, 1)]
+    reqs = []
+    for kind, ln, L in shapes:
+        n = 1 << ln
+        if kind == sv.EXT_PRODUCT:
+            reqs.append(sv.Request(kind, 10, 1, in0=rng.integers(0, EXT_Q, size=2048, dtype=np.uint64)))
+            continue
+        a = np.concatenate([rng.integers(0, grid[ln][j], size=n, dtype=np.uint64) for j in range(L)])
+        if kind == sv.POLY_MUL:
+            b = np.concatenate([rng.integers(0, grid[ln][j], size=n, dtype=np.uint64) for j in range(L)])
+            reqs.append(sv.Request(kind, ln, L, in0=a, in1=b))
+        else:
+            s, e, m = samples(n, rng)
+            reqs.append(sv.Request(kind, ln, L, in0=a, s=s, e=e, m=m))
+    print("full-range mix:", [(r.kind, r.log_n, r.limbs) for r in reqs], flush=True)
+    srv = sv.Server(grid, 10, 16, 16, ext_key=key, hbm_budget_bytes=8 << 30, max_batch=64, streams=2)
+    srv.pause()  # every request joins one batch
+    tickets = [srv.submit(r) for r in reqs]
+    srv.pause(False)
+    for t in tickets:
+        assert srv.wait(t) == 0
+    assert srv.stats()["batches"] == 1
+    mixed = [r.out.copy() for r in reqs]
+    srv.close()
+    for r, got in zip(reqs, mixed):
+        want = _expected(r, grid, key_np)
+        np.testing.assert_array_equal(got, want, err_msg="kind %d N=2^%d L=%d" % (r.kind, r.log_n, r.limbs))
+    alone = sv.Server(grid, 10, 16, 16, ext_key=key, hbm_budget_bytes=8 << 30, max_batch=1, streams=1,
+                      per_poly_launches=True)
+    for r, got in zip(reqs, mixed):
+        r.out[:] = 0
+        assert alone.wait(alone.submit(r)) == 0
+        np.testing.assert_array_equal(r.out, got)
+    alone.close()
+
+
+def test_server_cancel_queued_requests():
+    """Requests cancelled while queued complete at once with PH_ERR_CANCELLED
+    and their output buffers are never written; the rest run normally."""
+    srv, grid, key_np, key = _server(max_batch=16, streams=2)
+    reqs = _mix(grid, key_np, 9, seed=21, max_limbs=4)
+    srv.pause()
+    tickets = [srv.submit(r) for r in reqs]
+    cancelled = set(tickets[1::3])
+    for t in cancelled:
+        assert srv.cancel(t)
+    assert not srv.cancel(tickets[1])  # cancelling twice is refused
+    srv.pause(False)
+    for t, r in zip(tickets, reqs):
+        st = srv.wait(t)
+        if t in cancelled:
+            assert st == 9  # PH_ERR_CANCELLED
+            assert not r.out.any()
+        else:
+            assert st == 0
+            np.testing.assert_array_equal(r.out, _expected(r, grid, key_np))
+    assert not srv.cancel(tickets[0])  # finished
+    assert srv.stats()["cancelled"] == len(cancelled)
+    srv.close()
+
+
+def test_server_injected_fault_reports_per_request_status():
+    """A batch whose device work fails reports PH_ERR_DEVICE on every request
+    in it and leaves their outputs untouched; the server then goes on."""
+    srv, grid, key_np, key = _server(max_batch=16, streams=2)
+    reqs = _mix(grid, key_np, 6, seed=22, max_limbs=3)
+    srv.inject_fault(1)
+    srv.pause()
+    bad = [srv.submit(r) for r in reqs[:3]]
+    srv.pause(False)
+    for t, r in zip(bad, reqs[:3]):
+        assert srv.wait(t) == 6  # PH_ERR_DEVICE
+        assert not r.out.any()
+    good = [srv.submit(r) for r in reqs[3:]]
+    for t, r in zip(good, reqs[3:]):
+        assert srv.wait(t) == 0
+        np.testing.assert_array_equal(r.out, _expected(r, grid, key_np))
+    srv.close()
+
+
+def test_server_free_drains_outstanding_work():
+    """Freeing the server right after submitting still completes every
+    request, with its results in the caller's buffers."""
+    srv, grid, key_np, key = _server(max_batch=4, streams=2)
+    reqs = _mix(grid, key_np, 20, seed=23, max_limbs=4)
+    for r in reqs:
+        srv.submit(r)
+    srv.close()
+    for r in reqs:
+        np.testing.assert_array_equal(r.out, _expected(r, grid, key_np))
