@@ -53,7 +53,9 @@ def build(verbose=False):
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
     tmp = LIB_PATH + ".tmp"
-    res = subprocess.run([hipcc] + HIPCC_FLAGS + ["-pthread", "-o", tmp] + objs, capture_output=True, text=True)
+    # only the ph_* entry points of the header are exported (exports.map)
+    link = ["-pthread", "-Wl,--version-script=" + os.path.join(CSRC, "exports.map"), "-o", tmp]
+    res = subprocess.run([hipcc] + HIPCC_FLAGS + link + objs, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc link failed:\n" + res.stdout + res.stderr)
     os.replace(tmp, LIB_PATH)
@@ -140,6 +142,7 @@ def _declare(L):
         "ph_operand_free": (None, [vp]),
         "ph_dcrt_fwd_mul_inv_device": (i, [vp, vp, sz, vp, vp]),
         "ph_dcrt_negacyclic_mul_device": (i, [vp, vp, vp, sz, vp]),
+        "ph_dcrt_negacyclic_mul_device_noalloc": (i, [vp, vp, vp, sz, vp]),
         "ph_dcrt_negacyclic_mul_rows_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_hexl_compute_forward": (i, [vp, vp, sz, ctypes.c_uint64, ctypes.c_uint64]),
         "ph_ext_key_rns_new": (i, [vp, ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.POINTER(vp)]),

@@ -22,6 +22,32 @@ def test_every_header_function_is_exported():
     assert not missing, missing
 
 
+def _exported_functions(path):
+    """Defined function symbols of the library's dynamic symbol table."""
+    import shutil
+    import subprocess
+
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm is not installed")
+    out = subprocess.run([nm, "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    names = set()
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[1] in "TtWi":
+            names.add(parts[2].split("@")[0])
+    return names
+
+
+def test_exported_set_equals_the_header():
+    """Only the header's entry points leave the library: kernel stubs,
+    launchers and other internals are local (linker version script)."""
+    exported = _exported_functions(_lib.LIB_PATH)
+    declared = set(pa.header_functions())
+    assert exported == declared, {"not declared": sorted(exported - declared)[:20],
+                                  "not exported": sorted(declared - exported)[:20]}
+
+
 def test_status_strings():
     L = pa.lib()
     assert L.ph_status_string(0) == b"ok"
