@@ -188,6 +188,18 @@ int ph_dcrt_fwd_mul_inv_device(const ph_dcrt_table *table, uint64_t *d_polys, si
  * (primus_poly/src/poly/mul.rs:111-142). */
 int ph_dcrt_negacyclic_mul_device(const ph_dcrt_table *table, uint64_t *d_a, const uint64_t *d_b,
                                   size_t count, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ph_dcrt_negacyclic_mul_device with the scratch that
+ * ring degrees N >= 2^14 need (count * N words) taken from a fixed device
+ * array of PH_NOALLOC_SCRATCH_BYTES held by each GPU for the life of the
+ * process, so no call allocates or frees anything.  Larger batches are
+ * refused with PH_ERR_INVALID.  Every call on one GPU shares that GPU's
+ * array: calls whose work overlaps in time on one GPU (other streams, other
+ * threads) race and must be ordered by the caller. */
+#define PH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ph_dcrt_negacyclic_mul_device_noalloc(const ph_dcrt_table *table, uint64_t *d_a, const uint64_t *d_b,
+                                          size_t count, void *stream);
 /* The same product over limb-polynomials of uneven limb counts (no reference
  * counterpart: a batched form of the call above): row i of d_a and d_b is
  * reduced by the table's modulus d_limb_of[i] (< the table's limb count)

@@ -718,6 +718,62 @@ int ph_dcrt_negacyclic_mul_device(const ph_dcrt_table *d, uint64_t *d_a, const u
     return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of ph_dcrt_negacyclic_mul_device_noalloc: the column pass of the
+// second operand for two-pass ring degrees.  Every GPU the library runs on
+// holds its own copy for the life of the process.  In host code the name is
+// only a stand-in for the symbol, never a device address: kernels and copies
+// get the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchWords = PH_NOALLOC_SCRATCH_BYTES / 8;
+__device__ __align__(256) uint64_t scratch[kNoAllocScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up on the first call on
+// that device and kept.
+int noalloc_scratch(void **out) {
+    int dev = -1;
+    PH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(PH_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(PH_ERR_DEVICE, "no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ph_dcrt_negacyclic_mul_device_noalloc(const ph_dcrt_table *d, uint64_t *d_a, const uint64_t *d_b, size_t count,
+                                          void *stream) {
+    if (!d || ((!d_a || !d_b) && count)) return fail(PH_ERR_INVALID, "null argument");
+    if (count == 0) return PH_OK;
+    const uint64_t need = needs_scratch(Op::PolyMul, d->log_n) ? ((uint64_t)count << d->log_n) : 0;
+    if (need > kNoAllocScratchWords)
+        return fail(PH_ERR_INVALID, "count * N * 8 bytes exceeds the fixed scratch of PH_NOALLOC_SCRATCH_BYTES");
+    void *scr = nullptr;
+    if (need) {
+        const int rc = noalloc_scratch(&scr);
+        if (rc != PH_OK) return rc;
+    }
+    // every scratch word the column pass writes is read back by the chunk
+    // pass of the same launch sequence, so nothing needs clearing
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, (hipStream_t)stream,
+                  nullptr, scr);
+}
+
 int ph_dcrt_negacyclic_mul_rows_device(const ph_dcrt_table *d, uint64_t *d_a, const uint64_t *d_b,
                                        const uint8_t *d_limb_of, size_t count, void *stream) {
     if (!d || ((!d_a || !d_b || !d_limb_of) && count)) return fail(PH_ERR_INVALID, "null argument");
