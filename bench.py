@@ -80,12 +80,15 @@ def cpu_baseline(moduli, polys, operand, budget_s=8.0):
     }
 
 
+LATENCY_SAMPLES = 1000  # p50/p99 always come from at least this many single-step spans
+
+
 class Timer:
     """HIP events on the stream the kernels are launched on (torch's view of it).
 
-    run() times K steps twice: once bracketed by two events only (throughput:
-    nothing is queued between the launches), then once with an event after
-    every step (per-step latency for p50/p99)."""
+    run() times K steps bracketed by two events only (throughput: nothing is
+    queued between the launches), then max(K, LATENCY_SAMPLES) steps with an
+    event after every step (per-step latency for p50/p99)."""
 
     def __init__(self, torch, stream):
         self.torch, self.stream = torch, stream
@@ -101,22 +104,23 @@ class Timer:
         self.torch.cuda.synchronize()
         return e0.elapsed_time(e1)
 
-    def run(self, step, K, W, latency=True):
+    def run(self, step, K, W, latency=True, samples=None):
         for i in range(W):
             step(i)
         self.stream.synchronize()
         total = self._bracket(step, K)
         if not latency:
             return None, total
-        evs = [self.torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+        S = samples if samples is not None else max(K, LATENCY_SAMPLES)
+        evs = [self.torch.cuda.Event(enable_timing=True) for _ in range(S + 1)]
         self.torch.cuda.synchronize()
         evs[0].record(self.stream)
-        for i in range(K):
+        for i in range(S):
             step(i)
             evs[i + 1].record(self.stream)
         self.stream.synchronize()
         self.torch.cuda.synchronize()
-        per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(K)])
+        per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(S)])
         return per, total
 
     def run_pipelined(self, steps, streams, K, W):
@@ -468,29 +472,39 @@ def serving_main(args):
                         streams=args.serving_streams)
     plain = sv.Server(grid, 10, 16, 16, ext_key=key, hbm_budget_bytes=budget, max_batch=1, streams=1,
                       per_poly_launches=True)
-    # parity gate: the first requests of each kind against the oracle, through both servers
-    checked = 0
+    # parity gate: the first request of every (kind, ring degree) in the pool
+    # against the oracle, through both servers
+    gate = {}
     for r in pool:
-        if r.log_n != 10 or checked >= 3:
-            continue
+        gate.setdefault((r.kind, r.log_n), r)
+    for (kind, ln), r in sorted(gate.items()):
+        n = 1 << ln
+        if kind == sv.POLY_MUL:
+            want = []
+            for j in range(r.limbs):
+                a, b, q = r.in0[j * n:(j + 1) * n], r.in1[j * n:(j + 1) * n], grid[ln][j]
+                if ln <= 11:
+                    want.append(O.schoolbook(a, b, q))
+                else:  # the oracle's transforms, themselves pinned to schoolbook products
+                    t = O.Table(ln, q)
+                    want.append(t.inverse_transform(O.pointwise_mul(t.transform(a), t.transform(b), q)))
+            want = np.concatenate(want)
+        elif kind == sv.RLWE_ENCRYPT:
+            mods = grid[ln][:r.limbs]
+            want = np.concatenate([O.rlwe_encrypt(O.Table(ln, q), r.s, r.e, r.m, O.rlwe_delta(mods, q),
+                                                  r.in0[j * n:(j + 1) * n]) for j, q in enumerate(mods)])
+        else:
+            want = O.ext_product(O.Table(10, ext_q), O.Basis.new(ext_q, 8, 3), key_np,
+                                 r.in0.reshape(2, 1024)).ravel()
         for srv in (batched, plain):
             r.out[:] = 0
             srv.wait(srv.submit(r))
-            n = 1 << r.log_n
-            if r.kind == sv.POLY_MUL:
-                want = np.concatenate([O.schoolbook(r.in0[j * n:(j + 1) * n], r.in1[j * n:(j + 1) * n], grid[10][j])
-                                       for j in range(r.limbs)])
-            elif r.kind == sv.RLWE_ENCRYPT:
-                mods = grid[10][:r.limbs]
-                want = np.concatenate([O.rlwe_encrypt(O.Table(10, q), r.s, r.e, r.m, O.rlwe_delta(mods, q),
-                                                      r.in0[j * n:(j + 1) * n]) for j, q in enumerate(mods)])
-            else:
-                want = O.ext_product(O.Table(10, ext_q), O.Basis.new(ext_q, 8, 3), key_np,
-                                     r.in0.reshape(2, 1024)).ravel()
             if not np.array_equal(r.out, want):
-                print("bench: serving output differs from the oracle; refusing to time", file=sys.stderr)
+                print("bench: serving output (kind %d, N=2^%d, %d limbs) differs from the oracle; refusing to time"
+                      % (kind, ln, r.limbs), file=sys.stderr)
                 return 1
-        checked += 1
+    gate_checked = ["%s N=2^%d L=%d" % (("poly_mul", "rlwe_encrypt", "ext_product")[k], ln, r.limbs)
+                    for (k, ln), r in sorted(gate.items())]
 
     # the host link both paths share: page-locked 256 MiB copies each way
     link = {}
@@ -512,7 +526,8 @@ def serving_main(args):
                        "host buffers in and out (PCIe included; " +
                        ("pageable, staged by the server" if args.serving_pageable else "page-locked, copied directly") +
                        "); 64 GiB HBM admission budget",
-           "pool_distinct_requests": len(pool), "max_batch": args.serving_max_batch,
+           "pool_distinct_requests": len(pool), "parity_checked": gate_checked,
+           "max_batch": args.serving_max_batch,
            "stream_slots": args.serving_streams, "host_link": link,
            "mean_request_bytes": {"h2d": float(np.mean([request_bytes(r)[0] for r in pool])),
                                   "d2h": float(np.mean([request_bytes(r)[1] for r in pool]))},
@@ -521,10 +536,10 @@ def serving_main(args):
         count = int(min(args.serving_requests, rate * args.serving_seconds))
         b = run(batched, rate, count, seed=11 + k)
         p = run(plain, rate, int(min(count, args.serving_plain_requests)), seed=11 + k)
-        out["rates"].append({"batched": b, "plain": p})
+        out["rates"].append({"batched": b, "plain": p, "p99_ratio_plain_over_batched": p["p99_ms"] / b["p99_ms"]})
         print(json.dumps({"progress": "rate done", "rate": rate}), file=sys.stderr, flush=True)
     # saturation: everything offered at once, the highest throughput each path reaches
-    count = int(min(args.serving_requests, 20000))
+    count = int(min(args.serving_requests, args.serving_saturation_requests))
     b = run(batched, 1e7, count, seed=99)
     p = run(plain, 1e7, int(min(count, args.serving_plain_requests)), seed=99)
     out["saturation"] = {"batched": b, "plain": p,
@@ -533,6 +548,26 @@ def serving_main(args):
     plain.close()
     print(json.dumps(out))
     return 0
+
+
+def serving_leg():
+    """`bench.py --serving --serving-leg` in a child process; its JSON line, or
+    the failure."""
+    import subprocess
+
+    cmd = [sys.executable, os.path.abspath(__file__), "--serving-leg"]
+    print(json.dumps({"progress": "serving pass starts"}), file=sys.stderr, flush=True)
+    try:
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=400)
+    except subprocess.TimeoutExpired:
+        return {"error": "serving pass exceeded 400 s"}
+    sys.stderr.write(res.stderr[-2000:])
+    lines = [ln for ln in res.stdout.splitlines() if ln.startswith("{")]
+    if res.returncode != 0 or not lines:
+        return {"error": "serving pass exited with status %d" % res.returncode, "stderr": res.stderr[-500:]}
+    leg = json.loads(lines[-1])
+    leg["command"] = "python bench.py --serving-leg"
+    return leg
 
 
 def main():
@@ -562,12 +597,25 @@ def main():
                     help="stream slots of the batched server (batches in flight)")
     ap.add_argument("--serving-pageable", action="store_true",
                     help="requests in ordinary (pageable) host memory, staged by the server")
+    ap.add_argument("--serving-saturation-requests", type=int, default=20000)
+    ap.add_argument("--serving-leg", action="store_true",
+                    help="the bounded serving pass of the default run: 1k and 10k req/s and saturation, batched "
+                         "and plain on equal request counts (1000, 5000, 5000), a pool of 96 requests")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving pass of the default run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         return 2
 
+    if args.serving_leg:
+        args.serving = True
+        args.serving_rates = [1000.0, 10000.0]
+        args.serving_seconds = 1.0  # 1000 requests at 1k req/s, 5000 at 10k
+        args.serving_requests = 5000
+        args.serving_plain_requests = 5000
+        args.serving_saturation_requests = 5000
+        args.serving_pool = 96
     if args.serving:
         # one hardware queue per server stream (slots, input copies, result
         # copies): HIP's default of 4 makes streams share queues, and a copy
@@ -575,6 +623,13 @@ def main():
         # which has not happened yet in this process.
         os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(8, args.serving_streams + 4))))
         return serving_main(args)
+
+    # the serving pass runs first, in a child process of its own: it needs a
+    # hardware queue per server stream, which HIP reads when it starts, and
+    # this process has not touched the GPU yet
+    serving = None
+    if not (args.no_serving or args.no_secondary or args.profile):
+        serving = serving_leg()
 
     import torch
 
@@ -690,7 +745,8 @@ def main():
             L.ph_dcrt_fwd_mul_inv_device(limb_tabs[l].handle, ctypes.c_void_p(base + i * stride), 1,
                                          limb_ops[l].handle, sp)
 
-    plain_per, plain_total = timer.run(plain_step, args.plain_steps, 1)
+    # 1024 launches per step (about 11 ms): its latency pass is bounded
+    plain_per, plain_total = timer.run(plain_step, args.plain_steps, 1, samples=max(args.plain_steps, 100))
     plain_ms = plain_total / args.plain_steps
 
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated),
@@ -732,7 +788,11 @@ def main():
                        "alone: one launch at a time (%s)" % serial,
         "p50_ms": float(np.percentile(per, 50)),
         "p99_ms": float(np.percentile(per, 99)),
-        "latency_note": "p50/p99 of K single-batch spans between per-step HIP events on the launch stream",
+        "latency_samples": int(len(per)),
+        "latency_note": "p50/p99 of max(K, %d) single-batch spans between per-step HIP events on the launch "
+                        "stream" % LATENCY_SAMPLES,
+        "one_launch_ntt_per_s": NTT_PER_STEP / (kernel_ms * 1e-3),
+        "one_launch_note": "NTT/s of one batch at a time (roofline.kernel_ms); value may be the pipelined rate",
         "wall_s": wall,
         "plain": {
             "desc": "one launch per limb-polynomial (1024 launches per batch)",
@@ -740,6 +800,7 @@ def main():
             "ntt_per_s": NTT_PER_STEP / (plain_ms * 1e-3),
             "p50_ms": float(np.percentile(plain_per, 50)),
             "p99_ms": float(np.percentile(plain_per, 99)),
+            "latency_samples": int(len(plain_per)),
         },
         "batched_vs_plain": plain_ms / ms_per_step,
         "arith": table.arith(),
@@ -775,6 +836,8 @@ def main():
             "automorphism": secondary_automorphism(pa, timer, sp),
             "base_conversion": secondary_base_conversion(pa, timer, sp),
         }
+        if serving is not None:
+            out["secondary"]["serving"] = serving
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(moduli, polys, operand)
     print(json.dumps(out))

@@ -318,6 +318,9 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
         d.inv_n_p = in.wp;
         d.sr_f = sr.w;
         d.sr_p = sr.wp;
+        const ShoupF lastf = mkf(last, q);
+        d.last_f = lastf.w;
+        d.last_p = lastf.wp;
         d.fwd_f = t->d_fwd_f;
         d.inv_f = t->d_inv_f;
         for (int e = 0; e < 2; ++e) {
@@ -681,9 +684,13 @@ int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_opera
     op->n = n;
     int rc = upload((void **)&op->d, h.data(), h.size() * sizeof(Shoup));
     if (rc == PH_OK && d->f64) {
+        // the binary64 copies hold operand * N^-1: the fused kernel's inverse
+        // then needs no N^-1 multiply on its last stage's sums
         std::vector<ShoupF> hf(h.size());
-        for (uint32_t l = 0; l < d->L; ++l)
-            for (uint64_t i = 0; i < n; ++i) hf[l * n + i] = mkf(h[l * n + i], d->tables[l]->q);
+        for (uint32_t l = 0; l < d->L; ++l) {
+            const uint64_t q = d->tables[l]->q, inv_n = d->tables[l]->inv_n;
+            for (uint64_t i = 0; i < n; ++i) hf[l * n + i] = mkf(mk(h_mulmod(h[l * n + i].w, inv_n, q), q), q);
+        }
         for (int e = 0; e < 2 && rc == PH_OK; ++e) {
             const std::vector<ShoupF> reg = operand_register_order(hf, d->L, d->log_n, e + 3);
             rc = upload((void **)&op->d_f[e], reg.data(), reg.size() * sizeof(ShoupF));
@@ -1807,8 +1814,9 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
         }
         if (d->f64) {
             onesv.resize(ones.size());
-            for (uint32_t l = 0; l < d->L; ++l)
-                for (uint64_t i = 0; i < n; ++i) onesv[l * n + i] = mkf(ones[l * n + i], d->tables[l]->q);
+            for (uint32_t l = 0; l < d->L; ++l)  // stored times N^-1, as ph_operand_new does
+                for (uint64_t i = 0; i < n; ++i)
+                    onesv[l * n + i] = mkf(mk(d->tables[l]->inv_n, d->tables[l]->q), d->tables[l]->q);
             // all ones: every register order is the same array
             if (upload((void **)&op_dev.d_f[0], onesv.data(), onesv.size() * sizeof(ShoupF)) != PH_OK ||
                 upload((void **)&op_dev.d_f[1], onesv.data(), onesv.size() * sizeof(ShoupF)) != PH_OK) {

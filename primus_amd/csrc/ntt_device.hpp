@@ -100,6 +100,7 @@ struct DevTable {
     double qd, qinv;
     double q_2p52;            // q + 2^52 (exact for q < 2^50)
     double inv_n_f, inv_n_p, sr_f, sr_p;
+    double last_f, last_p;    // inv_root_powers[N-1] without N^-1 (for operands that carry it)
     const ShoupF *fwd_f;
     const ShoupF *inv_f;
     // the same in register order for single-pass launches with 2^(e+3)
@@ -132,6 +133,8 @@ struct IntArith {
         b = u + t.q2 - v;
     }
     static constexpr bool kReduceAtRoundStart = false;
+    // resident operands are kept unscaled (their Shoup pairs serve every mode)
+    static constexpr bool kScaledOperand = false;
     __device__ __forceinline__ static void round_start(T &, const DevTable &) {}
     // Gentleman-Sande inverse butterfly, primitive.rs:262-265.
     __device__ __forceinline__ static void gs(T &a, T &b, const Tw &w, uint32_t, const DevTable &t) {
@@ -186,10 +189,10 @@ struct F64Arith {
     }
 
     // x*w mod q, centred.  Twiddles are stored centred (|w| <= q/2, |wp| < 1/2
-    // with |wp - w/q| <= d = 2^-54 + 2^-64), so |x*wp| < 2^51 for |x| <= 3.9 q
-    // and the C-rounding is exact; |x*w/q - k| <= 1/2 + 3.9 q d < 0.744, so
-    // |r| <= 0.744 q.  |h - k q| <= |r| + ulp(h)/2 < 2^53 keeps the last two
-    // steps exact.
+    // with |wp - w/q| <= d = 2^-54 + 2^-64), so |x*wp| < 2^51 for |x| < 4q
+    // (4q < 2^52) and the C-rounding is exact; |x*w/q - k| <= 1/2 + |x| d,
+    // below 0.7503 for |x| < 4q, so |r| <= q (1/2 + |x| d) < 0.7503 q.
+    // |h - k q| <= |r| + ulp(h)/2 < 2^53 keeps the last two steps exact.
     __device__ __forceinline__ static double mulmod(double x, double w, double wp, double q) {
         const double h = x * w;
         const double l = __fma_rn(x, w, -h);
@@ -202,10 +205,19 @@ struct F64Arith {
         const double k = __fma_rn(a, t.qinv, kRound) - kRound;
         return __fma_rn(-k, t.qd, a);
     }
-    // Forward register round: every coefficient is reduced once at the start
-    // (|x| <= q/2 + 1); each of the <= 4 stages then adds at most 0.744 q, so
-    // stage inputs stay below 2.74 q and round outputs below 3.48 q < 3.9 q.
+    // Forward register round.  mulmod's output bound is q (1/2 + |x| d)
+    // <= q (1/2 + B/16) for |x| <= B q (q d < 1/16), and its operand may be
+    // anything below 4q.  Only the first stage's "a" inputs (the unmultiplied
+    // half) are reduced as a round starts (|u| <= q/2 + 1); its "b" inputs go
+    // straight into mulmod.  From inputs below 4q (a round's loaded values,
+    // or the previous round's outputs):
+    //   stage 1: v <= 0.75 q, outputs <= 1.25 q;  stage 2: v <= 0.58 q, <= 1.83 q;
+    //   stage 3: v <= 0.62 q, <= 2.45 q;  stage 4: v <= 0.66 q, <= 3.11 q,
+    // so every mulmod operand is below 4q and every round output below 3.2 q.
     static constexpr bool kReduceAtRoundStart = true;
+    // the resident operand's binary64 copies are stored times N^-1, and the
+    // inverse of a fused forward -> multiply -> inverse ends in gs_last_prescaled
+    static constexpr bool kScaledOperand = true;
     __device__ __forceinline__ static void round_start(T &a, const DevTable &t) { a = reduce(a, t); }
     __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
         const double v = mulmod(b, w.w, w.wp, t.qd);
@@ -228,6 +240,15 @@ struct F64Arith {
         const double u = a, v = b;
         a = mulmod(u + v, t.inv_n_f, t.inv_n_p, t.qd);
         b = mulmod(u - v, t.sr_f, t.sr_p, t.qd);
+    }
+    // Last stage when N^-1 is already in the values (a resident operand
+    // stored times N^-1): the sum is only reduced (<= q/2 + 1), the difference
+    // takes the stage's plain twiddle.  The transform is linear, so the
+    // outputs are those of gs_last on the unscaled values.
+    __device__ __forceinline__ static void gs_last_prescaled(T &a, T &b, const DevTable &t) {
+        const double u = a, v = b;
+        a = reduce(u + v, t);
+        b = mulmod(u - v, t.last_f, t.last_p, t.qd);
     }
     // u64 in [0, 4q) -> centred lane value, |x| <= q/2 + 1.  v < 2^52, so
     // the binary64 with exponent field 2^52 and v as its mantissa is 2^52 + v
@@ -253,7 +274,7 @@ struct F64Arith {
     __device__ __forceinline__ static uint64_t out_inv(T v, const DevTable &t) {
         return to_canon(v, t);  // |v| <= 0.744 q
     }
-    // Forward output (|x| <= 3.48 q) times a resident operand stored like the
+    // Forward output (|x| <= 3.2 q) times a resident operand stored like the
     // twiddles (centred w, w/q correctly rounded): the bounds of mulmod hold
     // as they are, and the product is <= 0.744 q.
     __device__ __forceinline__ static T mul_operand(T x, const ShoupF &w, const DevTable &t) {
@@ -280,8 +301,9 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
                                          gptr<typename A::Tw> tw, const DevTable &t) {
     using Tw = typename A::Tw;
     if constexpr (A::kReduceAtRoundStart) {
+        // the first stage's unmultiplied inputs: x[j], x[j + 2^(R-1)] pair up
 #pragma unroll
-        for (int j = 0; j < (1 << R); ++j) A::round_start(x[j], t);
+        for (int j = 0; j < (1 << (R - 1)); ++j) A::round_start(x[j], t);
     }
     // LANE: c differs between lanes; the loads go through a buffer resource
     // with the c-term as the lane offset (ld_lane)
@@ -324,7 +346,8 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
 // transform's final stage (gap N/2).  PERM: itw is a register-order table
 // (DevTable::inv_p): within stage P+u's segment the twiddle of chunk
 // c*2^(R-u-1) + sub sits at sub*(N >> (P+R)) + c.
-template <class A, int R, bool LAST, bool FENCE = true, bool PERM = false, bool LANE = false>
+// SCALED: the values already carry N^-1 (F64Arith only), see gs_last_prescaled.
+template <class A, int R, bool LAST, bool FENCE = true, bool PERM = false, bool LANE = false, bool SCALED = false>
 __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t c, uint32_t n,
                                          gptr<typename A::Tw> itw, const DevTable &t) {
     using Tw = typename A::Tw;
@@ -334,7 +357,10 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
         const int half = 1 << u;
         if (LAST && u == R - 1) {
 #pragma unroll
-            for (int jj = 0; jj < half; ++jj) A::gs_last(x[jj], x[jj + half], t);
+            for (int jj = 0; jj < half; ++jj) {
+                if constexpr (SCALED) A::gs_last_prescaled(x[jj], x[jj + half], t);
+                else A::gs_last(x[jj], x[jj + half], t);
+            }
         } else {
             const uint32_t lane = PERM ? c : (c << (R - u - 1));
             const uint32_t seg = 1u + n - (n >> (P + u));

@@ -111,13 +111,17 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
         }
     } else {
         constexpr bool LAST = (MODE & M_INV_LAST) != 0;
+        // N^-1 is in the operand's binary64 copies: the last stage only reduces sums
+        constexpr bool SCALED = (MODE & M_MUL_SHOUP) != 0 && LAST && A::kScaledOperand;
         __builtin_amdgcn_sched_barrier(0);
         // With a hook (the streaming kernel, which holds the next
         // polynomial's words through the last rounds) the inverse forms its
         // addresses afresh rather than keeping the forward's alive.
         uint32_t ti = t;
         if constexpr (!std::is_same<H, NoHook>::value) asm volatile("" : "+v"(ti));
-        InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder>::run(x, lds, ti, n, b, tab, hook);
+        // the inverse's first exchange stores the slots each thread loaded
+        // for the forward's last round: no barrier before it (SYNC0 false)
+        InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder, false, SCALED>::run(x, lds, ti, n, b, tab, hook);
         InvRounds<A, LOGC, LOGT, G::NR - 1, LAST, A::kRegisterOrder>::store_global(data, x, ti, tab);
     }
 }
@@ -152,15 +156,16 @@ __attribute__((amdgpu_waves_per_eu(1, Occupancy<LOGC, LOGT, MODE>::value))) void
             // transform the second operand first and keep it in registers
             T y[G::E];
             FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_global(a.data2 + off, y, t, tab);
-            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::run(y, lds, t, st0, b, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder, false>::run(y, lds, t, st0, b, tab);
             FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_global(data, x, t, tab);
-            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, t, st0, b, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder, true>::run(x, lds, t, st0, b, tab);
             // NttPolynomial::mul_assign (primus_poly/src/ntt/mul.rs:87-95)
 #pragma unroll
             for (int j = 0; j < G::E; ++j) x[j] = A::mul_pair(x[j], y[j], tab);
         } else {
+            // the first use of the LDS in this workgroup: no barrier before its first exchange
             FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::load_global(data, x, t, tab);
-            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, t, st0, b, tab);
+            FwdRounds<A, LOGC, LOGT, 0, true, A::kRegisterOrder, false>::run(x, lds, t, st0, b, tab);
         }
         after_forward<A, LOGC, LOGT, MODE>(a, x, lds, t, b, limb, n, tab, data);
     } else {
@@ -169,7 +174,7 @@ __attribute__((amdgpu_waves_per_eu(1, Occupancy<LOGC, LOGT, MODE>::value))) void
         for (int j = 0; j < G::E; ++j) x[j] = A::load(src[j], tab);
         constexpr bool LAST = (MODE & M_INV_LAST) != 0;
         __builtin_amdgcn_sched_barrier(0);
-        InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder>::run(x, lds, t, n, b, tab);
+        InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder, false>::run(x, lds, t, n, b, tab);
         InvRounds<A, LOGC, LOGT, G::NR - 1, LAST, A::kRegisterOrder>::store_global(data, x, t, tab);
     }
 }

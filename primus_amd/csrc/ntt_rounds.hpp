@@ -35,7 +35,7 @@ struct Geo {
 
 // Local slot of register (m, j) in the forward round covering local stages [S, S+R).
 template <int LOGC, int LOGT, int S, int R>
-__host__ __device__ __forceinline__ uint32_t fwd_slot(uint32_t t, int m, int j) {
+__host__ __device__ __forceinline__ constexpr uint32_t fwd_slot(uint32_t t, int m, int j) {
     constexpr int OB = LOGC - S - R;
     const uint32_t gi = t + ((uint32_t)m << LOGT);
     const uint32_t c = gi >> OB;
@@ -45,21 +45,83 @@ __host__ __device__ __forceinline__ uint32_t fwd_slot(uint32_t t, int m, int j) 
 
 // Local slot of register (m, j) in the inverse round covering local stages [P, P+R).
 template <int LOGC, int LOGT, int P, int R>
-__host__ __device__ __forceinline__ uint32_t inv_slot(uint32_t t, int m, int j) {
+__host__ __device__ __forceinline__ constexpr uint32_t inv_slot(uint32_t t, int m, int j) {
     const uint32_t gi = t + ((uint32_t)m << LOGT);
     const uint32_t c = gi >> P;
     const uint32_t o = gi & ((1u << P) - 1u);
     return (c << (P + R)) + ((uint32_t)j << P) + o;
 }
 
+// True when every wave of the workgroup covers the same LDS slots with the
+// layout `slot_a` as with `slot_b` (both bijections of (thread, m, j) onto
+// the chunk): each wave's slots under both are then one aligned block of
+// 64 * E, checked by its extent.  An exchange between two such layouts only
+// moves data inside each wave.
+template <int LOGC, int LOGT, class FA, class FB>
+constexpr bool same_wave_blocks(FA slot_a, FB slot_b) {
+    constexpr int T = 1 << LOGT, E = 1 << (LOGC - LOGT);
+    if (T <= 64) return true;
+    for (int w = 0; w < T / 64; ++w) {
+        uint32_t lo_a = ~0u, hi_a = 0, lo_b = ~0u, hi_b = 0;
+        for (int lane = 0; lane < 64; ++lane)
+            for (int i = 0; i < E; ++i) {
+                const uint32_t t = (uint32_t)(w * 64 + lane);
+                const uint32_t a = slot_a(t, i), b = slot_b(t, i);
+                lo_a = a < lo_a ? a : lo_a;
+                hi_a = a > hi_a ? a : hi_a;
+                lo_b = b < lo_b ? b : lo_b;
+                hi_b = b > hi_b ? b : hi_b;
+            }
+        if (lo_a != lo_b || hi_a - lo_a + 1 != 64u * E || hi_b - lo_b + 1 != 64u * E) return false;
+    }
+    return true;
+}
+
+// Orders one wave's LDS stores before its later LDS loads.  A wave's LDS
+// instructions execute in issue order, so only the compiler has to be kept
+// from moving a load above a store of another lane's slot.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroup barriers of one LDS exchange (store in the previous round's
+// layout, load in this round's).  Before the store: only when the exchange is
+// the first of a transform and the LDS was used before it (SYNC0); a later
+// exchange stores exactly the slots the same thread loaded in the exchange
+// before it, which no other thread reads.  Between store and load: a wave
+// barrier when the exchange stays inside each wave, else a workgroup barrier.
+template <bool PRE, bool LOCAL>
+struct Exchange {
+    __device__ __forceinline__ static void before_store() {
+        if constexpr (PRE) __syncthreads();
+    }
+    __device__ __forceinline__ static void before_load() {
+        if constexpr (LOCAL) wave_lds_sync();
+        else __syncthreads();
+    }
+};
+
 // FENCE: schedule each round stage by stage (ct_round); off for kernels that
 // keep two transforms in flight and need the scheduler's freedom.  PERM: read
 // the register-order twiddle tables of this launch shape (single pass only).
-template <class A, int LOGC, int LOGT, int I, bool FENCE = true, bool PERM = false>
+// SYNC0: the LDS may hold another transform's data that other threads still
+// read when this one starts (see Exchange).
+template <class A, int LOGC, int LOGT, int I, bool FENCE = true, bool PERM = false, bool SYNC0 = true>
 struct FwdRounds {
     using G = Geo<LOGC, LOGT>;
     using T = typename A::T;
     static constexpr int S = G::fs(I), R = G::fr(I);
+    // the exchange into this round (I > 0) stays inside each wave
+    static constexpr bool kLocal =
+        I == 0 ? true
+               : same_wave_blocks<LOGC, LOGT>(
+                     [](uint32_t t, int i) {
+                         constexpr int R0 = G::fr(I == 0 ? 0 : I - 1), S0 = G::fs(I == 0 ? 0 : I - 1);
+                         return fwd_slot<LOGC, LOGT, S0, R0>(t, i >> R0, i & ((1 << R0) - 1));
+                     },
+                     [](uint32_t t, int i) { return fwd_slot<LOGC, LOGT, S, R>(t, i >> R, i & ((1 << R) - 1)); });
 
     // LDS exchanges.  A slot is its thread part plus its register part, and
     // the two never carry into each other's low four bits (the thread part's
@@ -108,9 +170,10 @@ struct FwdRounds {
     __device__ __forceinline__ static void run(T *x, T *lds, uint32_t t, uint32_t st0, uint32_t b,
                                                const DevTable &tab) {
         if constexpr (I > 0) {
-            __syncthreads();
-            FwdRounds<A, LOGC, LOGT, I - 1, FENCE, PERM>::store(lds, x, t);
-            __syncthreads();
+            using X = Exchange<I == 1 && SYNC0, kLocal>;
+            X::before_store();
+            FwdRounds<A, LOGC, LOGT, I - 1, FENCE, PERM, SYNC0>::store(lds, x, t);
+            X::before_load();
             load(lds, x, t);
         }
 #pragma unroll
@@ -128,7 +191,7 @@ struct FwdRounds {
             else
                 ct_round<A, R, FENCE, false, LANE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
         }
-        if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1, FENCE, PERM>::run(x, lds, t, st0, b, tab);
+        if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1, FENCE, PERM, SYNC0>::run(x, lds, t, st0, b, tab);
     }
 };
 
@@ -137,11 +200,20 @@ struct NoHook {
     __device__ __forceinline__ void operator()() const {}
 };
 
-template <class A, int LOGC, int LOGT, int I, bool LAST, bool PERM = false>
+// SYNC0 as in FwdRounds; SCALED: the values carry N^-1 already (gs_round).
+template <class A, int LOGC, int LOGT, int I, bool LAST, bool PERM = false, bool SYNC0 = true, bool SCALED = false>
 struct InvRounds {
     using G = Geo<LOGC, LOGT>;
     using T = typename A::T;
     static constexpr int P = G::is(I), R = G::ir(I);
+    static constexpr bool kLocal =
+        I == 0 ? true
+               : same_wave_blocks<LOGC, LOGT>(
+                     [](uint32_t t, int i) {
+                         constexpr int R0 = G::ir(I == 0 ? 0 : I - 1), P0 = G::is(I == 0 ? 0 : I - 1);
+                         return inv_slot<LOGC, LOGT, P0, R0>(t, i >> R0, i & ((1 << R0) - 1));
+                     },
+                     [](uint32_t t, int i) { return inv_slot<LOGC, LOGT, P, R>(t, i >> R, i & ((1 << R) - 1)); });
 
     // thread base plus constant offsets, as in FwdRounds::store
     __device__ __forceinline__ static void store(T *lds, const T *x, uint32_t t) {
@@ -176,9 +248,10 @@ struct InvRounds {
     __device__ __forceinline__ static void run(T *x, T *lds, uint32_t t, uint32_t n, uint32_t b,
                                                const DevTable &tab, const H &hook = H()) {
         if constexpr (I > 0) {
-            __syncthreads();
-            InvRounds<A, LOGC, LOGT, I - 1, LAST, PERM>::store(lds, x, t);
-            __syncthreads();
+            using X = Exchange<I == 1 && SYNC0, kLocal>;
+            X::before_store();
+            InvRounds<A, LOGC, LOGT, I - 1, LAST, PERM, SYNC0, SCALED>::store(lds, x, t);
+            X::before_load();
             load(lds, x, t);
         }
         constexpr int HR = G::first_scalar_inv() < G::NR ? G::first_scalar_inv() : G::NR - 1;
@@ -196,13 +269,14 @@ struct InvRounds {
             if constexpr (P >= 6 && P < LOGT) cl = __builtin_amdgcn_readfirstlane(cl);  // as in FwdRounds
             constexpr bool LANE = P < 6 && P < LOGT;
             if constexpr (PERM)
-                gs_round<A, R, last_here, true, true, LANE>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n,
-                                                            A::inv_reg(tab, G::LOGE), tab);
+                gs_round<A, R, last_here, true, true, LANE, SCALED>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n,
+                                                                    A::inv_reg(tab, G::LOGE), tab);
             else
-                gs_round<A, R, last_here, true, false, LANE>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n,
-                                                             A::inv(tab), tab);
+                gs_round<A, R, last_here, true, false, LANE, SCALED>(x + (m << R), P, (b << (LOGC - P - R)) + cl,
+                                                                     n, A::inv(tab), tab);
         }
-        if constexpr (I + 1 < G::NR) InvRounds<A, LOGC, LOGT, I + 1, LAST, PERM>::run(x, lds, t, n, b, tab, hook);
+        if constexpr (I + 1 < G::NR)
+            InvRounds<A, LOGC, LOGT, I + 1, LAST, PERM, SYNC0, SCALED>::run(x, lds, t, n, b, tab, hook);
     }
 };
 
