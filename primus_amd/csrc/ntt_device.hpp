@@ -397,6 +397,30 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
 __device__ __forceinline__ uint64_t ld_stream(const uint64_t *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_stream(uint64_t *p, uint64_t v) { __builtin_nontemporal_store(v, p); }
 
+// A workgroup's contiguous run of `words` coefficients as a buffer resource:
+// the base sits in scalar registers, each access is one 32-bit lane offset
+// plus a constant (soffset), and no 64-bit address is formed or kept per
+// access.  Accesses are non-temporal (aux 2: "nt"), as ld_stream/st_stream.
+// `base` must be the same on every lane (it is read from the first).  The
+// record count is the run's extent, so an offset past it reads zero and
+// writes nothing.
+struct RowBuf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ RowBuf(const uint64_t *base, uint32_t words) {
+        const uint64_t a = (uint64_t)base;
+        const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+        const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+        r = __builtin_amdgcn_make_buffer_rsrc((void *)((hi << 32) | lo), (short)0, (int)(words * 8u), 0x00020000);
+    }
+    __device__ __forceinline__ uint64_t ld(uint32_t lane_words, uint32_t const_words) const {
+        return __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, lane_words * 8u, const_words * 8u, 2));
+    }
+    __device__ __forceinline__ void st(uint32_t lane_words, uint32_t const_words, uint64_t v) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                              r, lane_words * 8u, const_words * 8u, 2);
+    }
+};
+
 // LDS slot of local index k: one pad word per 16 keeps the stride-16 and
 // stride-1 access patterns of the rounds free of bank conflicts.
 __host__ __device__ __forceinline__ uint32_t lds_pos(uint32_t k) { return k + (k >> 4); }

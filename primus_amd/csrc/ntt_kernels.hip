@@ -43,9 +43,11 @@ __device__ __forceinline__ const ShoupF *operand_of(const KArgs &a, const F64Ari
 }
 
 // Waves per SIMD the launch bounds ask the register allocator for: as many as
-// the LDS footprint lets resident, at most 4 (128 VGPRs: 8 waves per SIMD
-// would leave 64, and the radix-8 rounds spill there), halved when a second
-// operand's transform is held in registers too.
+// the LDS footprint lets resident, at most 4 (128 VGPRs) with 16 coefficients
+// per thread and at most 8 (64 VGPRs) with 8 or fewer, halved when a second
+// operand's transform is held in registers too.  Coefficient rows go through
+// buffer resources (RowBuf), so no 64-bit address is kept per access and the
+// radix-8 rounds fit 64 VGPRs without spilling.
 template <int LOGC, int LOGT, int MODE>
 struct Occupancy {
     static constexpr int lds_bytes = (Geo<LOGC, LOGT>::LDS) * 8;
@@ -53,7 +55,8 @@ struct Occupancy {
     static constexpr int wgs = wgs_lds < 1 ? 1 : (wgs_lds > 32 ? 32 : wgs_lds);
     static constexpr int waves_wg = (1 << LOGT) < 64 ? 1 : (1 << LOGT) / 64;
     static constexpr int per_simd_raw = (wgs * waves_wg) / 4;
-    static constexpr int per_simd = per_simd_raw < 1 ? 1 : (per_simd_raw > 4 ? 4 : per_simd_raw);
+    static constexpr int cap = (LOGC - LOGT) <= 3 ? 8 : 4;
+    static constexpr int per_simd = per_simd_raw < 1 ? 1 : (per_simd_raw > cap ? cap : per_simd_raw);
     static constexpr int value = (MODE & M_MUL_POLY) ? (per_simd / 2 < 1 ? 1 : per_simd / 2) : per_simd;
 };
 
@@ -184,7 +187,8 @@ __attribute__((amdgpu_waves_per_eu(1, Occupancy<LOGC, LOGT, MODE>::value))) void
 // the current one's spill at 128 VGPRs.
 template <int LOGC, int LOGT, int MODE>
 struct StreamOccupancy {
-    static constexpr int base = Occupancy<LOGC, LOGT, MODE>::value;
+    static constexpr int base0 = Occupancy<LOGC, LOGT, MODE>::value;
+    static constexpr int base = base0 > 4 ? 4 : base0;  // the prefetched words need the registers
     static constexpr int value = (LOGC - LOGT) >= 4 ? (base > 2 ? 2 : base) : base;
 };
 

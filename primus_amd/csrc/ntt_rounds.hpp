@@ -145,24 +145,26 @@ struct FwdRounds {
     // Global accesses split each slot into its thread part, fwd_slot(t, 0, 0),
     // and its register part, fwd_slot(0, m, j), a constant (the slot is their
     // sum): every access is then a uniform base plus one 32-bit lane offset.
+    // g: the chunk's first word, the same for the whole workgroup (RowBuf).
     __device__ __forceinline__ static void load_global(const uint64_t *__restrict__ g, T *x, uint32_t t,
                                                        const DevTable &tab) {
+        const RowBuf rb(g, G::C);
         const uint32_t lt = fwd_slot<LOGC, LOGT, S, R>(t, 0, 0);
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j)
-                x[(m << R) + j] = A::load_fwd(ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(0, m, j) + lt), tab);
+                x[(m << R) + j] = A::load_fwd(rb.ld(lt, fwd_slot<LOGC, LOGT, S, R>(0, m, j)), tab);
     }
     // The words load_global reads, unconverted, in the same register order
     // (converted with A::load_fwd before the rounds).
     __device__ __forceinline__ static void load_raw(const uint64_t *__restrict__ g, uint64_t *r, uint32_t t) {
+        const RowBuf rb(g, G::C);
         const uint32_t lt = fwd_slot<LOGC, LOGT, S, R>(t, 0, 0);
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R); ++j)
-                r[(m << R) + j] = ld_stream(g + fwd_slot<LOGC, LOGT, S, R>(0, m, j) + lt);
+            for (int j = 0; j < (1 << R); ++j) r[(m << R) + j] = rb.ld(lt, fwd_slot<LOGC, LOGT, S, R>(0, m, j));
     }
 
     // Runs rounds I .. NR-1.  st0: global stage of local stage 0; b: chunk index
@@ -231,14 +233,16 @@ struct InvRounds {
             for (int j = 0; j < (1 << R); ++j) x[(m << R) + j] = base[lds_pos(inv_slot<LOGC, LOGT, P, R>(0, m, j))];
     }
     // thread part plus constant register part, as FwdRounds::load_global
+    // g: the chunk's first word, the same for the whole workgroup (RowBuf).
     __device__ __forceinline__ static void store_global(uint64_t *__restrict__ g, const T *x, uint32_t t,
                                                         const DevTable &tab) {
+        const RowBuf rb(g, G::C);
         const uint32_t lt = inv_slot<LOGC, LOGT, P, R>(t, 0, 0);
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m)
 #pragma unroll
             for (int j = 0; j < (1 << R); ++j)
-                st_stream(g + inv_slot<LOGC, LOGT, P, R>(0, m, j) + lt, A::out_inv(x[(m << R) + j], tab));
+                rb.st(lt, inv_slot<LOGC, LOGT, P, R>(0, m, j), A::out_inv(x[(m << R) + j], tab));
     }
 
     // Runs rounds I .. NR-1.  hook() is called once, as round H starts
