@@ -395,6 +395,16 @@ def secondary_base_conversion(pa, timer, sp):
     return out
 
 
+SECONDARY = {
+    "plain_n1024": secondary_plain_products,
+    "rlwe_uneven": secondary_rlwe,
+    "ext_product": secondary_ext_product,
+    "ext_product_rns": secondary_ext_product_rns,
+    "automorphism": secondary_automorphism,
+    "base_conversion": secondary_base_conversion,
+}
+
+
 def kernel_signature(shape):
     """The fused kernel a tuning shape launches, as rocprofv3 names it (without 'void ')."""
     from primus_amd import tuning
@@ -602,6 +612,8 @@ def main():
                     help="the bounded serving pass of the default run: 1k and 10k req/s and saturation, batched "
                          "and plain on equal request counts (1000, 5000, 5000), a pool of 96 requests")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving pass of the default run")
+    ap.add_argument("--secondary-only", choices=sorted(SECONDARY), default=None,
+                    help="run one secondary workload alone and print its JSON (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -628,7 +640,7 @@ def main():
     # hardware queue per server stream, which HIP reads when it starts, and
     # this process has not touched the GPU yet
     serving = None
-    if not (args.no_serving or args.no_secondary or args.profile):
+    if not (args.no_serving or args.no_secondary or args.profile or args.secondary_only):
         serving = serving_leg()
 
     import torch
@@ -655,6 +667,10 @@ def main():
     ptrs = [ctypes.c_void_p(bufs[r].data_ptr()) for r in range(ROTATE)]
     timer = Timer(torch, stream)
     L = pa.lib()
+    if args.secondary_only:
+        # one secondary workload alone (counter profiles of its kernels)
+        print(json.dumps({args.secondary_only: SECONDARY[args.secondary_only](pa, timer, sp)}))
+        return 0
 
     def launcher(tab, operand_, sp_=sp):
         def step(i):
@@ -828,14 +844,7 @@ def main():
         },
     }
     if not args.no_secondary:
-        out["secondary"] = {
-            "plain_n1024": secondary_plain_products(pa, timer, sp),
-            "rlwe_uneven": secondary_rlwe(pa, timer, sp),
-            "ext_product": secondary_ext_product(pa, timer, sp),
-            "ext_product_rns": secondary_ext_product_rns(pa, timer, sp),
-            "automorphism": secondary_automorphism(pa, timer, sp),
-            "base_conversion": secondary_base_conversion(pa, timer, sp),
-        }
+        out["secondary"] = {name: fn(pa, timer, sp) for name, fn in SECONDARY.items()}
         if serving is not None:
             out["secondary"]["serving"] = serving
     if not args.no_cpu:

@@ -22,6 +22,17 @@ namespace ph {
 
 __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
 
+// floor(a*b / 2^64) - e with e in {0, 1, 2}: a1*b1 plus the high halves of
+// the two middle products, without a0*b0 and without the low halves of the
+// middle products (together below 3 * 2^64, so the floor loses at most 2).
+// Two multiply-highs and one multiply-add instead of four multiplies and
+// their carries.  The sum stays below 2^64: (2^32-1)^2 + 2 (2^32-1) < 2^64.
+__device__ __forceinline__ uint64_t mulhi64_approx(uint64_t a, uint64_t b) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint32_t h1 = __umulhi(a1, b0), h2 = __umulhi(a0, b1);
+    return (uint64_t)a1 * b1 + h1 + h2;
+}
+
 __device__ __forceinline__ uint64_t sub_if_ge(uint64_t x, uint64_t m) { return x >= m ? x - m : x; }
 
 // w*b - floor(wq*b / 2^64)*q (mod 2^64), in [0, 2q); wq = floor(w*2^64/q).
@@ -31,6 +42,13 @@ __device__ __forceinline__ uint64_t shoup_lazy(uint64_t w, uint64_t wq, uint64_t
 
 __device__ __forceinline__ uint64_t shoup_full(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
     return sub_if_ge(shoup_lazy(w, wq, b, q), q);
+}
+
+// shoup_lazy with the approximate quotient: the quotient is at most two below
+// Shoup's, so the result is congruent to w*b and lies in [0, 4q) for any
+// 64-bit b (q < 2^62).  Used only where no caller sees lazy values.
+__device__ __forceinline__ uint64_t shoup_approx(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
+    return w * b - mulhi64_approx(wq, b) * q;
 }
 
 // Canonical (a*b + c) mod q by the 2-limb Barrett quotient with ratio

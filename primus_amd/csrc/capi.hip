@@ -35,6 +35,7 @@ struct ph_ntt_table {
     ShoupF *d_fwd_f = nullptr, *d_inv_f = nullptr;  // binary64 twiddles (q < 2^50)
     ShoupF *d_perm[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [loge - 3][fwd, inv], register order
     bool f64 = false;                                // canonical ops may use F64Arith
+    bool wide = false;                               // canonical ops may use IntCanonArith (q < 2^61)
     DevTable dev{};                    // device view (pointers into d_*)
     DevTable *d_tab = nullptr;         // the view itself in device memory (L = 1)
 };
@@ -42,6 +43,7 @@ struct ph_ntt_table {
 struct ph_dcrt_table {
     uint32_t log_n, L;
     bool f64 = false;
+    bool wide = false;  // every modulus below 2^61
     bool owns = true;  // false for a view over tables owned elsewhere
     std::vector<ph_ntt_table *> tables;
     DevTable *d_tabs = nullptr;
@@ -277,6 +279,7 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
     DevTable &d = t->dev;
     d.q = q;
     d.q2 = q << 1;
+    d.q4 = q << 2;  // meaningful for q < 2^62; IntCanonArith uses it for q < 2^61
     d.inv_n = t->inv_n;
     d.inv_n_q = t->inv_n_q;
     // scaled root of the last inverse stage: N^-1 * inv_root_powers[N-1] (:272-277)
@@ -289,6 +292,7 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
     d.ord = t->d_ord;
     d.log_n = log_n;
     t->f64 = q < kF64MaxModulus && !(flags & PH_TABLE_INTEGER_ONLY);
+    t->wide = q < kCanonIntMaxModulus;
     if (t->f64) {
         std::vector<ShoupF> ff(n), fi(n);
         for (uint64_t k = 0; k < n; ++k) {
@@ -366,6 +370,12 @@ int staging(size_t bytes, void **buf, hipStream_t *stream) {
 // ph_tuning_set / ph_autotune / ph_tuning_load, read by every launch.
 std::atomic<int> g_tuning[kMaxLogN + 1][5][2];
 
+// The arithmetic a table's launches may use (kArithF64 / kArithCanonInt bits).
+template <class Tab>
+int arith_of(const Tab *t) {
+    return (t->f64 ? kArithF64 : 0) | (t->wide ? kArithCanonInt : 0);
+}
+
 int tuning_of(uint32_t log_n, Op op, bool f64) {
     if (log_n > kMaxLogN) return 0;
     return g_tuning[log_n][(int)op][f64 ? 1 : 0].load(std::memory_order_relaxed);
@@ -374,7 +384,7 @@ int tuning_of(uint32_t log_n, Op op, bool f64) {
 // limb_of: per limb-polynomial table index (poly-multiply), or null for i % L.
 // caller_scratch: device memory of scratch_bytes(op, log_n, count) or more to
 // use instead of an allocation, or null.
-int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, uint64_t *d_data,
+int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, int arith, uint64_t *d_data,
            const uint64_t *d_data2, const ph_operand *operand, uint64_t count, hipStream_t s,
            const uint8_t *limb_of = nullptr, void *caller_scratch = nullptr) {
     if (count == 0) return PH_OK;
@@ -385,27 +395,28 @@ int run_op(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, 
     a.operand = operand ? operand->d : nullptr;
     a.operand_f[0] = operand ? operand->d_f[0] : nullptr;
     a.operand_f[1] = operand ? operand->d_f[1] : nullptr;
-    if (op == Op::FwdMulInv && f64 && (!a.operand_f[0] || !a.operand_f[1])) f64 = false;
+    if (op == Op::FwdMulInv && (!a.operand_f[0] || !a.operand_f[1])) arith &= ~kArithF64;
     a.tabs = d_tabs;
     a.L = L;
     void *scratch = caller_scratch;
     const bool own = needs_scratch(op, log_n) && !scratch;
     if (own) PH_HIP(hipMallocAsync(&scratch, (count << log_n) * 8, s));
-    hipError_t e = launch_transform(op, a, log_n, count, scratch, f64, tuning_of(log_n, op, f64), s);
+    const bool f64 = (arith & kArithF64) != 0 && op != Op::FwdLazy;
+    hipError_t e = launch_transform(op, a, log_n, count, scratch, arith, tuning_of(log_n, op, f64), s);
     if (own) (void)hipFreeAsync(scratch, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return PH_OK;
 }
 
 // Host slice -> device -> op -> host slice, synchronous.
-int host_roundtrip(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, bool f64, uint64_t *poly,
+int host_roundtrip(Op op, const DevTable *d_tabs, uint32_t L, uint32_t log_n, int arith, uint64_t *poly,
                    size_t len) {
     void *buf;
     hipStream_t s;
     int rc = staging(len * 8, &buf, &s);
     if (rc != PH_OK) return rc;
     PH_HIP(hipMemcpyAsync(buf, poly, len * 8, hipMemcpyHostToDevice, s));
-    rc = run_op(op, d_tabs, L, log_n, f64, (uint64_t *)buf, nullptr, nullptr, len >> log_n, s);
+    rc = run_op(op, d_tabs, L, log_n, arith, (uint64_t *)buf, nullptr, nullptr, len >> log_n, s);
     if (rc != PH_OK) return rc;
     PH_HIP(hipMemcpyAsync(poly, buf, len * 8, hipMemcpyDeviceToHost, s));
     PH_HIP(hipStreamSynchronize(s));
@@ -494,22 +505,22 @@ int ph_ntt_table_powers(const ph_ntt_table *t, int which, uint64_t *values, uint
 
 int ph_lazy_transform_slice(const ph_ntt_table *t, uint64_t *poly, size_t len) {
     int rc = check_slice(t, poly, len, t ? t->n : 0);
-    return rc != PH_OK ? rc : host_roundtrip(Op::FwdLazy, t->d_tab, 1, t->log_n, t->f64, poly, len);
+    return rc != PH_OK ? rc : host_roundtrip(Op::FwdLazy, t->d_tab, 1, t->log_n, arith_of(t), poly, len);
 }
 
 int ph_transform_slice(const ph_ntt_table *t, uint64_t *poly, size_t len) {
     int rc = check_slice(t, poly, len, t ? t->n : 0);
-    return rc != PH_OK ? rc : host_roundtrip(Op::FwdCanon, t->d_tab, 1, t->log_n, t->f64, poly, len);
+    return rc != PH_OK ? rc : host_roundtrip(Op::FwdCanon, t->d_tab, 1, t->log_n, arith_of(t), poly, len);
 }
 
 int ph_lazy_inverse_transform_slice(const ph_ntt_table *t, uint64_t *values, size_t len) {
     int rc = check_slice(t, values, len, t ? t->n : 0);
-    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, t->f64, values, len);
+    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, arith_of(t), values, len);
 }
 
 int ph_inverse_transform_slice(const ph_ntt_table *t, uint64_t *values, size_t len) {
     int rc = check_slice(t, values, len, t ? t->n : 0);
-    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, t->f64, values, len);
+    return rc != PH_OK ? rc : host_roundtrip(Op::Inv, t->d_tab, 1, t->log_n, arith_of(t), values, len);
 }
 
 int ph_hexl_compute_forward(const ph_ntt_table *t, uint64_t *operand, size_t len, uint64_t input_mod_factor,
@@ -558,6 +569,7 @@ int ph_dcrt_table_new_ex(uint32_t log_n, const uint64_t *moduli, uint32_t moduli
     d->L = moduli_count;
     std::vector<DevTable> views;
     d->f64 = true;
+    d->wide = true;
     for (uint32_t i = 0; i < moduli_count; ++i) {
         ph_ntt_table *t = nullptr;
         int rc = build_table(log_n, moduli[i], flags, &t);
@@ -568,6 +580,7 @@ int ph_dcrt_table_new_ex(uint32_t log_n, const uint64_t *moduli, uint32_t moduli
         d->tables.push_back(t);
         views.push_back(t->dev);
         d->f64 = d->f64 && t->f64;
+        d->wide = d->wide && t->wide;
     }
     int rc = upload((void **)&d->d_tabs, views.data(), views.size() * sizeof(DevTable));
     if (rc != PH_OK) {
@@ -590,11 +603,13 @@ int ph_dcrt_table_view(const ph_ntt_table *const *tables, uint32_t moduli_count,
     d->L = moduli_count;
     d->owns = false;
     d->f64 = true;
+    d->wide = true;
     std::vector<DevTable> views;
     for (uint32_t i = 0; i < moduli_count; ++i) {
         d->tables.push_back(const_cast<ph_ntt_table *>(tables[i]));
         views.push_back(tables[i]->dev);
         d->f64 = d->f64 && tables[i]->f64;
+        d->wide = d->wide && tables[i]->wide;
     }
     int rc = upload((void **)&d->d_tabs, views.data(), views.size() * sizeof(DevTable));
     if (rc != PH_OK) {
@@ -625,7 +640,7 @@ const ph_ntt_table *ph_dcrt_ntt_table(const ph_dcrt_table *d, uint32_t i) {
 
 static int dcrt_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len, Op op) {
     int rc = check_slice(d, poly, len, d ? ((size_t)d->L << d->log_n) : 0);
-    return rc != PH_OK ? rc : host_roundtrip(op, d->d_tabs, d->L, d->log_n, d->f64, poly, len);
+    return rc != PH_OK ? rc : host_roundtrip(op, d->d_tabs, d->L, d->log_n, arith_of(d), poly, len);
 }
 
 int ph_dcrt_lazy_transform_slice(const ph_dcrt_table *d, uint64_t *poly, size_t len) {
@@ -643,24 +658,24 @@ int ph_dcrt_inverse_transform_slice(const ph_dcrt_table *d, uint64_t *poly, size
 
 int ph_dcrt_transform_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t count, int lazy, void *stream) {
     if (!d || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
-    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, d->d_tabs, d->L, d->log_n, d->f64, d_polys, nullptr, nullptr,
+    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, d->d_tabs, d->L, d->log_n, arith_of(d), d_polys, nullptr, nullptr,
                   count, (hipStream_t)stream);
 }
 
 int ph_dcrt_inverse_transform_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t count, void *stream) {
     if (!d || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
-    return run_op(Op::Inv, d->d_tabs, d->L, d->log_n, d->f64, d_polys, nullptr, nullptr, count, (hipStream_t)stream);
+    return run_op(Op::Inv, d->d_tabs, d->L, d->log_n, arith_of(d), d_polys, nullptr, nullptr, count, (hipStream_t)stream);
 }
 
 int ph_ntt_transform_device(const ph_ntt_table *t, uint64_t *d_polys, size_t count, int lazy, void *stream) {
     if (!t || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
-    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, t->d_tab, 1, t->log_n, t->f64, d_polys, nullptr, nullptr, count,
+    return run_op(lazy ? Op::FwdLazy : Op::FwdCanon, t->d_tab, 1, t->log_n, arith_of(t), d_polys, nullptr, nullptr, count,
                   (hipStream_t)stream);
 }
 
 int ph_ntt_inverse_transform_device(const ph_ntt_table *t, uint64_t *d_polys, size_t count, void *stream) {
     if (!t || (!d_polys && count)) return fail(PH_ERR_INVALID, "null table or data");
-    return run_op(Op::Inv, t->d_tab, 1, t->log_n, t->f64, d_polys, nullptr, nullptr, count, (hipStream_t)stream);
+    return run_op(Op::Inv, t->d_tab, 1, t->log_n, arith_of(t), d_polys, nullptr, nullptr, count, (hipStream_t)stream);
 }
 
 int ph_operand_new(const ph_dcrt_table *d, const uint64_t *host_values, ph_operand **out) {
@@ -716,13 +731,13 @@ int ph_dcrt_fwd_mul_inv_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t
                                void *stream) {
     if (!d || !op || (!d_polys && count)) return fail(PH_ERR_INVALID, "null argument");
     if (op->L != d->L || op->n != (1ull << d->log_n)) return fail(PH_ERR_INVALID, "operand shape mismatch");
-    return run_op(Op::FwdMulInv, d->d_tabs, d->L, d->log_n, d->f64, d_polys, nullptr, op, count, (hipStream_t)stream);
+    return run_op(Op::FwdMulInv, d->d_tabs, d->L, d->log_n, arith_of(d), d_polys, nullptr, op, count, (hipStream_t)stream);
 }
 
 int ph_dcrt_negacyclic_mul_device(const ph_dcrt_table *d, uint64_t *d_a, const uint64_t *d_b, size_t count,
                                   void *stream) {
     if (!d || ((!d_a || !d_b) && count)) return fail(PH_ERR_INVALID, "null argument");
-    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, (hipStream_t)stream);
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, arith_of(d), d_a, d_b, nullptr, count, (hipStream_t)stream);
 }
 
 }  // extern "C"
@@ -777,14 +792,14 @@ int ph_dcrt_negacyclic_mul_device_noalloc(const ph_dcrt_table *d, uint64_t *d_a,
     }
     // every scratch word the column pass writes is read back by the chunk
     // pass of the same launch sequence, so nothing needs clearing
-    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, (hipStream_t)stream,
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, arith_of(d), d_a, d_b, nullptr, count, (hipStream_t)stream,
                   nullptr, scr);
 }
 
 int ph_dcrt_negacyclic_mul_rows_device(const ph_dcrt_table *d, uint64_t *d_a, const uint64_t *d_b,
                                        const uint8_t *d_limb_of, size_t count, void *stream) {
     if (!d || ((!d_a || !d_b || !d_limb_of) && count)) return fail(PH_ERR_INVALID, "null argument");
-    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, (hipStream_t)stream,
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, arith_of(d), d_a, d_b, nullptr, count, (hipStream_t)stream,
                   d_limb_of);
 }
 
@@ -1312,7 +1327,7 @@ int ph_ext_product_rns_device(const ph_ext_key_rns *key, const uint64_t *d_in, u
     hipError_t e = launch_ext_rns_digits(a, s);
     int rc = e == hipSuccess ? PH_OK : hip_fail(e, "digit kernel");
     // the digits' forward transforms: row r of [count][2][l][L][N] is limb r % L
-    if (rc == PH_OK) rc = run_op(Op::FwdCanon, d->d_tabs, d->L, d->log_n, d->f64, digits, nullptr, nullptr,
+    if (rc == PH_OK) rc = run_op(Op::FwdCanon, d->d_tabs, d->L, d->log_n, arith_of(d), digits, nullptr, nullptr,
                                  count * 2 * l * L, s);
     if (rc == PH_OK) {
         ExtRnsMacArgs m{};
@@ -1328,7 +1343,7 @@ int ph_ext_product_rns_device(const ph_ext_key_rns *key, const uint64_t *d_in, u
         if (e != hipSuccess) rc = hip_fail(e, "multiply-accumulate kernel");
     }
     if (rc == PH_OK && !ntt_output)
-        rc = run_op(Op::Inv, d->d_tabs, d->L, d->log_n, d->f64, d_out, nullptr, nullptr, count * 2 * L, s);
+        rc = run_op(Op::Inv, d->d_tabs, d->L, d->log_n, arith_of(d), d_out, nullptr, nullptr, count * 2 * L, s);
     (void)hipFreeAsync(digits, s);
     return rc;
 }
@@ -1377,6 +1392,7 @@ struct ph_rlwe_ctx {
     std::vector<ph_ntt_table *> tables;  // [(log_n - min_log_n) * max_limbs + i]
     DevTable *d_tabs = nullptr;
     int cus = 0;
+    bool wide = true;  // every modulus below 2^61: IntCanonArith kernels
 };
 
 void rlwe_ctx_free(ph_rlwe_ctx *c) {
@@ -1407,6 +1423,7 @@ int ph_rlwe_ctx_new(uint32_t min_log_n, uint32_t max_log_n, uint32_t max_limbs, 
                 return rc;
             }
             c->tables.push_back(t);
+            c->wide = c->wide && t->wide;
             views.push_back(t->dev);
         }
     int rc = upload((void **)&c->d_tabs, views.data(), views.size() * sizeof(DevTable));
@@ -1485,6 +1502,7 @@ static hipError_t rlwe_launch(const ph_rlwe_ctx *c, const RlweItem *d_items, uin
     a.m = d_m;
     a.scratch = scratch;
     a.scratch_words = rlwe_scratch_words(max_log_n);
+    a.canon_int = c->wide ? 1u : 0u;
     return launch_rlwe_encrypt(a, rlwe_grid(c, n_items), s);
 }
 
@@ -1668,6 +1686,7 @@ int ph_rlwe_encrypt_plan_device(const ph_rlwe_plan *p, const uint64_t *d_a, cons
     a.m = d_m;
     a.scratch = p->scratch;
     a.scratch_words = p->scratch_words;
+    a.canon_int = p->ctx->wide ? 1u : 0u;
     hipError_t e = launch_rlwe_encrypt(a, p->grid, (hipStream_t)stream);
     return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
 }
@@ -1767,6 +1786,7 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
     if (!d || op < 0 || op > 4 || count == 0 || reps < 1) return fail(PH_ERR_INVALID, "bad autotune request");
     const Op o = (Op)op;
     const bool f64 = d->f64 && o != Op::FwdLazy;
+    const int arith = arith_of(d);
     const uint64_t n = 1ull << d->log_n;
     const size_t words = count * n;
     // Rotate over enough copies of the batch (up to 16) that the working set
@@ -1833,7 +1853,7 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
         // warm-up, then timed repetitions
         for (int r = 0; r <= reps; ++r) {
             if (r == 1) (void)hipEventRecord(e0, s);
-            rc = run_op(o, d->d_tabs, d->L, d->log_n, f64, bufs[r % copies], bufs2[r % copies],
+            rc = run_op(o, d->d_tabs, d->L, d->log_n, arith, bufs[r % copies], bufs2[r % copies],
                         o == Op::FwdMulInv ? &op_dev : nullptr, count, s);
             if (rc != PH_OK) goto done;
         }
@@ -1917,7 +1937,7 @@ int negacyclic_mul_rows_staged(const ph_dcrt_table *d, uint64_t *d_a, const uint
                                uint64_t count, void *d_scratch, hipStream_t s) {
     if (!d || ((!d_a || !d_b || !d_limb_of) && count)) return fail(PH_ERR_INVALID, "null argument");
     if (negacyclic_mul_scratch_bytes(d, count) && !d_scratch) return fail(PH_ERR_INVALID, "null scratch");
-    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, d->f64, d_a, d_b, nullptr, count, s, d_limb_of, d_scratch);
+    return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, arith_of(d), d_a, d_b, nullptr, count, s, d_limb_of, d_scratch);
 }
 
 }  // namespace ph

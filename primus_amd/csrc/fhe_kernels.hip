@@ -333,14 +333,14 @@ __device__ __forceinline__ uint64_t rlwe_x(const RlweItem &it, int8_t e, uint8_t
 // t*E + j of s_hat and x_hat; both go through LDS so that the MAC runs in row
 // order and a_hat and b_hat move as whole rows (a wave's load or store covers
 // 64 consecutive words instead of 64 words E apart).
-template <int LOGC, int LOGT>
+template <class A, int LOGC, int LOGT>
 __device__ __forceinline__ void rlwe_mac_rows(const RlweArgs &a, const RlweItem &it, uint64_t base, const uint64_t *x,
                                               const uint64_t *sx, uint64_t *lds, uint32_t tid, const DevTable &t) {
     using G = Geo<LOGC, LOGT>;
     uint64_t as[G::E];
     __syncthreads();  // the rounds' last LDS reads are done
 #pragma unroll
-    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = IntArith::canon_fwd(sx[j], t);
+    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = A::canon_fwd(sx[j], t);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < G::E; ++i) as[i] = ld_stream(a.a + it.row + base + tid + ((uint32_t)i << LOGT));
@@ -352,7 +352,7 @@ __device__ __forceinline__ void rlwe_mac_rows(const RlweArgs &a, const RlweItem 
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = IntArith::canon_fwd(x[j], t);
+    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = A::canon_fwd(x[j], t);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < G::E; ++i) {
@@ -361,8 +361,10 @@ __device__ __forceinline__ void rlwe_mac_rows(const RlweArgs &a, const RlweItem 
     }
 }
 
-// N <= 2^12: both transforms in LDS, 256 threads, E = N / 256.
-template <int LOGN>
+// N <= 2^12: both transforms in LDS, 256 threads, E = N / 256.  A: IntArith,
+// or IntCanonArith when every modulus is below 2^61 (only canonical values
+// leave the kernel).
+template <class A, int LOGN>
 __device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &it, const DevTable &t, uint64_t *lds,
                                             uint32_t tid) {
     constexpr int LOGT = 8;
@@ -388,14 +390,14 @@ __device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &i
         sx[i] = lift_small(sv[i], t.q);
         x[i] = rlwe_x(it, ev[i], mv[i], t);
     }
-    FwdRounds<IntArith, LOGN, LOGT, 0, true>::run(sx, lds, tid, 0, 0, t);
-    FwdRounds<IntArith, LOGN, LOGT, 0, true>::run(x, lds, tid, 0, 0, t);
-    rlwe_mac_rows<LOGN, LOGT>(a, it, 0, x, sx, lds, tid, t);
+    FwdRounds<A, LOGN, LOGT, 0, true>::run(sx, lds, tid, 0, 0, t);
+    FwdRounds<A, LOGN, LOGT, 0, true>::run(x, lds, tid, 0, 0, t);
+    rlwe_mac_rows<A, LOGN, LOGT>(a, it, 0, x, sx, lds, tid, t);
 }
 
 // N = 2^(12+R): column pass (stages 0..R-1) of s into scratch and of x into
 // the output row, then 2^R chunk passes of 4096 in LDS with the MAC fused.
-template <int R>
+template <class A, int R>
 __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem &it, const DevTable &t, uint64_t *lds,
                                               uint32_t tid, uint64_t *scr) {
     constexpr int LOGC = 12, LOGT = 8;
@@ -419,8 +421,8 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
             v[i] = lift_small(sv[i], t.q);
             w[i] = rlwe_x(it, ev[i], mv[i], t);
         }
-        ct_round<IntArith, R, false>(v, 0, 0, IntArith::fwd(t), t);
-        ct_round<IntArith, R, false>(w, 0, 0, IntArith::fwd(t), t);
+        ct_round<A, R, false>(v, 0, 0, A::fwd(t), t);
+        ct_round<A, R, false>(w, 0, 0, A::fwd(t), t);
 #pragma unroll
         for (int i = 0; i < (1 << R); ++i) {
             scr[(uint64_t)i * C + col] = v[i];
@@ -431,16 +433,17 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
     for (uint32_t b = 0; b < (1u << R); ++b) {
         asm volatile("" : "+v"(tid));  // keep slot addresses out of loop-invariant code motion
         uint64_t sx[G::E], x[G::E];
-        FwdRounds<IntArith, LOGC, LOGT, 0, true>::load_global(scr + (uint64_t)b * C, sx, tid, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0, true>::run(sx, lds, tid, R, b, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0, true>::load_global(a.out + it.row + (uint64_t)b * C, x, tid, t);
-        FwdRounds<IntArith, LOGC, LOGT, 0, true>::run(x, lds, tid, R, b, t);
-        rlwe_mac_rows<LOGC, LOGT>(a, it, (uint64_t)b * C, x, sx, lds, tid, t);
+        FwdRounds<A, LOGC, LOGT, 0, true>::load_global(scr + (uint64_t)b * C, sx, tid, t);
+        FwdRounds<A, LOGC, LOGT, 0, true>::run(sx, lds, tid, R, b, t);
+        FwdRounds<A, LOGC, LOGT, 0, true>::load_global(a.out + it.row + (uint64_t)b * C, x, tid, t);
+        FwdRounds<A, LOGC, LOGT, 0, true>::run(x, lds, tid, R, b, t);
+        rlwe_mac_rows<A, LOGC, LOGT>(a, it, (uint64_t)b * C, x, sx, lds, tid, t);
     }
 }
 
 // Two waves per SIMD: the 2^12 path holds both transforms (s and e + Delta m) in
 // registers at once, and capping it lower spills.
+template <class A>
 __global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs a) {
     __shared__ uint64_t lds[Geo<12, 8>::LDS];
     uint64_t *scr = a.scratch + (uint64_t)blockIdx.x * a.scratch_words;
@@ -453,13 +456,13 @@ __global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs 
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         switch (it.log_n) {
-            case 10: rlwe_single<10>(a, it, t, lds, tid); break;
-            case 11: rlwe_single<11>(a, it, t, lds, tid); break;
-            case 12: rlwe_single<12>(a, it, t, lds, tid); break;
-            case 13: rlwe_two_pass<1>(a, it, t, lds, tid, scr); break;
-            case 14: rlwe_two_pass<2>(a, it, t, lds, tid, scr); break;
-            case 15: rlwe_two_pass<3>(a, it, t, lds, tid, scr); break;
-            case 16: rlwe_two_pass<4>(a, it, t, lds, tid, scr); break;
+            case 10: rlwe_single<A, 10>(a, it, t, lds, tid); break;
+            case 11: rlwe_single<A, 11>(a, it, t, lds, tid); break;
+            case 12: rlwe_single<A, 12>(a, it, t, lds, tid); break;
+            case 13: rlwe_two_pass<A, 1>(a, it, t, lds, tid, scr); break;
+            case 14: rlwe_two_pass<A, 2>(a, it, t, lds, tid, scr); break;
+            case 15: rlwe_two_pass<A, 3>(a, it, t, lds, tid, scr); break;
+            case 16: rlwe_two_pass<A, 4>(a, it, t, lds, tid, scr); break;
             default: break;
         }
         __syncthreads();  // LDS and scratch are reused by the next item
@@ -500,7 +503,8 @@ hipError_t launch_rlwe_sample(const SampleArgs &a, hipStream_t st) {
 
 hipError_t launch_rlwe_encrypt(const RlweArgs &a, uint32_t grid, hipStream_t s) {
     if (a.n_items == 0 || grid == 0) return hipSuccess;
-    rlwe_encrypt_kernel<<<dim3(grid), dim3(kRlweThreads), 0, s>>>(a);
+    if (a.canon_int) rlwe_encrypt_kernel<IntCanonArith><<<dim3(grid), dim3(kRlweThreads), 0, s>>>(a);
+    else rlwe_encrypt_kernel<IntArith><<<dim3(grid), dim3(kRlweThreads), 0, s>>>(a);
     return hipGetLastError();
 }
 

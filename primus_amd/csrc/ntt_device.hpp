@@ -89,7 +89,7 @@ __device__ __forceinline__ Tw ld_lane(__amdgpu_buffer_rsrc_t r, uint32_t lane_by
 
 // Per-modulus device view of one UintNttTable.
 struct DevTable {
-    uint64_t q, q2;
+    uint64_t q, q2, q4;       // q4 = 4q (used by IntCanonArith, q < 2^61)
     uint64_t inv_n, inv_n_q;  // N^-1 (Shoup)
     uint64_t sr, sr_q;        // N^-1 * inv_root_powers[N-1] (Shoup)
     uint64_t br0, br1;        // Barrett ratio floor(2^128/q)
@@ -163,6 +163,64 @@ struct IntArith {
         return shoup_full(w.w, w.wq, x, t.q);
     }
     // forward output * forward output, canonical (NttPolynomial::mul_assign, Barrett)
+    __device__ __forceinline__ static T mul_pair(T x, T y, const DevTable &t) {
+        return barrett_mul(canon_fwd(x, t), canon_fwd(y, t), t.q, t.br0, t.br1);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// IntCanonArith (q < 2^61, canonical outputs only)
+// ---------------------------------------------------------------------------
+
+// The integer butterflies for operations whose callers see only canonical
+// values, with the headroom of q < 2^61 (8q < 2^64) spent on a cheaper
+// quotient: shoup_approx (three multiply-adds instead of four multiplies;
+// products in [0, 4q)), one conditional subtraction of 4q per butterfly, and
+// every intermediate below 8q.  The last inverse stage uses exact Shoup
+// products, whose outputs are canonical after one subtraction of q.
+struct IntCanonArith {
+    using T = uint64_t;
+    using Tw = Shoup;
+
+    __device__ __forceinline__ static gptr<Tw> fwd(const DevTable &t) { return as_global(t.fwd); }
+    __device__ __forceinline__ static gptr<Tw> inv(const DevTable &t) { return as_global(t.inv); }
+    static constexpr bool kRegisterOrder = false;
+    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t, int) { return fwd(t); }
+    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t, int) { return inv(t); }
+
+    // forward: inputs below 8q; u < 4q, v < 4q, outputs below 8q
+    __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
+        const uint64_t u = sub_if_ge(a, t.q4);
+        const uint64_t v = shoup_approx(w.w, w.wq, b, t.q);
+        a = u + v;
+        b = u + t.q4 - v;
+    }
+    static constexpr bool kReduceAtRoundStart = false;
+    static constexpr bool kScaledOperand = false;
+    __device__ __forceinline__ static void round_start(T &, const DevTable &) {}
+    // inverse: inputs below 4q; the sum reduced below 4q, the difference
+    // (positive, below 8q) multiplied to below 4q
+    __device__ __forceinline__ static void gs(T &a, T &b, const Tw &w, uint32_t, const DevTable &t) {
+        const uint64_t u = a, v = b;
+        a = sub_if_ge(u + v, t.q4);
+        b = shoup_approx(w.w, w.wq, u + t.q4 - v, t.q);
+    }
+    // last inverse stage, N^-1 folded in (primitive.rs:270-285), exact Shoup: canonical
+    __device__ __forceinline__ static void gs_last(T &a, T &b, const DevTable &t) {
+        const uint64_t u = a, v = b;
+        a = shoup_full(t.inv_n, t.inv_n_q, u + v, t.q);
+        b = shoup_full(t.sr, t.sr_q, u + t.q4 - v, t.q);
+    }
+    __device__ __forceinline__ static T load(uint64_t v, const DevTable &) { return v; }      // [0, 2q)
+    __device__ __forceinline__ static T load_fwd(uint64_t v, const DevTable &) { return v; }  // [0, 4q)
+    __device__ __forceinline__ static uint64_t canon_fwd(T v, const DevTable &t) {
+        return sub_if_ge(sub_if_ge(sub_if_ge(v, t.q4), t.q2), t.q);
+    }
+    __device__ __forceinline__ static uint64_t out_inv(T v, const DevTable &) { return v; }
+    // forward output times a resident operand: [0, 4q), an inverse input as it is
+    __device__ __forceinline__ static T mul_operand(T x, const Shoup &w, const DevTable &t) {
+        return shoup_approx(w.w, w.wq, x, t.q);
+    }
     __device__ __forceinline__ static T mul_pair(T x, T y, const DevTable &t) {
         return barrett_mul(canon_fwd(x, t), canon_fwd(y, t), t.q, t.br0, t.br1);
     }

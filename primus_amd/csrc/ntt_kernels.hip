@@ -38,6 +38,7 @@ enum : int {
 };
 
 __device__ __forceinline__ const Shoup *operand_of(const KArgs &a, const IntArith *, int) { return a.operand; }
+__device__ __forceinline__ const Shoup *operand_of(const KArgs &a, const IntCanonArith *, int) { return a.operand; }
 __device__ __forceinline__ const ShoupF *operand_of(const KArgs &a, const F64Arith *, int loge) {
     return a.operand_f[loge - 3];
 }
@@ -403,11 +404,11 @@ static hipError_t launch_fwd_first(const KArgs &a, uint64_t count, bool stream, 
 // thread, the mode chosen at run time; `stream` selects the persistent
 // streaming kernel where the mode allows it.
 template <int LOGN, int LOGE>
-static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, bool f64, bool stream, hipStream_t s) {
+static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, int arith, bool stream, hipStream_t s) {
     constexpr int LOGT = LOGN > LOGE ? LOGN - LOGE : 0;
     a.blocks_per_poly = 1;
     if constexpr (LOGN >= 6) {
-        if (f64) {
+        if (arith & kArithF64) {
             switch (op) {
                 case Op::FwdLazy: break;  // lazy values are defined by the integer butterflies
                 case Op::FwdCanon:
@@ -419,6 +420,17 @@ static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, bool f64, bool s
                 case Op::PolyMul:
                     return launch_chunk<F64Arith, LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
             }
+        }
+    }
+    if (arith & kArithCanonInt) {
+        using C = IntCanonArith;
+        switch (op) {
+            case Op::FwdLazy: break;  // lazy values are defined by IntArith's butterflies
+            case Op::FwdCanon: return launch_fwd_first<C, LOGN, LOGT, M_FWD | M_CANON>(a, count, stream, s);
+            case Op::Inv: return launch_chunk<C, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
+            case Op::FwdMulInv:
+                return launch_fwd_first<C, LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count, stream, s);
+            case Op::PolyMul: return launch_chunk<C, LOGN, LOGT, M_FWD | M_MUL_POLY | M_INV | M_INV_LAST>(a, count, s);
         }
     }
     switch (op) {
@@ -437,12 +449,12 @@ static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, bool f64, bool s
 // Coefficients per thread: 16 by default; 8 (twice the waves, one more LDS
 // round) for 2^10 <= N <= 2^13 when the tuning asks for it.
 template <int LOGN>
-static hipError_t single_pass(Op op, KArgs a, uint64_t count, bool f64, int shape, hipStream_t s) {
+static hipError_t single_pass(Op op, KArgs a, uint64_t count, int arith, int shape, hipStream_t s) {
     const bool stream = (shape & kShapeStream) != 0;
     if constexpr (LOGN >= 10) {
-        if ((shape & 15) == 3) return single_pass_e<LOGN, 3>(op, a, count, f64, stream, s);
+        if ((shape & 15) == 3) return single_pass_e<LOGN, 3>(op, a, count, arith, stream, s);
     }
-    return single_pass_e<LOGN, 4>(op, a, count, f64, stream, s);
+    return single_pass_e<LOGN, 4>(op, a, count, arith, stream, s);
 }
 
 // Two-pass schedule for log_n = 12 + LOGR: columns of 2^LOGR, chunks of 4096.
@@ -496,22 +508,22 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
 
 bool needs_scratch(Op op, uint32_t log_n) { return op == Op::PolyMul && log_n > kMaxSinglePassLogN; }
 
-hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, bool f64,
+hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, int arith,
                             int shape, hipStream_t s) {
     switch (log_n) {
-        case 1: return single_pass<1>(op, a, count, f64, shape, s);
-        case 2: return single_pass<2>(op, a, count, f64, shape, s);
-        case 3: return single_pass<3>(op, a, count, f64, shape, s);
-        case 4: return single_pass<4>(op, a, count, f64, shape, s);
-        case 5: return single_pass<5>(op, a, count, f64, shape, s);
-        case 6: return single_pass<6>(op, a, count, f64, shape, s);
-        case 7: return single_pass<7>(op, a, count, f64, shape, s);
-        case 8: return single_pass<8>(op, a, count, f64, shape, s);
-        case 9: return single_pass<9>(op, a, count, f64, shape, s);
-        case 10: return single_pass<10>(op, a, count, f64, shape, s);
-        case 11: return single_pass<11>(op, a, count, f64, shape, s);
-        case 12: return single_pass<12>(op, a, count, f64, shape, s);
-        case 13: return single_pass<13>(op, a, count, f64, shape, s);
+        case 1: return single_pass<1>(op, a, count, arith, shape, s);
+        case 2: return single_pass<2>(op, a, count, arith, shape, s);
+        case 3: return single_pass<3>(op, a, count, arith, shape, s);
+        case 4: return single_pass<4>(op, a, count, arith, shape, s);
+        case 5: return single_pass<5>(op, a, count, arith, shape, s);
+        case 6: return single_pass<6>(op, a, count, arith, shape, s);
+        case 7: return single_pass<7>(op, a, count, arith, shape, s);
+        case 8: return single_pass<8>(op, a, count, arith, shape, s);
+        case 9: return single_pass<9>(op, a, count, arith, shape, s);
+        case 10: return single_pass<10>(op, a, count, arith, shape, s);
+        case 11: return single_pass<11>(op, a, count, arith, shape, s);
+        case 12: return single_pass<12>(op, a, count, arith, shape, s);
+        case 13: return single_pass<13>(op, a, count, arith, shape, s);
         case 14: return two_pass<2>(op, a, count, scratch, s);
         case 15: return two_pass<3>(op, a, count, scratch, s);
         case 16: return two_pass<4>(op, a, count, scratch, s);

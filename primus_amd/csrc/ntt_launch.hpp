@@ -12,6 +12,14 @@ constexpr uint32_t kMinLogN = 1;
 constexpr uint32_t kMaxLogN = 17;
 constexpr uint32_t kMaxSinglePassLogN = 13;
 constexpr uint64_t kF64MaxModulus = 1ull << 50;
+constexpr uint64_t kCanonIntMaxModulus = 1ull << 61;
+
+// Arithmetic a launch may use, as bits: kArithF64 (every modulus < 2^50:
+// F64Arith for canonical-output operations), kArithCanonInt (every modulus
+// < 2^61: IntCanonArith for canonical-output operations).  Lazy-output
+// operations always run IntArith, whose lazy values are the reference's.
+constexpr int kArithF64 = 2;
+constexpr int kArithCanonInt = 1;
 
 // Kernel arguments, passed by value.
 struct KArgs {
@@ -44,9 +52,10 @@ bool needs_scratch(Op op, uint32_t log_n);
 constexpr int kShapeStream = 16;
 inline bool valid_shape(int shape) { return shape == 0 || shape == 3 || shape == 4 || shape == 19 || shape == 20; }
 
-// f64: use F64Arith where it applies (every modulus < 2^50, canonical outputs,
-// 2^6 <= N <= 2^13); otherwise IntArith.
-hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, bool f64,
+// arith: kArithF64 / kArithCanonInt bits.  F64Arith where it applies
+// (kArithF64, canonical outputs, 2^6 <= N <= 2^13), else IntCanonArith for
+// canonical single-pass outputs (kArithCanonInt), else IntArith.
+hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, int arith,
                             int shape, hipStream_t s);
 hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, uint64_t coeff,
                            uint64_t degree, int kind, hipStream_t s);

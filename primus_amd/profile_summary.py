@@ -2,6 +2,8 @@
 
     python -m primus_amd.profile_summary stats RESULTS_DB
         per-kernel dispatch count, grid, average / min / max duration
+    python -m primus_amd.profile_summary sq RESULTS_DB KERNEL_SUBSTRING
+        every counter of the run averaged per dispatch of the kernel
     python -m primus_amd.profile_summary pmc FETCH_DB WRITE_DB KERNEL_SUBSTRING COPY_BYTES [JSON_FILE]
         FETCH_SIZE and WRITE_SIZE per dispatch of the kernel, calibrated by the
         byte counts the same counters give for the largest device-to-device
@@ -14,10 +16,10 @@ import sqlite3
 import sys
 
 
-def _rows(db, sql):
+def _rows(db, sql, params=()):
     con = sqlite3.connect(db)
     try:
-        return list(con.execute(sql))
+        return list(con.execute(sql, params))
     finally:
         con.close()
 
@@ -85,8 +87,24 @@ def pmc(fetch_db, write_db, kernel, copy_bytes, json_file=None):
             fh.write("\n")
 
 
+def sq(db, kernel):
+    """Per-dispatch averages of every counter collected for the kernel."""
+    rows = _rows(db, "select counter_name, dispatch_id, sum(value) from counters_collection "
+                     "where kernel_name like ? group by counter_name, dispatch_id", ("%" + kernel + "%",))
+    per = {}
+    for name, _, v in rows:
+        per.setdefault(name, []).append(v)
+    print("| counter | dispatches | average per dispatch |")
+    print("|---|---|---|")
+    for name in sorted(per):
+        vals = per[name]
+        print("| %s | %d | %.0f |" % (name, len(vals), sum(vals) / len(vals)))
+
+
 if __name__ == "__main__":
-    if len(sys.argv) >= 3 and sys.argv[1] == "stats":
+    if len(sys.argv) >= 4 and sys.argv[1] == "sq":
+        sq(sys.argv[2], sys.argv[3])
+    elif len(sys.argv) >= 3 and sys.argv[1] == "stats":
         stats(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 0)
     elif len(sys.argv) >= 6 and sys.argv[1] == "pmc":
         pmc(*sys.argv[2:7])

@@ -99,7 +99,8 @@ def test_all_degrees_bit_exact(log_n):
 
 
 @pytest.mark.parametrize("q", [17, 12289, 2147473409, 4294957057, 1125899906826241, 1125899906260993,
-                               1152921504606830593, 4611686018427322369])
+                               576460752303415297, 1152921504606830593, 2305843009213683713,
+                               4611686018427322369])
 def test_moduli_edge_cases(q):
     log_n = min(10, ((q - 1) & -(q - 1)).bit_length() - 2)
     n = 1 << log_n
@@ -308,6 +309,44 @@ def test_f64_extreme_operands(q):
         pa.synchronize()
         want = ot.inverse_transform(O.pointwise_mul(ot.transform(b), ot.transform(np.full(n, q - 1, dtype=np.uint64)), q))
         np.testing.assert_array_equal(da.download(), want)
+
+
+@pytest.mark.parametrize("q", [576460752303415297, 2305843009213554689, 4611686018427322369, 1125899906826241])
+def test_integer_extreme_operands(q):
+    """The integer butterflies at their largest magnitudes: the approximate-
+    quotient path (q < 2^61, intermediates up to 8q) at a 59-bit prime and at
+    the largest prime below 2^61 with q = 1 mod 8192, and the exact path above
+    it, with every input 4q - 1 (or q - 1) and every operand q - 1."""
+    log_n = 12
+    n = 1 << log_n
+    tab = pa.UintCrtNttTable.new(log_n, [q], integer_only=True)
+    ot = O.Table(log_n, q)
+    rng = np.random.default_rng(q % 991)
+    cases = [np.full(n, q - 1, dtype=np.uint64), np.full(n, 4 * q - 1, dtype=np.uint64),
+             np.where(rng.integers(0, 2, size=n) == 1, 4 * q - 1, 0).astype(np.uint64)]
+    for a in cases:
+        fw = ot.transform(a)
+        x = pa.DeviceBuffer.from_array(a)
+        pa.transform_device(tab, x.ptr, 1)
+        pa.synchronize()
+        np.testing.assert_array_equal(x.download(), fw)
+        for opv in [np.full(n, q - 1, dtype=np.uint64), rng.integers(0, q, size=n, dtype=np.uint64)]:
+            op = pa.Operand(tab, opv)
+            d = pa.DeviceBuffer.from_array(a)
+            pa.fwd_mul_inv_device(tab, d.ptr, 1, op)
+            pa.synchronize()
+            np.testing.assert_array_equal(d.download(), ot.inverse_transform(O.pointwise_mul(fw, opv, q)))
+    b = np.full(n, 2 * q - 1, dtype=np.uint64)
+    y = pa.DeviceBuffer.from_array(b)
+    pa.inverse_transform_device(tab, y.ptr, 1)
+    pa.synchronize()
+    np.testing.assert_array_equal(y.download(), ot.inverse_transform(b))
+    qm = np.full(n, q - 1, dtype=np.uint64)
+    da, db = pa.DeviceBuffer.from_array(qm), pa.DeviceBuffer.from_array(qm)
+    pa.negacyclic_mul_device(tab, da.ptr, db.ptr, 1)
+    pa.synchronize()
+    np.testing.assert_array_equal(da.download(), ot.inverse_transform(O.pointwise_mul(ot.transform(qm),
+                                                                                      ot.transform(qm), q)))
 
 
 @pytest.mark.parametrize("log_n", [10, 11, 12, 13])
