@@ -35,7 +35,7 @@ struct ph_ntt_table {
     ShoupF *d_fwd_f = nullptr, *d_inv_f = nullptr;  // binary64 twiddles (q < 2^50)
     ShoupF *d_perm[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [loge - 3][fwd, inv], register order
     bool f64 = false;                                // canonical ops may use F64Arith
-    bool wide = false;                               // canonical ops may use IntCanonArith (q < 2^61)
+    bool wide = false;                               // canonical ops may use IntCanonArith (q < 2^60)
     DevTable dev{};                    // device view (pointers into d_*)
     DevTable *d_tab = nullptr;         // the view itself in device memory (L = 1)
 };
@@ -43,7 +43,7 @@ struct ph_ntt_table {
 struct ph_dcrt_table {
     uint32_t log_n, L;
     bool f64 = false;
-    bool wide = false;  // every modulus below 2^61
+    bool wide = false;  // every modulus below 2^60
     bool owns = true;  // false for a view over tables owned elsewhere
     std::vector<ph_ntt_table *> tables;
     DevTable *d_tabs = nullptr;
@@ -279,7 +279,8 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
     DevTable &d = t->dev;
     d.q = q;
     d.q2 = q << 1;
-    d.q4 = q << 2;  // meaningful for q < 2^62; IntCanonArith uses it for q < 2^61
+    d.q4 = q << 2;  // meaningful for q < 2^62
+    d.q8 = q << 3;  // meaningful for q < 2^61; IntCanonArith uses both for q < 2^60
     d.inv_n = t->inv_n;
     d.inv_n_q = t->inv_n_q;
     // scaled root of the last inverse stage: N^-1 * inv_root_powers[N-1] (:272-277)
@@ -1392,7 +1393,7 @@ struct ph_rlwe_ctx {
     std::vector<ph_ntt_table *> tables;  // [(log_n - min_log_n) * max_limbs + i]
     DevTable *d_tabs = nullptr;
     int cus = 0;
-    bool wide = true;  // every modulus below 2^61: IntCanonArith kernels
+    bool wide = true;  // every modulus below 2^60: IntCanonArith kernels
 };
 
 void rlwe_ctx_free(ph_rlwe_ctx *c) {

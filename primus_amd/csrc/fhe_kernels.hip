@@ -332,37 +332,43 @@ __device__ __forceinline__ uint64_t rlwe_x(const RlweItem &it, int8_t e, uint8_t
 // canonical (a*s + x) mod q.  After the forward rounds thread t holds slots
 // t*E + j of s_hat and x_hat; both go through LDS so that the MAC runs in row
 // order and a_hat and b_hat move as whole rows (a wave's load or store covers
-// 64 consecutive words instead of 64 words E apart).
+// 64 consecutive words instead of 64 words E apart).  s_hat and x_hat are
+// staged as the forward rounds leave them (below 8q): the Barrett reduction
+// of a*s + x makes the sum canonical whatever their representatives, since
+// a*s + x < 8q^2 + 8q stays far below 2^127, where its quotient estimate is
+// at most one short.
 template <class A, int LOGC, int LOGT>
 __device__ __forceinline__ void rlwe_mac_rows(const RlweArgs &a, const RlweItem &it, uint64_t base, const uint64_t *x,
                                               const uint64_t *sx, uint64_t *lds, uint32_t tid, const DevTable &t) {
     using G = Geo<LOGC, LOGT>;
-    uint64_t as[G::E];
+    uint64_t xv[G::E], av[G::E];
     __syncthreads();  // the rounds' last LDS reads are done
 #pragma unroll
-    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = A::canon_fwd(sx[j], t);
+    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = x[j];
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < G::E; ++i) as[i] = ld_stream(a.a + it.row + base + tid + ((uint32_t)i << LOGT));
-    __builtin_amdgcn_sched_barrier(0);  // every row load issued before the first product
+    for (int i = 0; i < G::E; ++i) xv[i] = lds[lds_pos(tid + ((uint32_t)i << LOGT))];
 #pragma unroll
-    for (int i = 0; i < G::E; ++i) {
-        const uint32_t k = tid + ((uint32_t)i << LOGT);
-        as[i] = barrett_mul(as[i], lds[lds_pos(k)], t.q, t.br0, t.br1);
-    }
+    for (int i = 0; i < G::E; ++i) av[i] = ld_stream(a.a + it.row + base + tid + ((uint32_t)i << LOGT));
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = A::canon_fwd(x[j], t);
+    for (int j = 0; j < G::E; ++j) lds[lds_pos((tid << G::LOGE) + j)] = sx[j];
     __syncthreads();
+    // four products at a time: unfenced, the scheduler interleaves all E
+    // Barrett reductions and their temporaries no longer fit 128 VGPRs
 #pragma unroll
-    for (int i = 0; i < G::E; ++i) {
-        const uint32_t k = tid + ((uint32_t)i << LOGT);
-        st_stream(a.out + it.row + base + k, add_mod(as[i], lds[lds_pos(k)], t.q));
+    for (int i0 = 0; i0 < G::E; i0 += 4) {
+#pragma unroll
+        for (int i = i0; i < i0 + 4 && i < G::E; ++i) {
+            const uint32_t k = tid + ((uint32_t)i << LOGT);
+            st_stream(a.out + it.row + base + k, barrett_mul_add(av[i], lds[lds_pos(k)], xv[i], t.q, t.br0, t.br1));
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // N <= 2^12: both transforms in LDS, 256 threads, E = N / 256.  A: IntArith,
-// or IntCanonArith when every modulus is below 2^61 (only canonical values
+// or IntCanonArith when every modulus is below 2^60 (only canonical values
 // leave the kernel).
 template <class A, int LOGN>
 __device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &it, const DevTable &t, uint64_t *lds,
@@ -395,6 +401,24 @@ __device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &i
     rlwe_mac_rows<A, LOGN, LOGT>(a, it, 0, x, sx, lds, tid, t);
 }
 
+// Forward stage 0 of a column of the ternary secret: the multiplied inputs
+// are -1, 0 or 1, so each product w*s is 0, w or q - w (w = root_powers[1])
+// and no modular multiply is needed.  Outputs are below 2q, a valid input
+// of the stages after it in either integer arithmetic.
+template <int R>
+__device__ __forceinline__ void ternary_stage0(uint64_t *v, const int8_t *sv, const DevTable &t) {
+    constexpr int H = 1 << (R - 1);
+    const uint64_t w = ld(as_global(t.fwd) + 1).w;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        const int8_t b = sv[j + H];
+        const uint64_t p = b == 0 ? 0ull : (b > 0 ? w : t.q - w);
+        const uint64_t a = v[j];  // canonical
+        v[j] = a + p;
+        v[j + H] = a + t.q - p;
+    }
+}
+
 // N = 2^(12+R): column pass (stages 0..R-1) of s into scratch and of x into
 // the output row, then 2^R chunk passes of 4096 in LDS with the MAC fused.
 template <class A, int R>
@@ -421,7 +445,11 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
             v[i] = lift_small(sv[i], t.q);
             w[i] = rlwe_x(it, ev[i], mv[i], t);
         }
-        ct_round<A, R, false>(v, 0, 0, A::fwd(t), t);
+        ternary_stage0<R>(v, sv, t);
+        if constexpr (R > 1) {
+            ct_round<A, R - 1, false>(v, 1, 0, A::fwd(t), t);
+            ct_round<A, R - 1, false>(v + (1 << (R - 1)), 1, 1, A::fwd(t), t);
+        }
         ct_round<A, R, false>(w, 0, 0, A::fwd(t), t);
 #pragma unroll
         for (int i = 0; i < (1 << R); ++i) {

@@ -94,7 +94,7 @@ struct RlweArgs {
     const uint8_t *m;       // message
     uint64_t *scratch;      // scratch_words per workgroup (ring degrees above 2^12)
     uint64_t scratch_words; // the largest ring degree of the launch
-    uint32_t canon_int;     // every modulus below 2^61: IntCanonArith kernel, else IntArith
+    uint32_t canon_int;     // every modulus below 2^60: IntCanonArith kernel, else IntArith
     uint32_t pad2_;
 };
 

@@ -89,7 +89,7 @@ __device__ __forceinline__ Tw ld_lane(__amdgpu_buffer_rsrc_t r, uint32_t lane_by
 
 // Per-modulus device view of one UintNttTable.
 struct DevTable {
-    uint64_t q, q2, q4;       // q4 = 4q (used by IntCanonArith, q < 2^61)
+    uint64_t q, q2, q4, q8;   // q4 = 4q, q8 = 8q (IntCanonArith, q < 2^60)
     uint64_t inv_n, inv_n_q;  // N^-1 (Shoup)
     uint64_t sr, sr_q;        // N^-1 * inv_root_powers[N-1] (Shoup)
     uint64_t br0, br1;        // Barrett ratio floor(2^128/q)
@@ -125,7 +125,9 @@ struct IntArith {
     __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t, int) { return fwd(t); }
     __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t, int) { return inv(t); }
 
-    // Harvey forward butterfly, primitive.rs:225-228.
+    // Harvey forward butterfly, primitive.rs:225-228 (every stage reduces:
+    // REDUCE is for IntCanonArith's schedule).
+    template <bool REDUCE = true>
     __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
         uint64_t u = sub_if_ge(a, t.q2);
         uint64_t v = shoup_lazy(w.w, w.wq, b, t.q);
@@ -169,15 +171,20 @@ struct IntArith {
 };
 
 // ---------------------------------------------------------------------------
-// IntCanonArith (q < 2^61, canonical outputs only)
+// IntCanonArith (q < 2^60, canonical outputs only)
 // ---------------------------------------------------------------------------
 
 // The integer butterflies for operations whose callers see only canonical
-// values, with the headroom of q < 2^61 (8q < 2^64) spent on a cheaper
-// quotient: shoup_approx (three multiply-adds instead of four multiplies;
-// products in [0, 4q)), one conditional subtraction of 4q per butterfly, and
-// every intermediate below 8q.  The last inverse stage uses exact Shoup
-// products, whose outputs are canonical after one subtraction of q.
+// values, with the headroom of q < 2^60 (16q < 2^64) spent on cheaper
+// steps.  Products use shoup_approx (two multiply-highs and a multiply-add for
+// the quotient instead of four multiplies; results in [0, 4q)).  The forward
+// reduces its unmultiplied inputs only on alternate stages (ct<REDUCE>): a
+// stage that reduces (u = a - 8q if a >= 8q) takes inputs below 16q to
+// outputs below 12q, a stage that does not takes inputs below 12q to outputs
+// below 16q, and no two stages in a row skip it (ct_round reduces on its even
+// local stages, so its first stage always does).  The inverse subtracts 4q
+// from each sum (intermediates below 8q) and ends with exact Shoup products,
+// canonical after one subtraction of q.
 struct IntCanonArith {
     using T = uint64_t;
     using Tw = Shoup;
@@ -188,9 +195,10 @@ struct IntCanonArith {
     __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t, int) { return fwd(t); }
     __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t, int) { return inv(t); }
 
-    // forward: inputs below 8q; u < 4q, v < 4q, outputs below 8q
+    // forward, see above: u < 12q, v < 4q, outputs below 16q
+    template <bool REDUCE = true>
     __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
-        const uint64_t u = sub_if_ge(a, t.q4);
+        const uint64_t u = REDUCE ? sub_if_ge(a, t.q8) : a;
         const uint64_t v = shoup_approx(w.w, w.wq, b, t.q);
         a = u + v;
         b = u + t.q4 - v;
@@ -213,8 +221,9 @@ struct IntCanonArith {
     }
     __device__ __forceinline__ static T load(uint64_t v, const DevTable &) { return v; }      // [0, 2q)
     __device__ __forceinline__ static T load_fwd(uint64_t v, const DevTable &) { return v; }  // [0, 4q)
+    // forward output, below 16q
     __device__ __forceinline__ static uint64_t canon_fwd(T v, const DevTable &t) {
-        return sub_if_ge(sub_if_ge(sub_if_ge(v, t.q4), t.q2), t.q);
+        return sub_if_ge(sub_if_ge(sub_if_ge(sub_if_ge(v, t.q8), t.q4), t.q2), t.q);
     }
     __device__ __forceinline__ static uint64_t out_inv(T v, const DevTable &) { return v; }
     // forward output times a resident operand: [0, 4q), an inverse input as it is
@@ -277,6 +286,7 @@ struct F64Arith {
     // inverse of a fused forward -> multiply -> inverse ends in gs_last_prescaled
     static constexpr bool kScaledOperand = true;
     __device__ __forceinline__ static void round_start(T &a, const DevTable &t) { a = reduce(a, t); }
+    template <bool REDUCE = true>  // unused: F64Arith reduces at round starts
     __device__ __forceinline__ static void ct(T &a, T &b, const Tw &w, const DevTable &t) {
         const double v = mulmod(b, w.w, w.wp, t.qd);
         const double u = a;
@@ -389,7 +399,10 @@ __device__ __forceinline__ void ct_round(typename A::T *x, uint32_t S, uint32_t 
 #pragma unroll
             for (int jj = 0; jj < half; ++jj) {
                 const int j0 = sub * 2 * half + jj;
-                A::ct(x[j0], x[j0 + half], w[sub], t);
+                // even local stages reduce (IntCanonArith's schedule); u is a
+                // constant once the loop is unrolled, so the branch folds away
+                if (u & 1) A::template ct<false>(x[j0], x[j0 + half], w[sub], t);
+                else A::template ct<true>(x[j0], x[j0 + half], w[sub], t);
             }
         }
         // one stage at a time: hoisting every stage's twiddles to the top of

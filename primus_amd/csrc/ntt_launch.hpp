@@ -12,11 +12,11 @@ constexpr uint32_t kMinLogN = 1;
 constexpr uint32_t kMaxLogN = 17;
 constexpr uint32_t kMaxSinglePassLogN = 13;
 constexpr uint64_t kF64MaxModulus = 1ull << 50;
-constexpr uint64_t kCanonIntMaxModulus = 1ull << 61;
+constexpr uint64_t kCanonIntMaxModulus = 1ull << 60;
 
 // Arithmetic a launch may use, as bits: kArithF64 (every modulus < 2^50:
 // F64Arith for canonical-output operations), kArithCanonInt (every modulus
-// < 2^61: IntCanonArith for canonical-output operations).  Lazy-output
+// < 2^60: IntCanonArith for canonical-output operations).  Lazy-output
 // operations always run IntArith, whose lazy values are the reference's.
 constexpr int kArithF64 = 2;
 constexpr int kArithCanonInt = 1;
