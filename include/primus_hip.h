@@ -290,8 +290,12 @@ void ph_ext_key_rns_free(ph_ext_key_rns *key);
  * (primus_rns/src/base.rs:396-450), its gadget digits, each digit wrapped
  * into every limb (base.rs:201-236), transformed, and multiply-accumulated
  * with the key (glwe/dcrt.rs:170-247); then, unless ntt_output, the inverse
- * transform of the accumulators.  A digit pass, the batched transforms and a
- * multiply-accumulate pass, on the stream; scratch by hipMallocAsync. */
+ * transform of the accumulators.  For 2^6 <= N <= 2^12 with the digits
+ * inside one 64-bit window of the composed value (l * log_basis, plus one
+ * bit when bits are dropped, at most 64) this is one fused launch per batch
+ * that allocates nothing and can be captured in a HIP graph; otherwise a digit
+ * pass, the batched transforms and a multiply-accumulate pass, with scratch
+ * by hipMallocAsync on the stream. */
 int ph_ext_product_rns_device(const ph_ext_key_rns *key, const uint64_t *d_in, uint64_t *d_out, size_t count,
                               int ntt_output, void *stream);
 

@@ -1185,6 +1185,13 @@ struct ph_ext_key_rns {
     const ph_dcrt_table *table;  // not owned
     ExtRnsArgs proto;
     Shoup *d_key = nullptr;  // the key with Shoup quotients
+    // the fused kernel's copies, rows permuted for its register layout
+    // (slot t*E + j at j*(N/E) + t), when it applies
+    bool fused = false;
+    int arith = 0;
+    uint32_t words = 0;
+    Shoup *d_key_perm = nullptr;
+    ShoupF *d_key_perm_f = nullptr;
 };
 
 int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t decompose_length, const uint64_t *key_ntt,
@@ -1301,6 +1308,33 @@ int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t deco
         delete key;
         return fail(PH_ERR_DEVICE, "key upload");
     }
+    // The fused kernel: single-pass ring degrees, digits within one 64-bit
+    // window of the composed value (l * log_basis bits, plus the carry bit
+    // below them when bits are dropped).
+    key->fused = d->log_n >= kExtMinLogN && d->log_n <= kExtMaxLogN &&
+                 (uint64_t)decompose_length * log_basis + (g.drop_bits > 0 ? 1 : 0) <= 64;
+    if (key->fused) {
+        key->arith = arith_of(d);
+        key->words = (bits + 63) / 64;
+        const uint64_t E = 1ull << kExtLogE, T = n / E, rows = 4ull * decompose_length * d->L;
+        std::vector<Shoup> hp(words);
+        std::vector<ShoupF> hpf;
+        if (d->f64) hpf.resize(words);
+        for (uint64_t r = 0; r < rows; ++r) {
+            const uint64_t q = d->tables[r % d->L]->q;
+            for (uint64_t k = 0; k < n; ++k) {
+                const uint64_t pos = r * n + (k % E) * T + k / E;
+                hp[pos] = hk[r * n + k];
+                if (d->f64) hpf[pos] = mkf(hp[pos], q);
+            }
+        }
+        int rc = upload((void **)&key->d_key_perm, hp.data(), hp.size() * sizeof(Shoup));
+        if (rc == PH_OK && d->f64) rc = upload((void **)&key->d_key_perm_f, hpf.data(), hpf.size() * sizeof(ShoupF));
+        if (rc != PH_OK) {
+            ph_ext_key_rns_free(key);
+            return rc;
+        }
+    }
     key->proto = g;
     *out = key;
     return PH_OK;
@@ -1309,6 +1343,8 @@ int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t deco
 void ph_ext_key_rns_free(ph_ext_key_rns *key) {
     if (!key) return;
     if (key->d_key) (void)hipFree(key->d_key);
+    if (key->d_key_perm) (void)hipFree(key->d_key_perm);
+    if (key->d_key_perm_f) (void)hipFree(key->d_key_perm_f);
     delete key;
 }
 
@@ -1318,6 +1354,21 @@ int ph_ext_product_rns_device(const ph_ext_key_rns *key, const uint64_t *d_in, u
     if (count == 0) return PH_OK;
     const ph_dcrt_table *d = key->table;
     hipStream_t s = (hipStream_t)stream;
+    if (key->fused) {
+        // one launch, nothing allocated: capturable in a HIP graph
+        ExtRnsFusedArgs f{};
+        f.g = key->proto;
+        f.g.in = d_in;
+        f.g.digits = nullptr;
+        f.g.count = count;
+        f.out = d_out;
+        f.key = key->d_key_perm;
+        f.key_f = key->d_key_perm_f;
+        f.tabs = d->d_tabs;
+        f.ntt_output = ntt_output ? 1u : 0u;
+        hipError_t e = launch_ext_rns_fused(f, key->arith, key->words, s);
+        return e == hipSuccess ? PH_OK : hip_fail(e, "fused external product kernel");
+    }
     const uint64_t n = 1ull << d->log_n, L = d->L, l = key->proto.l;
     uint64_t *digits = nullptr;
     PH_HIP(hipMallocAsync((void **)&digits, count * 2 * l * L * n * 8, s));

@@ -13,6 +13,7 @@
 //   counts in one launch.  Follows NttRlweSecretKey::encrypt_inplace_with_embedding
 //   (primus_fhe_core/src/secret_key/rlwe.rs:227-268).
 #include "fhe_launch.hpp"
+#include "ntt_launch.hpp"
 #include "ntt_rounds.hpp"
 
 namespace ph {
@@ -313,6 +314,279 @@ hipError_t launch_ext_rns_mac(const ExtRnsMacArgs &a, hipStream_t s) {
     if (threads == 0) return hipSuccess;
     ext_rns_mac_kernel<<<dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s>>>(a);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// External product over several limbs, fused
+// ---------------------------------------------------------------------------
+
+// Multiply-accumulate of one transformed digit with a key entry, per
+// arithmetic: canonical (IntArith), below 4q (IntCanonArith: products in
+// [0, 4q), one subtraction of 4q per term), centred and reduced every third
+// term (F64Arith, as ext_mac).
+template <class A>
+struct ExtRnsOps;
+template <>
+struct ExtRnsOps<IntArith> {
+    __device__ __forceinline__ static void mac(uint64_t &acc, uint64_t x, const Shoup &k, const DevTable &t, bool) {
+        acc = add_mod(acc, shoup_full(k.w, k.wq, x, t.q), t.q);
+    }
+    __device__ __forceinline__ static uint64_t finish(uint64_t acc, const DevTable &) { return acc; }
+    __device__ __forceinline__ static uint64_t canon(uint64_t acc, const DevTable &) { return acc; }
+};
+template <>
+struct ExtRnsOps<IntCanonArith> {
+    __device__ __forceinline__ static void mac(uint64_t &acc, uint64_t x, const Shoup &k, const DevTable &t, bool) {
+        acc = sub_if_ge(acc + shoup_approx(k.w, k.wq, x, t.q), t.q4);
+    }
+    __device__ __forceinline__ static uint64_t finish(uint64_t acc, const DevTable &) { return acc; }  // < 4q
+    __device__ __forceinline__ static uint64_t canon(uint64_t acc, const DevTable &t) {
+        return sub_if_ge(sub_if_ge(acc, t.q2), t.q);
+    }
+};
+template <>
+struct ExtRnsOps<F64Arith> {
+    __device__ __forceinline__ static void mac(double &acc, double x, const ShoupF &k, const DevTable &t, bool red) {
+        ext_mac(acc, x, k, t, red);
+    }
+    __device__ __forceinline__ static double finish(double acc, const DevTable &t) { return F64Arith::reduce(acc, t); }
+    __device__ __forceinline__ static uint64_t canon(double acc, const DevTable &t) { return F64Arith::to_canon(acc, t); }
+};
+__device__ __forceinline__ gptr<Shoup> ext_rns_key(const ExtRnsFusedArgs &a, const IntArith *) {
+    return as_global(a.key);
+}
+__device__ __forceinline__ gptr<Shoup> ext_rns_key(const ExtRnsFusedArgs &a, const IntCanonArith *) {
+    return as_global(a.key);
+}
+__device__ __forceinline__ gptr<ShoupF> ext_rns_key(const ExtRnsFusedArgs &a, const F64Arith *) {
+    return as_global(a.key_f);
+}
+
+// One coefficient of one input polynomial composed into W words (the value
+// below Q, compose_inplace, primus_rns/src/base.rs:396-420) and shifted by
+// the split adjustment (init_value_carry, primus_decompose/src/big_integer/
+// basis.rs:307-331).  Returns its digit window: the 64 bits from bit
+// drop_bits - 1 (the carry into the first digit, when drop_bits > 0) or from
+// bit 0, which hold every digit when l * log_basis + 1 <= 64.  src: the
+// coefficient in limb 0, limbs N words apart.
+template <int W>
+__device__ __forceinline__ uint64_t ext_rns_window(const ExtRnsArgs &g, const uint64_t *src, uint64_t n) {
+    uint64_t v[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) v[w] = 0;
+    for (uint32_t i = 0; i < g.L; ++i) {
+        const uint64_t prod = shoup_full(g.inv[i], g.inv_q[i], src[(uint64_t)i * n], g.q[i]);
+        uint64_t carry = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint64_t lo = prod * g.punct[i][w], hi = mulhi64(prod, g.punct[i][w]);
+            const uint64_t s1 = v[w] + lo;
+            uint64_t c = s1 < lo;
+            const uint64_t s2 = s1 + carry;
+            c += s2 < carry;
+            v[w] = s2;
+            carry = hi + c;
+        }
+        // v >= Q (or a carry out of the W words): subtract Q
+        bool ge = carry != 0;
+        if (!ge) {
+            ge = true;
+#pragma unroll
+            for (int w = W - 1; w >= 0; --w)
+                if (v[w] != g.Q[w]) {
+                    ge = v[w] > g.Q[w];
+                    break;
+                }
+        }
+        if (ge) {
+            uint64_t borrow = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint64_t d = v[w] - g.Q[w];
+                const uint64_t b1 = v[w] < g.Q[w];
+                v[w] = d - borrow;
+                borrow = b1 | (d < borrow);
+            }
+        }
+    }
+    if (g.has_split) {
+        bool ge = true;
+#pragma unroll
+        for (int w = W - 1; w >= 0; --w)
+            if (v[w] != g.split[w]) {
+                ge = v[w] > g.split[w];
+                break;
+            }
+        if (ge) {
+            uint64_t carry = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint64_t s1 = v[w] + g.pow2_sub[w];
+                const uint64_t c1 = s1 < g.pow2_sub[w];
+                v[w] = s1 + carry;
+                carry = c1 | (v[w] < carry);
+            }
+        }
+    }
+    const uint32_t ws = g.drop_bits > 0 ? g.drop_bits - 1 : 0;
+    const uint32_t wi = ws >> 6, off = ws & 63;
+    uint64_t lo = v[0], hi = 0;
+#pragma unroll
+    for (int w = 1; w < W; ++w) {
+        if (wi == (uint32_t)w) lo = v[w];
+        if (wi + 1 == (uint32_t)w) hi = v[w];
+    }
+    return off ? (lo >> off) | (hi << (64 - off)) : lo;
+}
+
+// Digit d of a coefficient from its window, with the carry chain
+// (unsigned_decompose, primus_decompose/src/big_integer/common.rs:141-152),
+// wrapped into limb q (wrapping_decompose_small_values_inplace,
+// primus_rns/src/base.rs:201-236).  first: 1 when the window's bit 0 is the
+// carry into digit 0 (drop_bits > 0), else 0.
+__device__ __forceinline__ uint64_t ext_rns_digit(const ExtRnsArgs &g, uint64_t window, uint32_t first, uint32_t d,
+                                                  uint32_t &carry, uint64_t q) {
+    const uint64_t bits = window >> (first + d * g.log_basis);
+    const uint64_t temp = (bits & g.basis_minus_one) + carry;
+    carry = (temp & g.carry_mask) != 0;
+    const uint64_t digit = temp & g.basis_minus_one;
+    return (g.basis == 2 || digit < g.half) ? digit : q - g.basis + digit;
+}
+
+// Workgroup b handles product 8*(b / (8L)) + b % 8 and limb (b % (8L)) / 8:
+// the L workgroups of a product sit on one XCD (workgroups go to the XCDs in
+// turn) and start close together, so the L - 1 re-reads of its input rows are
+// served by that XCD's L2.
+// Requires l * log_basis + (drop_bits > 0) <= 64 (the digits fit one window).
+// Three waves per SIMD: the two accumulators, the windows, a transform and
+// the key batch need about 140 VGPRs, and 128 spills.
+template <class A, int LOGN, int W>
+__global__ __launch_bounds__(1 << (LOGN - kExtLogE), 3) void ext_rns_fused_kernel(ExtRnsFusedArgs a) {
+    constexpr int LOGT = LOGN - kExtLogE;
+    using G = Geo<LOGN, LOGT>;
+    using T = typename A::T;
+    using Ops = ExtRnsOps<A>;
+    static_assert(G::E == (1 << kExtLogE), "key permutation assumes this register layout");
+    constexpr int R0 = G::fr(0);
+    __shared__ T lds[G::LDS];
+    const ExtRnsArgs &g = a.g;
+    const uint32_t t = threadIdx.x;
+    const uint32_t group = 8u * g.L;
+    const uint32_t r = blockIdx.x % group;
+    const uint64_t prod = (uint64_t)(blockIdx.x / group) * 8u + (r & 7u);
+    const uint32_t limb = r >> 3;
+    if (prod >= g.count) return;  // the whole workgroup: no barrier reached
+    const DevTable tab = a.tabs[limb];
+    const uint64_t n = 1ull << LOGN;
+    const auto key = ext_rns_key(a, (const A *)nullptr);
+    T acc[2][G::E];
+#pragma unroll
+    for (int j = 0; j < G::E; ++j) acc[0][j] = acc[1][j] = (T)0;
+
+    uint32_t term = 0;
+    for (uint32_t p = 0; p < 2; ++p) {
+        // the thread's first-round coefficients, composed
+        const uint64_t *src = g.in + (prod * 2 + p) * g.L * n;
+        const uint32_t first = g.drop_bits > 0 ? 1u : 0u;
+        uint64_t win[G::E];
+        uint32_t carry[G::E];
+#pragma unroll
+        for (int m = 0; m < (G::E >> R0); ++m)
+#pragma unroll
+            for (int j = 0; j < (1 << R0); ++j) {
+                const int e = (m << R0) + j;
+                win[e] = ext_rns_window<W>(g, src + fwd_slot<LOGN, LOGT, 0, R0>(t, m, j), n);
+                carry[e] = (uint32_t)(win[e] & first);
+            }
+        for (uint32_t d = 0; d < g.l; ++d) {
+            T x[G::E];
+#pragma unroll
+            for (int e = 0; e < G::E; ++e)
+                x[e] = A::load_fwd(ext_rns_digit(g, win[e], first, d, carry[e], tab.q), tab);
+            // as in ext_product_kernel: keep the twiddle pointers and slot
+            // addresses from being hoisted out of the digit loop
+            DevTable td = tab;
+            opaque(td);
+            uint32_t tl = t;
+            asm volatile("" : "+v"(tl));
+            FwdRounds<A, LOGN, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, tl, 0, 0, td);
+            // thread t holds slots t*E + j; key row (p, d, c, limb) stores them at j*T + t
+            const bool red = (++term % 3) == 0;
+            using Wt = typename A::Tw;
+            const uint64_t row0 = (((uint64_t)p * g.l + d) * 2) * g.L + limb;
+            const auto k0 = table_rsrc(key + row0 * n);
+            const auto k1 = table_rsrc(key + (row0 + g.L) * n);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j0 = 0; j0 < G::E; j0 += kExtKeyBatch) {
+                Wt w0[kExtKeyBatch], w1[kExtKeyBatch];
+#pragma unroll
+                for (int jj = 0; jj < kExtKeyBatch; ++jj) {
+                    w0[jj] = ld_lane<Wt>(k0, tl * (uint32_t)sizeof(Wt), (j0 + jj) * G::T * (uint32_t)sizeof(Wt));
+                    w1[jj] = ld_lane<Wt>(k1, tl * (uint32_t)sizeof(Wt), (j0 + jj) * G::T * (uint32_t)sizeof(Wt));
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int jj = 0; jj < kExtKeyBatch; ++jj) {
+                    Ops::mac(acc[0][j0 + jj], x[j0 + jj], w0[jj], td, red);
+                    Ops::mac(acc[1][j0 + jj], x[j0 + jj], w1[jj], td, red);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        uint64_t *dst = a.out + ((prod * 2 + c) * g.L + limb) * n;
+        T y[G::E];
+#pragma unroll
+        for (int j = 0; j < G::E; ++j) y[j] = Ops::finish(acc[c][j], tab);
+        if (a.ntt_output) {
+            // NTT-form result in the reference's slot order, canonical
+            const RowBuf rb(dst, (uint32_t)n);
+#pragma unroll
+            for (int j = 0; j < G::E; ++j) rb.st(t << G::LOGE, (uint32_t)j, Ops::canon(y[j], tab));
+        } else {
+            InvRounds<A, LOGN, LOGT, 0, true, A::kRegisterOrder>::run(y, lds, t, (uint32_t)n, 0, tab);
+            InvRounds<A, LOGN, LOGT, G::NR - 1, true, A::kRegisterOrder>::store_global(dst, y, t, tab);
+        }
+    }
+}
+
+template <class A, int LOGN>
+static hipError_t launch_ext_rns_fused_n(const ExtRnsFusedArgs &a, uint32_t words, hipStream_t s) {
+    const uint64_t groups = (a.g.count + 7) / 8;
+    const dim3 grid((uint32_t)(groups * 8 * a.g.L)), block(1u << (LOGN - kExtLogE));
+    // three-word moduli products run on four words (the fourth is zero)
+    switch (words) {
+        case 1: ext_rns_fused_kernel<A, LOGN, 1><<<grid, block, 0, s>>>(a); break;
+        case 2: ext_rns_fused_kernel<A, LOGN, 2><<<grid, block, 0, s>>>(a); break;
+        case 3:
+        case 4: ext_rns_fused_kernel<A, LOGN, 4><<<grid, block, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int LOGN>
+static hipError_t launch_ext_rns_fused_a(const ExtRnsFusedArgs &a, int arith, uint32_t words, hipStream_t s) {
+    if (arith & kArithF64) return launch_ext_rns_fused_n<F64Arith, LOGN>(a, words, s);
+    if (arith & kArithCanonInt) return launch_ext_rns_fused_n<IntCanonArith, LOGN>(a, words, s);
+    return launch_ext_rns_fused_n<IntArith, LOGN>(a, words, s);
+}
+
+hipError_t launch_ext_rns_fused(const ExtRnsFusedArgs &a, int arith, uint32_t words, hipStream_t s) {
+    if (a.g.count == 0) return hipSuccess;
+    switch (a.g.log_n) {
+        case 6: return launch_ext_rns_fused_a<6>(a, arith, words, s);
+        case 7: return launch_ext_rns_fused_a<7>(a, arith, words, s);
+        case 8: return launch_ext_rns_fused_a<8>(a, arith, words, s);
+        case 9: return launch_ext_rns_fused_a<9>(a, arith, words, s);
+        case 10: return launch_ext_rns_fused_a<10>(a, arith, words, s);
+        case 11: return launch_ext_rns_fused_a<11>(a, arith, words, s);
+        case 12: return launch_ext_rns_fused_a<12>(a, arith, words, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------

@@ -67,6 +67,25 @@ struct ExtRnsMacArgs {
 hipError_t launch_ext_rns_digits(const ExtRnsArgs &a, hipStream_t s);
 hipError_t launch_ext_rns_mac(const ExtRnsMacArgs &a, hipStream_t s);
 
+// The fused form: one workgroup per (product, output limb) composes each
+// coefficient's residues, takes its digits, transforms each digit's residues
+// for its limb, multiply-accumulates them with the key and inverse-transforms
+// the two accumulators, with no intermediate in device memory.  Ring degrees
+// kExtMinLogN .. kExtMaxLogN; E = 2^kExtLogE coefficients per thread.
+struct ExtRnsFusedArgs {
+    ExtRnsArgs g;              // gadget constants and the input (g.digits unused)
+    uint64_t *out;             // [count][2][L][N]
+    const Shoup *key;          // [2][l][2][L] rows of N, slot t*E + j at j*(N/E) + t (integer arithmetic)
+    const ShoupF *key_f;       // the same as centred binary64 pairs (every limb below 2^50)
+    const DevTable *tabs;      // L tables
+    uint32_t ntt_output;
+    uint32_t pad_;
+};
+
+// arith: kArithF64 / kArithCanonInt bits (ntt_launch.hpp) of the limbs; words:
+// 64-bit words of Q (1..kExtRnsWords).
+hipError_t launch_ext_rns_fused(const ExtRnsFusedArgs &a, int arith, uint32_t words, hipStream_t s);
+
 // RLWE encryption: ring degrees 2^10 .. 2^16 in one persistent launch.
 constexpr uint32_t kRlweMinLogN = 10;
 constexpr uint32_t kRlweMaxLogN = 16;

@@ -97,6 +97,14 @@ def test_rns_ext_product_one_limb_equals_c_oracle():
     (2, 59, 8, 3, 14, 1),     # two-pass transforms
     (2, 59, 2, 59, 8, 1),     # Q of 118 bits: every bit a digit bit (drop_bits 0, split + 1)
     (2, 31, 1, 62, 8, 1),     # binary digits, drop_bits 0 (no split value)
+    # the fused kernel (digits within one 64-bit window) in each arithmetic
+    # and word count, at the smallest and largest ring degrees it serves
+    (2, 62, 8, 3, 10, 2),     # integer Shoup, limbs >= 2^60
+    (4, 62, 20, 3, 12, 2),    # four words, window of 61 bits
+    (3, 50, 16, 3, 11, 3),    # binary64, three words
+    (2, 50, 8, 3, 6, 5),      # N = 2^6
+    (2, 59, 8, 3, 12, 9),     # approximate-quotient integer path, N = 2^12
+    (1, 59, 20, 2, 9, 4),     # one limb, one word
 ])
 def test_gpu_ext_product_rns(L, bits, log_b, l, log_n, count):
     from primus_amd.fhe import ExtKeyRns

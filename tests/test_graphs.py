@@ -142,6 +142,42 @@ def test_graph_ext_product():
     _replay_and_check(torch, g, stream, [dout], [want])
 
 
+@pytest.mark.parametrize("bits,L", [(59, 2), (50, 3), (62, 2)])
+def test_graph_ext_product_rns(bits, L):
+    """The fused external product over several limbs (one launch, no
+    allocation), captured and replayed, for each of its arithmetics."""
+    torch = _torch()
+    import primus_amd as pa
+    from primus_amd.fhe import ExtKeyRns
+
+    log_n, count, l = 10, 24, 3
+    n = 1 << log_n
+    moduli = pa.ntt_primes_below(bits, 2 * n, L)
+    table = pa.UintCrtNttTable.new(log_n, moduli)
+    rng = np.random.default_rng(29 + L)
+    qcol = np.array(moduli, dtype=np.uint64).reshape(-1, 1)
+    key_np = np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol
+                       for _ in range(4 * l)]).reshape(2, l, 2, L, n)
+    key = ExtKeyRns(table, 8, l, key_np)
+    x = np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol
+                  for _ in range(2 * count)]).reshape(count, 2, L, n)
+    dev = torch.device("cuda:0")
+    din = torch.from_numpy(x.view(np.int64)).to(dev)
+    dout = torch.zeros_like(din)
+    stream = torch.cuda.Stream(device=dev)
+    key.product_device(_vp(din), _vp(dout), count, stream=stream.cuda_stream)
+    stream.synchronize()
+    want = dout.cpu().numpy().view(np.uint64).copy()
+    tables = [O.Table(log_n, q) for q in moduli]
+    np.testing.assert_array_equal(want[count - 1], O.ext_product_rns(tables, O.RnsBasis(moduli, 8, l), key_np,
+                                                                    x[count - 1]))
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=stream):
+        key.product_device(_vp(din), _vp(dout), count, stream=stream.cuda_stream)
+    _replay_and_check(torch, g, stream, [dout], [want])
+
+
 def test_graph_rlwe_sample_and_encrypt():
     """Device sampling and encryption of an uneven batch, both from one plan,
     captured together and replayed."""
