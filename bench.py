@@ -306,40 +306,41 @@ def secondary_ext_product(pa, timer, sp):
 
 
 def secondary_ext_product_rns(pa, timer, sp):
-    """External product over 2 limbs of 59-bit primes (Q of 118 bits): 1024
-    products, N=1024, base 2^8, l=3 (the top 24 bits of Q), one digit pass,
-    the batched digit transforms, one multiply-accumulate pass, the inverse."""
+    """External product over 2 limbs (Q of 118 bits for 59-bit primes, 100 bits
+    for 50-bit ones): 1024 products, N=1024, base 2^8, l=3 (the top 24 bits of
+    Q), one fused launch per batch (composition, digits, digit transforms,
+    multiply-accumulate with the key, inverse transforms)."""
     from primus_amd.fhe import ExtKeyRns
 
     log_n, L, l, count = 10, 2, 3, 1024
     n = 1 << log_n
-    moduli = pa.ntt_primes_below(59, 2 * n, L)
-    table = pa.UintCrtNttTable.new(log_n, moduli)
-    rng = np.random.default_rng(SEED + 8)
-    qcol = np.array(moduli, dtype=np.uint64).reshape(-1, 1)
-    key = ExtKeyRns(table, 8, l, np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol
-                                           for _ in range(4 * l)]).reshape(2, l, 2, L, n))
-    x = np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol for _ in range(2 * count)])
-    rot = 4  # 4 x 64 MiB in + out
-    pairs = [(pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(count * 2 * L * n)) for _ in range(rot)]
-    # the launch shapes of its digit transforms and inverse transforms, tuned on
-    # this batch's polynomial counts (outputs do not depend on them)
-    from primus_amd import tuning
-    shapes = {op: tuning.autotune(table, op, cnt, reps=5)[0]
-              for op, cnt in (("forward", count * 2 * l * L), ("inverse", count * 2 * L))}
+    out = {}
+    for bits in (59, 50):
+        moduli = pa.ntt_primes_below(bits, 2 * n, L)
+        table = pa.UintCrtNttTable.new(log_n, moduli)
+        rng = np.random.default_rng(SEED + 8)
+        qcol = np.array(moduli, dtype=np.uint64).reshape(-1, 1)
+        key = ExtKeyRns(table, 8, l, np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol
+                                               for _ in range(4 * l)]).reshape(2, l, 2, L, n))
+        x = np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol for _ in range(2 * count)])
+        rot = 4  # 4 x 64 MiB in + out
+        pairs = [(pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(count * 2 * L * n)) for _ in range(rot)]
 
-    def step(i):
-        din, dout = pairs[i % rot]
-        key.product_device(din.ptr, dout.ptr, count, stream=sp)
+        def step(i, pairs=pairs, key=key):
+            din, dout = pairs[i % rot]
+            key.product_device(din.ptr, dout.ptr, count, stream=sp)
 
-    per, tot = timer.run(step, 200, 10)
-    ms = tot / 200
-    return {"workload": "1024 external products over 2 limbs (59-bit primes), N=1024, B=2^8, l=3: digit pass, "
-                        "digit transforms, multiply-accumulate, inverse",
+        per, tot = timer.run(step, 200, 10)
+        ms = tot / 200
+        out["%d_bit" % bits] = {
+            "moduli": moduli, "arith": table.arith() if bits == 50 else "integer (approximate-quotient Shoup)",
             "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
             "ntt_per_s": (2 * l + 2) * L * count / (ms * 1e-3),
-            "algorithmic_gbps": 4 * L * n * 8 * count / (ms * 1e-3) / 1e9, "rotating_buffers": rot,
-            "transform_shapes": {op: tuning.describe(v) for op, v in shapes.items()}}
+            "algorithmic_gbps": 4 * L * n * 8 * count / (ms * 1e-3) / 1e9,
+            "hbm_frac": 4 * L * n * 8 * count / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, "rotating_buffers": rot}
+    out["workload"] = ("1024 external products over 2 limbs, N=1024, B=2^8, l=3: one fused launch per batch "
+                       "(composition, digits, digit transforms, multiply-accumulate, inverse)")
+    return out
 
 
 def secondary_automorphism(pa, timer, sp):

@@ -208,6 +208,15 @@ int ph_dcrt_negacyclic_mul_device_noalloc(const ph_dcrt_table *table, uint64_t *
  * index is L or more is left unchanged. */
 int ph_dcrt_negacyclic_mul_rows_device(const ph_dcrt_table *table, uint64_t *d_a, const uint64_t *d_b,
                                        const uint8_t *d_limb_of, size_t count, void *stream);
+/* x <- factors[i] * x (mod q_i) over `count` limb-polynomials (limb i = poly % L)
+ * through MultiplyFactor (primus_factor/src/multiply.rs:4-76) at bit_shift 64
+ * (ShoupFactor: NttPolynomial::mul_factor_assign, primus_poly/src/ntt/mul.rs:63-68),
+ * 52 or 32 (HEXL's factor forms, hexl/mod.rs:231-271); lazy != 0 leaves
+ * lazy_mul_modulo's [0, 2q) values, else mul_modulo's [0, q).  factors[i] <= q_i
+ * (host memory); at shifts 32 and 52 every modulus and input must be below
+ * 2^bit_shift (checked for the moduli). */
+int ph_dcrt_mul_factor_device(const ph_dcrt_table *table, uint64_t *d_polys, size_t count, const uint64_t *factors,
+                              uint32_t bit_shift, int lazy, void *stream);
 /* NttPolynomial::mul_inplace (primus_poly/src/ntt/mul.rs:105-119): d_out = d_a * d_b. */
 int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *table, const uint64_t *d_a, const uint64_t *d_b,
                                  uint64_t *d_out, size_t count, void *stream);

@@ -58,6 +58,8 @@ def lib():
             ("po_lazy_transform", None, [ctypes.c_void_p, _p64]),
             ("po_transform", None, [ctypes.c_void_p, _p64]),
             ("po_lazy_inverse_transform", None, [ctypes.c_void_p, _p64]),
+            ("po_hexl_forward", None, [ctypes.c_void_p, _p64, ctypes.c_uint32]),
+            ("po_hexl_inverse", None, [ctypes.c_void_p, _p64, ctypes.c_uint32]),
             ("po_inverse_transform", None, [ctypes.c_void_p, _p64]),
             ("po_transform_monomial", None, [ctypes.c_void_p, _u64, ctypes.c_size_t, _p64]),
             ("po_transform_coeff_one_monomial", None, [ctypes.c_void_p, ctypes.c_size_t, _p64]),
@@ -175,6 +177,14 @@ class Table:
 
     def lazy_inverse_transform(self, x):
         return self._run(lib().po_lazy_inverse_transform, x)
+
+    def hexl_forward(self, x, output_mod_factor):
+        """HexlNttTable::compute_forward on its radix-2 path (hexl/radix2.rs)."""
+        return self._run(lambda h, p: lib().po_hexl_forward(h, p, output_mod_factor), x)
+
+    def hexl_inverse(self, x, output_mod_factor):
+        """HexlNttTable::compute_inverse on its radix-2 path (hexl/radix2.rs)."""
+        return self._run(lambda h, p: lib().po_hexl_inverse(h, p, output_mod_factor), x)
 
     def transform_monomial(self, coeff, degree):
         out = np.zeros(self.n, dtype=np.uint64)

@@ -357,6 +357,81 @@ void po_inverse_transform(const po_table *t, uint64_t *values) {
     for (size_t i = 0; i < t->n; ++i) values[i] = reduce_once(values[i], t->q);
 }
 
+/* HexlNttTable's portable radix-2 path (the one compute_forward and
+ * compute_inverse take without AVX-512, primus_ntt/src/ntt/hexl/mod.rs:530-538,
+ * 623-632) over the same tables (hexl/mod.rs:188-204 build them as UintNttTable
+ * does), with its own primitives: MultiplyFactor precons at shift 64
+ * (primus_factor/src/multiply.rs) and hexl::number_theory::multiply_mod_lazy.
+ *
+ * forward_transform_to_bit_reverse_radix2_inplace, hexl/radix2.rs:52-194:
+ * Harvey butterflies, one twiddle per group taken in table order from index 1;
+ * output_mod_factor 1 ends with reduce_twice. */
+void po_hexl_forward(const po_table *t, uint64_t *x, uint32_t output_mod_factor) {
+    const uint64_t q = t->q, q2 = q << 1;
+    const size_t n = t->n;
+    size_t wi = 1; /* w_iter.next() skips w[0] */
+    for (size_t gap = n >> 1, m = 1; m < n; gap >>= 1, m <<= 1) {
+        for (size_t base = 0; base < n; base += gap << 1) {
+            const uint64_t w = t->rp[wi];
+            const uint64_t wp = po_multiply_factor(w, 64, q);
+            ++wi;
+            for (size_t i = base; i < base + gap; ++i) {
+                /* fwd_butterfly_radix2, radix2.rs:40-50 */
+                const uint64_t tx = reduce_once(x[i], q2);
+                const uint64_t tt = po_multiply_mod_lazy(x[i + gap], w, wp, 64, q);
+                x[i] = tx + tt;
+                x[i + gap] = tx + q2 - tt;
+            }
+        }
+    }
+    if (output_mod_factor == 1)
+        for (size_t i = 0; i < n; ++i) {
+            uint64_t v = x[i]; /* reduce_twice, hexl/number_theory.rs:27-38 */
+            if (v >= q2) v -= q2;
+            if (v >= q) v -= q;
+            x[i] = v;
+        }
+}
+
+/* inverse_transform_from_bit_reverse_radix2_inplace, hexl/radix2.rs:220-382:
+ * Harvey inverse butterflies with inverse twiddles in table order from index
+ * 1, then the last stage with N^-1 and N^-1 * w folded in, both products lazy
+ * ([0, 2q)) and the sum reduced once first; output_mod_factor 1 ends with
+ * reduce_once. */
+void po_hexl_inverse(const po_table *t, uint64_t *x, uint32_t output_mod_factor) {
+    const uint64_t q = t->q, q2 = q << 1;
+    const size_t n = t->n;
+    size_t wi = 1;
+    for (size_t gap = 1, m = n >> 1; m > 1; gap <<= 1, m >>= 1) {
+        for (size_t base = 0; base < n; base += gap << 1) {
+            const uint64_t w = t->irp[wi];
+            const uint64_t wp = po_multiply_factor(w, 64, q);
+            ++wi;
+            for (size_t i = base; i < base + gap; ++i) {
+                /* inv_butterfly_radix2, radix2.rs:210-218 */
+                const uint64_t tx = x[i] + x[i + gap];
+                const uint64_t y = x[i] + q2 - x[i + gap];
+                x[i] = reduce_once(tx, q2);
+                x[i + gap] = po_multiply_mod_lazy(y, w, wp, 64, q);
+            }
+        }
+    }
+    const uint64_t w = t->irp[wi];
+    const uint64_t inv_n = t->inv_n, inv_n_p = po_multiply_factor(inv_n, 64, q);
+    /* MultiplyFactor::mul_modulo of w by N^-1: canonical */
+    const uint64_t inv_n_w = reduce_once(po_multiply_mod_lazy(w, inv_n, inv_n_p, 64, q), q);
+    const uint64_t inv_n_w_p = po_multiply_factor(inv_n_w, 64, q);
+    const size_t h = n >> 1;
+    for (size_t i = 0; i < h; ++i) {
+        const uint64_t tx = reduce_once(x[i] + x[i + h], q2);
+        const uint64_t ty = x[i] + q2 - x[i + h];
+        x[i] = po_multiply_mod_lazy(tx, inv_n, inv_n_p, 64, q);
+        x[i + h] = po_multiply_mod_lazy(ty, inv_n_w, inv_n_w_p, 64, q);
+    }
+    if (output_mod_factor == 1)
+        for (size_t i = 0; i < n; ++i) x[i] = reduce_once(x[i], q);
+}
+
 /* transform_monomial: out[k] = coeff * psi^(((2 rev(k) + 1) * degree) mod 2N).
  * primus_ntt/src/ntt/primitive.rs:298-347 */
 void po_transform_monomial(const po_table *t, uint64_t coeff, size_t degree, uint64_t *values) {

@@ -69,6 +69,10 @@ void po_lazy_transform(const po_table *t, uint64_t *poly);
 void po_transform(const po_table *t, uint64_t *poly);
 void po_lazy_inverse_transform(const po_table *t, uint64_t *values);
 void po_inverse_transform(const po_table *t, uint64_t *values);
+/* HexlNttTable's radix-2 path (hexl/radix2.rs): forward output_mod_factor 1
+ * or 4, inverse output_mod_factor 1 or 2. */
+void po_hexl_forward(const po_table *t, uint64_t *x, uint32_t output_mod_factor);
+void po_hexl_inverse(const po_table *t, uint64_t *x, uint32_t output_mod_factor);
 void po_transform_monomial(const po_table *t, uint64_t coeff, size_t degree, uint64_t *values);
 void po_transform_coeff_one_monomial(const po_table *t, size_t degree, uint64_t *values);
 void po_transform_coeff_minus_one_monomial(const po_table *t, size_t degree, uint64_t *values);

@@ -18,6 +18,7 @@ from .ntt import (  # noqa: F401
     automorphism_device,
     fwd_mul_inv_device,
     inverse_transform_device,
+    mul_factor_device,
     negacyclic_mul_device,
     negacyclic_mul_rows_device,
     pointwise_mul_add_device,

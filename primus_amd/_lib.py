@@ -150,6 +150,7 @@ def _declare(L):
         "ph_ext_product_rns_device": (i, [vp, vp, vp, sz, ctypes.c_int, vp]),
         "ph_hexl_compute_inverse": (i, [vp, vp, sz, ctypes.c_uint64, ctypes.c_uint64]),
         "ph_dcrt_pointwise_mul_device": (i, [vp, vp, vp, vp, sz, vp]),
+        "ph_dcrt_mul_factor_device": (i, [vp, vp, sz, p64, u32, i, vp]),
         "ph_dcrt_pointwise_mul_add_device": (i, [vp, vp, vp, vp, sz, vp]),
         "ph_dcrt_automorphism_device": (i, [vp, vp, vp, sz, u64, vp]),
         "ph_base_converter_new": (i, [p64, u32, p64, u32, pvp]),

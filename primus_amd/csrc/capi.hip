@@ -369,7 +369,7 @@ int staging(size_t bytes, void **buf, hipStream_t *stream) {
 // Autotuned launch shape per (log_n, op, arithmetic): log2 of the coefficients
 // each thread holds in single-pass kernels (0 = the default).  Written by
 // ph_tuning_set / ph_autotune / ph_tuning_load, read by every launch.
-std::atomic<int> g_tuning[kMaxLogN + 1][5][2];
+std::atomic<int> g_tuning[kMaxLogN + 1][kOpCount][2];
 
 // The arithmetic a table's launches may use (kArithF64 / kArithCanonInt bits).
 template <class Tab>
@@ -537,9 +537,11 @@ int ph_hexl_compute_inverse(const ph_ntt_table *t, uint64_t *operand, size_t len
                             uint64_t output_mod_factor) {
     if (input_mod_factor != 1 && input_mod_factor != 2) return fail(PH_ERR_INVALID, "input_mod_factor must be 1 or 2");
     if (output_mod_factor != 1 && output_mod_factor != 2) return fail(PH_ERR_INVALID, "output_mod_factor must be 1 or 2");
-    // the inverse takes [0,2q) and returns [0,q) either way
-    return output_mod_factor == 1 ? ph_inverse_transform_slice(t, operand, len)
-                                  : ph_lazy_inverse_transform_slice(t, operand, len);
+    // output_mod_factor 2: HEXL's radix-2 inverse, whose last stage leaves
+    // lazy products in [0, 2q) (hexl/radix2.rs:358-374); 1: canonical
+    if (output_mod_factor == 1) return ph_inverse_transform_slice(t, operand, len);
+    const int rc = check_slice(t, operand, len, t ? t->n : 0);
+    return rc != PH_OK ? rc : host_roundtrip(Op::InvHexlLazy, t->d_tab, 1, t->log_n, arith_of(t), operand, len);
 }
 
 int ph_transform_monomial(const ph_ntt_table *t, uint64_t coeff, size_t degree, uint64_t *values,
@@ -802,6 +804,32 @@ int ph_dcrt_negacyclic_mul_rows_device(const ph_dcrt_table *d, uint64_t *d_a, co
     if (!d || ((!d_a || !d_b || !d_limb_of) && count)) return fail(PH_ERR_INVALID, "null argument");
     return run_op(Op::PolyMul, d->d_tabs, d->L, d->log_n, arith_of(d), d_a, d_b, nullptr, count, (hipStream_t)stream,
                   d_limb_of);
+}
+
+int ph_dcrt_mul_factor_device(const ph_dcrt_table *d, uint64_t *d_polys, size_t count, const uint64_t *factors,
+                              uint32_t bit_shift, int lazy, void *stream) {
+    if (!d || !factors || (!d_polys && count)) return fail(PH_ERR_INVALID, "null argument");
+    if (bit_shift != 32 && bit_shift != 52 && bit_shift != 64)
+        return fail(PH_ERR_INVALID, "bit_shift must be 32, 52 or 64 (MultiplyFactor::new)");
+    if (d->L > kMulFactorMaxLimbs) return fail(PH_ERR_UNSUPPORTED, "at most 64 limbs");
+    MulFactorArgs a{};
+    a.data = d_polys;
+    a.total = (uint64_t)count << d->log_n;
+    a.L = d->L;
+    a.log_n = d->log_n;
+    a.lazy = lazy ? 1u : 0u;
+    for (uint32_t i = 0; i < d->L; ++i) {
+        const uint64_t q = d->tables[i]->q, c = factors[i];
+        // MultiplyFactor::new asserts operand <= modulus; the shift bounds q
+        // (multiply_mod_lazy's debug_assert, hexl/number_theory.rs:74-76)
+        if (c > q) return fail(PH_ERR_INVALID, "factor must not exceed its limb modulus");
+        if (bit_shift < 64 && q >= (1ull << bit_shift)) return fail(PH_ERR_INVALID, "modulus exceeds 2^bit_shift");
+        a.factor[i] = c;
+        a.barrett[i] = (uint64_t)((((u128)c) << bit_shift) / q);
+        a.q[i] = q;
+    }
+    const hipError_t e = launch_mul_factor(a, bit_shift, (hipStream_t)stream);
+    return e == hipSuccess ? PH_OK : hip_fail(e, "multiply-factor kernel");
 }
 
 int ph_dcrt_pointwise_mul_device(const ph_dcrt_table *d, const uint64_t *d_a, const uint64_t *d_b,
@@ -1217,7 +1245,10 @@ int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t deco
         g.q[i] = q;
         g.inv[i] = inv;
         g.inv_q[i] = h_shoup_quotient(inv, q);
-        for (uint32_t w = 0; w < kExtRnsWords; ++w) g.punct[i][w] = p[w];
+        for (uint32_t w = 0; w < kExtRnsWords; ++w) {
+            g.punct[i][w] = p[w];
+            if (p[w]) g.punct_words = std::max(g.punct_words, w + 1);
+        }
     }
     // BigUintApproxSignedBasis::new(Q, log_basis, Some(l)) (primus_decompose/src/big_integer/basis.rs:34-195)
     const uint32_t bits = words_bits(Q);

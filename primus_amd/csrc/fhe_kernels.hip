@@ -379,7 +379,10 @@ __device__ __forceinline__ uint64_t ext_rns_window(const ExtRnsArgs &g, const ui
         uint64_t carry = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const uint64_t lo = prod * g.punct[i][w], hi = mulhi64(prod, g.punct[i][w]);
+            // Q/q_i has at most punct_words nonzero words (uniform): the
+            // words above them only take the carry
+            const bool mul = (uint32_t)w < g.punct_words;
+            const uint64_t lo = mul ? prod * g.punct[i][w] : 0, hi = mul ? mulhi64(prod, g.punct[i][w]) : 0;
             const uint64_t s1 = v[w] + lo;
             uint64_t c = s1 < lo;
             const uint64_t s2 = s1 + carry;

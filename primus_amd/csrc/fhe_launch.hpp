@@ -52,6 +52,8 @@ struct ExtRnsArgs {
     uint64_t basis, basis_minus_one, carry_mask, half;
     uint64_t count;
     uint32_t L, l, log_basis, drop_bits, has_split, log_n;
+    uint32_t punct_words;  // the most nonzero words of any Q/q_i
+    uint32_t pad_;
 };
 
 // The multiply-accumulate of the transformed digits with the key.

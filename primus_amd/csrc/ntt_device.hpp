@@ -170,6 +170,19 @@ struct IntArith {
     }
 };
 
+// IntArith with HexlNttTable's last inverse stage (the radix-2 path,
+// primus_ntt/src/ntt/hexl/radix2.rs:358-374): the sum is reduced below 2q
+// before its N^-1 product, and both products stay lazy in [0, 2q).  Every
+// other butterfly of HEXL's radix-2 transforms is IntArith's (radix2.rs:40-50,
+// 210-218), on the same tables.
+struct HexlLazyArith : IntArith {
+    __device__ __forceinline__ static void gs_last(T &a, T &b, const DevTable &t) {
+        const uint64_t u = a, v = b;
+        a = shoup_lazy(t.inv_n, t.inv_n_q, sub_if_ge(u + v, t.q2), t.q);
+        b = shoup_lazy(t.sr, t.sr_q, u + t.q2 - v, t.q);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // IntCanonArith (q < 2^60, canonical outputs only)
 // ---------------------------------------------------------------------------

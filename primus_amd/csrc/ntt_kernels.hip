@@ -235,9 +235,10 @@ __global__ __launch_bounds__(1 << LOGT, (StreamOccupancy<LOGC, LOGT, MODE>::valu
     }
 }
 
-// Column pass of the two-pass schedule (IntArith): forward stages [0, LOGR) or
-// inverse stages [logN-LOGR, logN) on columns k = i*C + j, C = N >> LOGR.
-template <int LOGR, int MODE>
+// Column pass of the two-pass schedule (IntArith, or HexlLazyArith for HEXL's
+// lazy inverse): forward stages [0, LOGR) or inverse stages [logN-LOGR, logN)
+// on columns k = i*C + j, C = N >> LOGR.
+template <int LOGR, int MODE, class CA = IntArith>
 __global__ __launch_bounds__(256) void column_kernel(KArgs a) {
     const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     const uint32_t logc = a.tabs[0].log_n - LOGR;
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(256) void column_kernel(KArgs a) {
     if constexpr ((MODE & M_FWD) != 0) {
         ct_round<IntArith, LOGR>(x, 0, 0, IntArith::fwd(tab), tab);
     } else {
-        gs_round<IntArith, LOGR, true>(x, logc, 0, n, IntArith::inv(tab), tab);
+        gs_round<CA, LOGR, true>(x, logc, 0, n, CA::inv(tab), tab);
     }
 #pragma unroll
     for (int i = 0; i < (1 << LOGR); ++i) dst[(uint64_t)i << logc] = x[i];
@@ -329,6 +330,42 @@ __global__ __launch_bounds__(256) void automorphism_kernel(const uint64_t *in, u
     st_stream(out + g, in[(row << log_n) + auto_source(d, log_n, k)]);
 }
 
+// x <- c_i * x mod q_i by MultiplyFactor (primus_factor/src/multiply.rs:50-70)
+// at BIT_SHIFT 32, 52 or 64, for limb-major [count][N] data, limb = poly % L:
+// the quotient estimate is (x * factor) >> BIT_SHIFT, the result
+// c*x - estimate*q (mod 2^64) in [0, 2q) (lazy) or [0, q).  Shift 64 is
+// ShoupFactor's factor_mul_modulo (NttPolynomial::mul_factor_assign,
+// primus_poly/src/ntt/mul.rs:63-68); shifts 32 and 52 are HEXL's factor forms
+// (inputs and q below 2^BIT_SHIFT; at 32 the product is formed in 64 bits, as
+// the reference does).
+template <int BIT_SHIFT>
+__global__ __launch_bounds__(256) void mul_factor_kernel(MulFactorArgs a) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.total) return;
+    const uint32_t limb = (uint32_t)((i >> a.log_n) % a.L);
+    const uint64_t c = a.factor[limb], bf = a.barrett[limb], q = a.q[limb];
+    const uint64_t x = a.data[i];
+    uint64_t hw;
+    if constexpr (BIT_SHIFT == 32) hw = (bf * x) >> 32;
+    else if constexpr (BIT_SHIFT == 52) hw = (mulhi64(bf, x) << 12) | ((bf * x) >> 52);
+    else hw = mulhi64(bf, x);
+    uint64_t r = c * x - q * hw;
+    if (!a.lazy) r = sub_if_ge(r, q);
+    a.data[i] = r;
+}
+
+hipError_t launch_mul_factor(const MulFactorArgs &a, uint32_t bit_shift, hipStream_t s) {
+    if (a.total == 0) return hipSuccess;
+    const dim3 grid((uint32_t)((a.total + 255) / 256)), block(256);
+    switch (bit_shift) {
+        case 32: mul_factor_kernel<32><<<grid, block, 0, s>>>(a); break;
+        case 52: mul_factor_kernel<52><<<grid, block, 0, s>>>(a); break;
+        case 64: mul_factor_kernel<64><<<grid, block, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // out = a * b (mod q) or out = acc + a * b (mod q), limb-major [count][N],
 // limb = poly % L.  primus_poly/src/ntt/mul.rs:87-119, primus_poly/src/ntt/mod.rs:97-108.
 __global__ void pointwise_kernel(const DevTable *tabs, uint32_t L, uint32_t log_n, uint64_t total,
@@ -383,11 +420,11 @@ static hipError_t launch_stream(const KArgs &a0, uint64_t count, hipStream_t s) 
     return hipGetLastError();
 }
 
-template <int LOGR, int MODE>
+template <int LOGR, int MODE, class CA = IntArith>
 static hipError_t launch_column(const KArgs &a, uint32_t log_n, uint64_t count, hipStream_t s) {
     const uint64_t threads = count << (log_n - LOGR);
     if (threads == 0) return hipSuccess;
-    column_kernel<LOGR, MODE><<<dim3((uint32_t)(threads / 256u)), dim3(256), 0, s>>>(a);
+    column_kernel<LOGR, MODE, CA><<<dim3((uint32_t)(threads / 256u)), dim3(256), 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -410,7 +447,8 @@ static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, int arith, bool 
     if constexpr (LOGN >= 6) {
         if (arith & kArithF64) {
             switch (op) {
-                case Op::FwdLazy: break;  // lazy values are defined by the integer butterflies
+                case Op::FwdLazy:
+                case Op::InvHexlLazy: break;  // lazy values are defined by the integer butterflies
                 case Op::FwdCanon:
                     return launch_fwd_first<F64Arith, LOGN, LOGT, M_FWD | M_CANON>(a, count, stream, s);
                 case Op::Inv: return launch_chunk<F64Arith, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
@@ -425,7 +463,8 @@ static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, int arith, bool 
     if (arith & kArithCanonInt) {
         using C = IntCanonArith;
         switch (op) {
-            case Op::FwdLazy: break;  // lazy values are defined by IntArith's butterflies
+            case Op::FwdLazy:
+            case Op::InvHexlLazy: break;  // lazy values are defined by IntArith's butterflies
             case Op::FwdCanon: return launch_fwd_first<C, LOGN, LOGT, M_FWD | M_CANON>(a, count, stream, s);
             case Op::Inv: return launch_chunk<C, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
             case Op::FwdMulInv:
@@ -437,6 +476,7 @@ static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, int arith, bool 
         case Op::FwdLazy: return launch_fwd_first<IntArith, LOGN, LOGT, M_FWD>(a, count, stream, s);
         case Op::FwdCanon: return launch_fwd_first<IntArith, LOGN, LOGT, M_FWD | M_CANON>(a, count, stream, s);
         case Op::Inv: return launch_chunk<IntArith, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
+        case Op::InvHexlLazy: return launch_chunk<HexlLazyArith, LOGN, LOGT, M_INV | M_INV_LAST>(a, count, s);
         case Op::FwdMulInv:
             return launch_fwd_first<IntArith, LOGN, LOGT, M_FWD | M_MUL_SHOUP | M_INV | M_INV_LAST>(a, count,
                                                                                                   stream, s);
@@ -479,6 +519,11 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
             if (e != hipSuccess) return e;
             col.data2 = a.data;
             return launch_column<LOGR, M_INV>(col, log_n, count, s);
+        case Op::InvHexlLazy:
+            e = launch_chunk<A, LOGC, LOGT, M_INV>(a, count, s);
+            if (e != hipSuccess) return e;
+            col.data2 = a.data;
+            return launch_column<LOGR, M_INV, HexlLazyArith>(col, log_n, count, s);
         case Op::FwdMulInv:
             col.data2 = a.data;
             e = launch_column<LOGR, M_FWD>(col, log_n, count, s);

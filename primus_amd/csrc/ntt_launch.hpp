@@ -42,7 +42,9 @@ enum class Op : int {
     Inv,        // (lazy_)inverse_transform_slice: [0,2q) -> [0,q)
     FwdMulInv,  // forward, Shoup multiply by a resident operand, inverse
     PolyMul,    // negacyclic product: forward both, Barrett multiply, inverse
+    InvHexlLazy,  // HexlNttTable::compute_inverse(.., 2): [0,2q) -> [0,2q), HEXL's last stage
 };
+constexpr int kOpCount = 6;
 
 bool needs_scratch(Op op, uint32_t log_n);
 // Launch shape of the single-pass kernels (N <= 2^13): log2 of the
@@ -62,6 +64,15 @@ hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, 
 // NTT-domain automorphism x -> x^k (k odd, < 2N) of `rows` rows of 2^log_n words.
 hipError_t launch_automorphism(const uint64_t *in, uint64_t *out, uint32_t log_n, uint32_t k, uint64_t rows,
                                hipStream_t s);
+// Scalar multiply of limb-major data by one factor per limb (MultiplyFactor).
+constexpr uint32_t kMulFactorMaxLimbs = 64;
+struct MulFactorArgs {
+    uint64_t *data;
+    uint64_t total;  // words: count << log_n
+    uint64_t factor[kMulFactorMaxLimbs], barrett[kMulFactorMaxLimbs], q[kMulFactorMaxLimbs];
+    uint32_t L, log_n, lazy, pad_;
+};
+hipError_t launch_mul_factor(const MulFactorArgs &a, uint32_t bit_shift, hipStream_t s);
 hipError_t launch_pointwise(const DevTable *tabs, uint32_t L, uint32_t log_n, uint64_t total,
                             const uint64_t *a, const uint64_t *b, uint64_t *out, int mac,
                             hipStream_t s);

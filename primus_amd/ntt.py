@@ -342,6 +342,16 @@ def negacyclic_mul_rows_device(table, da, db, d_limb_of, count, stream=None):
     _check(lib().ph_dcrt_negacyclic_mul_rows_device(table.handle, da, db, d_limb_of, count, _stream(stream)))
 
 
+def mul_factor_device(table, dptr, count, factors, bit_shift=64, lazy=False, stream=None):
+    """x <- factors[i] * x mod q_i per limb-polynomial (MultiplyFactor at bit_shift
+    32, 52 or 64; lazy: [0, 2q))."""
+    f = np.ascontiguousarray(factors, dtype=np.uint64)
+    if f.size != table.moduli_count():
+        raise ValueError("one factor per limb")
+    _check(lib().ph_dcrt_mul_factor_device(table.handle, dptr, count, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                           bit_shift, int(bool(lazy)), _stream(stream)))
+
+
 def pointwise_mul_device(table, da, db, dout, count, stream=None):
     _check(lib().ph_dcrt_pointwise_mul_device(table.handle, da, db, dout, count, _stream(stream)))
 

@@ -484,3 +484,43 @@ def test_automorphism_rejects():
         pa.automorphism_device(tab, d.ptr, pa.DeviceBuffer(n).ptr, 1, 4)  # even degree
     with pytest.raises(ValueError):
         pa.automorphism_device(tab, d.ptr, ctypes.c_void_p(d.ptr.value + 8 * n), 2, 3)  # overlapping rows
+
+
+@pytest.mark.parametrize("bit_shift,moduli", [
+    (64, [576460752303415297, 1125899906826241]),   # ShoupFactor (mul_factor_assign)
+    (52, [562949953392641, 1125899906826241]),      # HEXL's IFMA factor form
+    (32, [536813569, 2147473409]),                  # HEXL's 32-bit factor form
+])
+def test_mul_factor_device(bit_shift, moduli):
+    """MultiplyFactor's lazy_mul_modulo / mul_modulo (primus_factor/src/multiply.rs:50-70)
+    on the device, per limb, against the oracle's restatement (pinned by exact
+    remainders in test_oracle.py)."""
+    log_n, count = 10, 6  # every modulus here is 1 mod 2048
+    n = 1 << log_n
+    L = len(moduli)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    rng = np.random.default_rng(bit_shift)
+    bound = [min(4 * q, 1 << bit_shift) if bit_shift < 64 else 4 * q for q in moduli]
+    x = np.stack([rng.integers(0, bound[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    x[0, :3] = [0, moduli[0] - 1, bound[0] - 1]
+    factors = [int(rng.integers(0, q)) for q in moduli]
+    factors[0] = moduli[0] - 1
+    lib = O.lib()
+    for lazy in (True, False):
+        d = pa.DeviceBuffer.from_array(x)
+        pa.mul_factor_device(tab, d.ptr, count, factors, bit_shift=bit_shift, lazy=lazy)
+        pa.synchronize()
+        got = d.download((count, n))
+        for i in range(count):
+            q, c = moduli[i % L], factors[i % L]
+            bf = lib.po_multiply_factor(c, bit_shift, q)
+            want = np.array([lib.po_multiply_mod_lazy(int(v), c, bf, bit_shift, q) for v in x[i]], dtype=np.uint64)
+            if not lazy:
+                want = np.where(want >= q, want - np.uint64(q), want)
+            np.testing.assert_array_equal(got[i], want, err_msg="row %d lazy %s" % (i, lazy))
+            assert int(got[i].max()) < (2 if lazy else 1) * q
+    with pytest.raises(ValueError):
+        pa.mul_factor_device(tab, d.ptr, count, factors, bit_shift=48)
+    with pytest.raises(ValueError):
+        pa.mul_factor_device(tab, d.ptr, count, [moduli[0] + 1] + factors[1:], bit_shift=bit_shift)
+
