@@ -288,6 +288,11 @@ int build_table(uint32_t log_n, uint64_t q, uint32_t flags, ph_ntt_table **out) 
     d.sr = (uint64_t)(((u128)last.w * t->inv_n) % q);
     d.sr_q = h_shoup_quotient(d.sr, q);
     h_barrett_ratio(q, &d.br0, &d.br1);
+    if (q < (1ull << 59) && q > 2) {
+        const uint32_t b = 64 - __builtin_clzll(q);
+        d.bar_sh = b - 1;
+        d.bar_mu = (uint64_t)((((u128)1) << (b - 1 + 64)) / q);
+    }
     d.fwd = t->d_fwd;
     d.inv = t->d_inv;
     d.ord = t->d_ord;

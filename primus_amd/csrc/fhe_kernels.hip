@@ -12,6 +12,8 @@
 //   b = NTT(e + Delta*m) + a (.) NTT(s) over mixed ring degrees and limb
 //   counts in one launch.  Follows NttRlweSecretKey::encrypt_inplace_with_embedding
 //   (primus_fhe_core/src/secret_key/rlwe.rs:227-268).
+#include <type_traits>
+
 #include "fhe_launch.hpp"
 #include "ntt_launch.hpp"
 #include "ntt_rounds.hpp"
@@ -600,8 +602,31 @@ __device__ __forceinline__ uint64_t lift_small(int v, uint64_t q) { return v >= 
 
 // e + Delta*m (mod q) from the small values e and m: sample_gaussian_values_inplace
 // then add_mul_factor_assign (primus_fhe_core/src/secret_key/rlwe.rs:245-260).
+template <class A>
 __device__ __forceinline__ uint64_t rlwe_x(const RlweItem &it, int8_t e, uint8_t m, const DevTable &t) {
-    return add_mod(lift_small(e, t.q), shoup_full(it.delta, it.delta_q, m, t.q), t.q);
+    if constexpr (std::is_same<A, IntCanonArith>::value)
+        // any representative below 16q is a forward input here: Delta*m in
+        // [0, 4q) plus e lifted below q
+        return lift_small(e, t.q) + shoup_approx(it.delta, it.delta_q, m, t.q);
+    else
+        return add_mod(lift_small(e, t.q), shoup_full(it.delta, it.delta_q, m, t.q), t.q);
+}
+
+// (a*s + x) mod q for a < q and s, x below 16q, q < 2^59 (b bits, b <= 59):
+// Barrett with the one-word ratio mu = floor(2^(b-1+64) / q) on
+// t = X >> (b-1) < 2^64 (X < 16 q^2 + 16q < 2^(2b+4)): the quotient estimate
+// mulhi(t, mu) is at most two short, so two conditional subtractions make the
+// remainder canonical.  The two-limb reduction (barrett_mul_add) needs twelve
+// partial products where this needs six.
+__device__ __forceinline__ uint64_t mac59(uint64_t a, uint64_t s, uint64_t x, const DevTable &t) {
+    uint64_t lo = a * s, hi = mulhi64(a, s);
+    lo += x;
+    hi += lo < x;
+    const uint32_t sh = t.bar_sh;  // b - 1, 1 <= sh <= 58
+    const uint64_t tq = (lo >> sh) | (hi << (64 - sh));
+    const uint64_t k = mulhi64(tq, t.bar_mu);
+    const uint64_t r = lo - k * t.q;
+    return sub_if_ge(sub_if_ge(r, t.q2), t.q);
 }
 
 // b_hat = x_hat + a_hat (.) s_hat over one chunk of C = 2^LOGC slots starting
@@ -638,7 +663,9 @@ __device__ __forceinline__ void rlwe_mac_rows(const RlweArgs &a, const RlweItem 
 #pragma unroll
         for (int i = i0; i < i0 + 4 && i < G::E; ++i) {
             const uint32_t k = tid + ((uint32_t)i << LOGT);
-            st_stream(a.out + it.row + base + k, barrett_mul_add(av[i], lds[lds_pos(k)], xv[i], t.q, t.br0, t.br1));
+            const uint64_t sv = lds[lds_pos(k)];
+        st_stream(a.out + it.row + base + k,
+                  t.bar_mu ? mac59(av[i], sv, xv[i], t) : barrett_mul_add(av[i], sv, xv[i], t.q, t.br0, t.br1));
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -671,7 +698,7 @@ __device__ __forceinline__ void rlwe_single(const RlweArgs &a, const RlweItem &i
 #pragma unroll
     for (int i = 0; i < G::E; ++i) {
         sx[i] = lift_small(sv[i], t.q);
-        x[i] = rlwe_x(it, ev[i], mv[i], t);
+        x[i] = rlwe_x<A>(it, ev[i], mv[i], t);
     }
     FwdRounds<A, LOGN, LOGT, 0, true>::run(sx, lds, tid, 0, 0, t);
     FwdRounds<A, LOGN, LOGT, 0, true>::run(x, lds, tid, 0, 0, t);
@@ -720,7 +747,7 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
 #pragma unroll
         for (int i = 0; i < (1 << R); ++i) {
             v[i] = lift_small(sv[i], t.q);
-            w[i] = rlwe_x(it, ev[i], mv[i], t);
+            w[i] = rlwe_x<A>(it, ev[i], mv[i], t);
         }
         ternary_stage0<R>(v, sv, t);
         if constexpr (R > 1) {

@@ -93,6 +93,8 @@ struct DevTable {
     uint64_t inv_n, inv_n_q;  // N^-1 (Shoup)
     uint64_t sr, sr_q;        // N^-1 * inv_root_powers[N-1] (Shoup)
     uint64_t br0, br1;        // Barrett ratio floor(2^128/q)
+    uint64_t bar_mu;          // q < 2^59 of b bits: floor(2^(b-1+64)/q) (one-word Barrett), else 0
+    uint32_t bar_sh, pad0_;   // b - 1
     const Shoup *fwd;         // root_powers, N entries (bit-reversed order)
     const Shoup *inv;         // inv_root_powers, N entries (scrambled order)
     const Shoup *ord;         // ordinal_root_powers, 2N entries
