@@ -1532,8 +1532,9 @@ uint64_t ph_rlwe_ctx_modulus(const ph_rlwe_ctx *c, uint32_t log_n, uint32_t limb
     return c->tables[(log_n - c->min_log_n) * c->max_limbs + limb]->q;
 }
 
-// One work item per (request, limb), largest ring degrees first (the
-// workgroups' strided walk then balances), with Delta mod q_i.
+// One work item per (request, limb), largest ring degrees first (the order
+// the kernel's work queue hands them out), with Delta mod q_i, and one zeroed
+// element after them: the launch's work queue (RlweArgs::queue).
 static int rlwe_items(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t n_reqs,
                       std::vector<RlweItem> &items) {
     items.clear();
@@ -1563,6 +1564,7 @@ static int rlwe_items(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_
     }
     std::stable_sort(items.begin(), items.end(),
                      [](const RlweItem &x, const RlweItem &y) { return x.log_n > y.log_n; });
+    items.push_back(RlweItem{});
     return PH_OK;
 }
 
@@ -1576,12 +1578,13 @@ static uint32_t rlwe_grid(const ph_rlwe_ctx *c, uint64_t n_items) {
 // one polynomial of the largest ring degree.
 static uint64_t rlwe_scratch_words(uint32_t max_log_n) { return max_log_n > 12 ? 1ull << max_log_n : 0; }
 
-static hipError_t rlwe_launch(const ph_rlwe_ctx *c, const RlweItem *d_items, uint64_t n_items, uint32_t max_log_n,
+static hipError_t rlwe_launch(const ph_rlwe_ctx *c, RlweItem *d_items, uint64_t n_items, uint32_t max_log_n,
                               const uint64_t *d_a, const int8_t *d_s, const int8_t *d_e, const uint8_t *d_m,
                               uint64_t *d_out, uint64_t *scratch, hipStream_t s) {
     RlweArgs a{};
     a.items = d_items;
     a.n_items = (uint32_t)n_items;
+    a.queue = (uint32_t *)(d_items + n_items);
     a.tabs = c->d_tabs;
     a.a = d_a;
     a.out = d_out;
@@ -1605,14 +1608,15 @@ int ph_rlwe_encrypt_device(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, ui
     if (rc != PH_OK) return rc;
     RlweItem *d_items = nullptr;
     uint64_t *scratch = nullptr;
+    const uint64_t n_items = items.size() - 1;  // and the queue
     PH_HIP(hipMallocAsync((void **)&d_items, items.size() * sizeof(RlweItem), s));
     PH_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(RlweItem), hipMemcpyHostToDevice, s));
     const uint32_t max_log_n = items.front().log_n;
     const uint64_t scratch_words = rlwe_scratch_words(max_log_n);
     if (scratch_words)
-        PH_HIP(hipMallocAsync((void **)&scratch, (size_t)rlwe_grid(c, items.size()) * scratch_words * 8, s));
+        PH_HIP(hipMallocAsync((void **)&scratch, (size_t)rlwe_grid(c, n_items) * scratch_words * 8, s));
     PH_HIP(hipStreamSynchronize(s));  // the host item list may go away after this
-    hipError_t e = rlwe_launch(c, d_items, items.size(), max_log_n, d_a, d_s, d_e, d_m, d_out, scratch, s);
+    hipError_t e = rlwe_launch(c, d_items, n_items, max_log_n, d_a, d_s, d_e, d_m, d_out, scratch, s);
     if (scratch) (void)hipFreeAsync(scratch, s);
     (void)hipFreeAsync(d_items, s);
     return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
@@ -1723,12 +1727,12 @@ int ph_rlwe_plan_new(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint32_t
     ph_rlwe_plan *p = new (std::nothrow) ph_rlwe_plan();
     if (!p) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
     p->ctx = c;
-    p->n_items = (uint32_t)items.size();
+    p->n_items = (uint32_t)items.size() - 1;  // the list's last element is the queue
     // room for up to four workgroups per compute unit (the autotuner's range);
     // two, the kernel's occupancy, by default
     const uint64_t cus = (uint64_t)std::max(c->cus, 1);
-    p->max_grid = (uint32_t)std::min<uint64_t>(items.size(), 4 * cus);
-    p->grid = (uint32_t)std::min<uint64_t>(items.size(), 2 * cus);
+    p->max_grid = (uint32_t)std::min<uint64_t>(p->n_items, 4 * cus);
+    p->grid = (uint32_t)std::min<uint64_t>(p->n_items, 2 * cus);
     p->scratch_words = items.front().log_n > 12 ? 1ull << items.front().log_n : 0;
     rc = upload((void **)&p->d_items, items.data(), items.size() * sizeof(RlweItem));
     if (rc == PH_OK && p->scratch_words && hipMalloc((void **)&p->scratch, (size_t)p->max_grid * p->scratch_words * 8) !=
@@ -1766,6 +1770,7 @@ int ph_rlwe_encrypt_plan_device(const ph_rlwe_plan *p, const uint64_t *d_a, cons
     RlweArgs a{};
     a.items = p->d_items;
     a.n_items = p->n_items;
+    a.queue = (uint32_t *)(p->d_items + p->n_items);
     a.tabs = p->ctx->d_tabs;
     a.a = d_a;
     a.out = d_out;
@@ -1992,7 +1997,7 @@ RlweStageSize rlwe_stage_size(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs,
         n_items += q.limbs;
         max_log_n = std::max(max_log_n, q.log_n);
     }
-    z.item_bytes = n_items * sizeof(RlweItem);
+    z.item_bytes = (n_items + 1) * sizeof(RlweItem);  // and the work queue
     z.scratch_bytes = rlwe_grid(c, n_items) * rlwe_scratch_words(max_log_n) * 8;
     return z;
 }
@@ -2012,7 +2017,7 @@ int rlwe_encrypt_staged(const ph_rlwe_ctx *c, const ph_rlwe_request *reqs, uint3
         return fail(PH_ERR_INVALID, "staging buffers smaller than the request list needs");
     std::memcpy(host_items, items.data(), need.item_bytes);
     PH_HIP(hipMemcpyAsync(d_items, host_items, need.item_bytes, hipMemcpyHostToDevice, s));
-    hipError_t e = rlwe_launch(c, (const RlweItem *)d_items, items.size(), items.front().log_n, d_a, d_s, d_e, d_m,
+    hipError_t e = rlwe_launch(c, (RlweItem *)d_items, items.size() - 1, items.front().log_n, d_a, d_s, d_e, d_m,
                                d_out, need.scratch_bytes ? (uint64_t *)d_scratch : nullptr, s);
     return e == hipSuccess ? PH_OK : hip_fail(e, "RLWE encryption kernel");
 }

@@ -778,8 +778,16 @@ __device__ __forceinline__ void rlwe_two_pass(const RlweArgs &a, const RlweItem 
 template <class A>
 __global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs a) {
     __shared__ uint64_t lds[Geo<12, 8>::LDS];
+    __shared__ uint32_t next;
     uint64_t *scr = a.scratch + (uint64_t)blockIdx.x * a.scratch_words;
-    for (uint32_t i = blockIdx.x; i < a.n_items; i += gridDim.x) {
+    // Items are taken from the queue, largest ring degrees first, by whichever
+    // workgroup is free: the batch's uneven items balance as they finish
+    // rather than by a fixed stride.
+    for (;;) {
+        if (threadIdx.x == 0) next = __hip_atomic_fetch_add(a.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t i = next;
+        if (i >= a.n_items) break;
         const RlweItem it = a.items[i];
         const DevTable t = a.tabs[it.tab];
         // Slot addresses depend on the thread alone; hidden from loop-invariant
@@ -797,7 +805,17 @@ __global__ __launch_bounds__(kRlweThreads, 2) void rlwe_encrypt_kernel(RlweArgs 
             case 16: rlwe_two_pass<A, 4>(a, it, t, lds, tid, scr); break;
             default: break;
         }
-        __syncthreads();  // LDS and scratch are reused by the next item
+        __syncthreads();  // LDS, scratch and `next` are reused by the next item
+    }
+    // Every workgroup counts itself out after its last take from the queue
+    // (release); the last one (acquire: all takes are done) zeroes the queue
+    // for the next launch over the same list.
+    if (threadIdx.x == 0) {
+        const uint32_t done = __hip_atomic_fetch_add(a.queue + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (done + 1 == gridDim.x) {
+            __hip_atomic_store(a.queue, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(a.queue + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 

@@ -105,7 +105,7 @@ struct RlweItem {
 
 struct RlweArgs {
     const RlweItem *items;  // sorted by decreasing ring degree
-    uint32_t n_items;
+    uint32_t n_items;       // work items; the list holds one more, the work queue
     uint32_t pad_;
     const DevTable *tabs;
     const uint64_t *a;      // a_hat rows (NTT form)
@@ -117,6 +117,10 @@ struct RlweArgs {
     uint64_t scratch_words; // the largest ring degree of the launch
     uint32_t canon_int;     // every modulus below 2^60: IntCanonArith kernel, else IntArith
     uint32_t pad2_;
+    // Work queue: {next item, workgroups done}, the item list's trailing
+    // element (zero when uploaded; the launch's last workgroup zeroes it again,
+    // so launches of one list run one after the other, never at once).
+    uint32_t *queue;
 };
 
 hipError_t launch_rlwe_encrypt(const RlweArgs &a, uint32_t grid, hipStream_t s);
