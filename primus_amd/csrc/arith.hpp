@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "host_tables.hpp"  // u128, Shoup, ShoupF and the host arithmetic
+
 namespace ph {
 
 // ---------------------------------------------------------------------------
@@ -109,35 +111,6 @@ __device__ __forceinline__ uint64_t barrett_mul_add(uint64_t a, uint64_t b, uint
 // (a + b) mod q for a, b < q.
 __device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) {
     return sub_if_ge(a + b, q);
-}
-
-// ---------------------------------------------------------------------------
-// host (table construction only)
-// ---------------------------------------------------------------------------
-
-typedef unsigned __int128 u128;
-
-inline uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)(((u128)a * b) % q); }
-
-inline uint64_t h_powmod(uint64_t b, uint64_t e, uint64_t q) {
-    uint64_t r = 1 % q;
-    b %= q;
-    while (e) {
-        if (e & 1) r = h_mulmod(r, b, q);
-        b = h_mulmod(b, b, q);
-        e >>= 1;
-    }
-    return r;
-}
-
-inline uint64_t h_shoup_quotient(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
-
-inline void h_barrett_ratio(uint64_t q, uint64_t *r0, uint64_t *r1) {
-    u128 all = ~(u128)0;
-    u128 r = all / q;
-    if (all % q == q - 1) r += 1;
-    *r0 = (uint64_t)r;
-    *r1 = (uint64_t)(r >> 64);
 }
 
 }  // namespace ph

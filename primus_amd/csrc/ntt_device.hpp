@@ -41,16 +41,6 @@
 
 namespace ph {
 
-struct alignas(16) Shoup {
-    uint64_t w;   // value
-    uint64_t wq;  // floor(w * 2^64 / q)
-};
-
-struct alignas(16) ShoupF {
-    double w;   // value (exact integer)
-    double wp;  // w / q rounded to binary64
-};
-
 // Pointer into the constant address space (read-only global memory).  Table
 // pointers read out of a DevTable are generic otherwise, and generic loads are
 // FLAT instructions that count against both memory counters (every LDS wait

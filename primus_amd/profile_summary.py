@@ -2,8 +2,10 @@
 
     python -m primus_amd.profile_summary stats RESULTS_DB
         per-kernel dispatch count, grid, average / min / max duration
-    python -m primus_amd.profile_summary sq RESULTS_DB KERNEL_SUBSTRING
-        every counter of the run averaged per dispatch of the kernel
+    python -m primus_amd.profile_summary sq RESULTS_DB KERNEL_SUBSTRING [JSON_FILE]
+        every counter of the run averaged per dispatch of the kernel; with
+        SQ_INSTS_VALU among them, the kernel's VALU issue floor, merged into
+        JSON_FILE's entry for the kernel when given
     python -m primus_amd.profile_summary pmc FETCH_DB WRITE_DB KERNEL_SUBSTRING COPY_BYTES [JSON_FILE]
         FETCH_SIZE and WRITE_SIZE per dispatch of the kernel, calibrated by the
         byte counts the same counters give for the largest device-to-device
@@ -74,20 +76,39 @@ def pmc(fetch_db, write_db, kernel, copy_bytes, json_file=None):
              "write_scale": write_scale, "traffic_bytes": traffic, "dispatches": len(kf)}
     print(json.dumps(entry))
     if json_file:
-        try:
-            with open(json_file) as fh:
-                table = json.load(fh)
-        except (OSError, ValueError):
-            table = {}
-        if "kernel" in table:  # a single-entry file: key it by its kernel
-            table = {table["kernel"]: table}
-        table[kernel] = entry
-        with open(json_file, "w") as fh:
-            json.dump(table, fh, indent=1)
-            fh.write("\n")
+        _merge(json_file, kernel, entry)
 
 
-def sq(db, kernel):
+def _merge(json_file, kernel, fields):
+    """Updates JSON_FILE's entry for the kernel with fields."""
+    try:
+        with open(json_file) as fh:
+            table = json.load(fh)
+    except (OSError, ValueError):
+        table = {}
+    if "kernel" in table:  # a single-entry file: key it by its kernel
+        table = {table["kernel"]: table}
+    table.setdefault(kernel, {"kernel": kernel}).update(fields)
+    with open(json_file, "w") as fh:
+        json.dump(table, fh, indent=1)
+        fh.write("\n")
+
+
+# MI355X: 256 compute units x 4 SIMDs; a wave64 VALU instruction occupies its
+# 16-lane SIMD for at least 4 cycles, at the 2.4 GHz peak engine clock.
+SIMDS = 1024
+VALU_CYCLES = 4
+CLOCK_HZ = 2.4e9
+
+
+def valu_floor_us(insts):
+    """Shortest time in which the chip issues `insts` wave-level VALU
+    instructions: every SIMD busy, one instruction per 4 cycles.  Multi-pass
+    instructions (64-bit multiplies, transcendentals) only lengthen it."""
+    return insts * VALU_CYCLES / (SIMDS * CLOCK_HZ) * 1e6
+
+
+def sq(db, kernel, json_file=None):
     """Per-dispatch averages of every counter collected for the kernel."""
     rows = _rows(db, "select counter_name, dispatch_id, sum(value) from counters_collection "
                      "where kernel_name like ? group by counter_name, dispatch_id", ("%" + kernel + "%",))
@@ -99,11 +120,20 @@ def sq(db, kernel):
     for name in sorted(per):
         vals = per[name]
         print("| %s | %d | %.0f |" % (name, len(vals), sum(vals) / len(vals)))
+    valu = per.get("SQ_INSTS_VALU")
+    if valu:
+        insts = sum(valu) / len(valu)
+        floor = valu_floor_us(insts)
+        print()
+        print("VALU issue floor: %.0f wave instructions x %d cycles / (%d SIMDs x %.1f GHz) = %.2f us"
+              % (insts, VALU_CYCLES, SIMDS, CLOCK_HZ / 1e9, floor))
+        if json_file:
+            _merge(json_file, kernel, {"valu_insts": insts, "valu_floor_us": floor, "valu_dispatches": len(valu)})
 
 
 if __name__ == "__main__":
     if len(sys.argv) >= 4 and sys.argv[1] == "sq":
-        sq(sys.argv[2], sys.argv[3])
+        sq(*sys.argv[2:5])
     elif len(sys.argv) >= 3 and sys.argv[1] == "stats":
         stats(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 0)
     elif len(sys.argv) >= 6 and sys.argv[1] == "pmc":
