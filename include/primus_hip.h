@@ -67,6 +67,19 @@ typedef struct ph_operand ph_operand;       /* device-resident Shoup-prepared NT
 const char *ph_status_string(int status);
 /* Text of the last failure on the calling thread ("" if none). */
 const char *ph_last_error(void);
+/* Debug builds only (no reference counterpart; the reference debug_assert!s
+ * its value ranges, primus_ntt/src/ntt/hexl/mod.rs:452-465, 546-558, and
+ * primus_reduce/src/slice_ops.rs:12-17).  The library variant built with
+ * PH_RANGE_CHECK=1 (libprimus_hip_rc.so) checks every value at the transform
+ * kernels' register-round boundaries against its arithmetic's range -- the
+ * integer butterflies' [0, 4q) forward, [0, 2q) inverse and [0, q) canonical
+ * outputs, and the wider ranges of the canonical-only arithmetics -- and counts
+ * violations without changing any result.  report = {violations, first site
+ * (1 forward input, 2 forward round, 3 inverse input, 4 inverse round, 5
+ * inverse output), the first offending value's bits, its modulus}; clear != 0
+ * zeroes the counters.  Synchronise the streams that ran the kernels first.
+ * Default builds return PH_ERR_UNSUPPORTED. */
+int ph_debug_range_report(uint64_t report[4], int clear);
 
 /* ---- NttTable ---- */
 

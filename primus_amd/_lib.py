@@ -9,6 +9,9 @@ import subprocess
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libprimus_hip.so")
+# debug variant: every transform kernel checks its values' ranges at round
+# boundaries (PH_RANGE_CHECK=1, read back with ph_debug_range_report)
+RC_LIB_PATH = os.path.join(PKG_DIR, "libprimus_hip_rc.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "primus_hip.h")
 CSRC = os.path.join(PKG_DIR, "csrc")
 SOURCES = ["capi.hip", "ntt_kernels.hip", "fhe_kernels.hip", "rns_kernels.hip", "copy_kernels.hip", "serving.cpp"]
@@ -28,37 +31,43 @@ HIPCC_FLAGS = [
 
 
 def build(verbose=False):
-    """Compile every HIP source into primus_amd/libprimus_hip.so for gfx950.
+    """Compile every HIP source into primus_amd/libprimus_hip.so for gfx950,
+    and the same sources with PH_RANGE_CHECK=1 into libprimus_hip_rc.so.
 
-    Each source is compiled to an object in parallel (one hipcc per source),
-    then the objects are linked; every call recompiles everything."""
+    Each (variant, source) is compiled to an object in parallel (one hipcc
+    each), then each variant's objects are linked; every call recompiles
+    everything."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    objdir = os.path.join(PKG_DIR, "_build")
-    os.makedirs(objdir, exist_ok=True)
     compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    variants = [(LIB_PATH, os.path.join(PKG_DIR, "_build"), []),
+                (RC_LIB_PATH, os.path.join(PKG_DIR, "_build", "rc"), ["-DPH_RANGE_CHECK=1"])]
     procs = []
-    for src in srcs:
-        obj = os.path.join(objdir, os.path.basename(src) + ".o")
-        cmd = [hipcc] + compile_flags + ["-pthread", "-c", "-o", obj, src]
-        procs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
-    objs, errors = [], []
-    for obj, p in procs:
+    for lib_path, objdir, defs in variants:
+        os.makedirs(objdir, exist_ok=True)
+        for src in srcs:
+            obj = os.path.join(objdir, os.path.basename(src) + ".o")
+            cmd = [hipcc] + compile_flags + defs + ["-pthread", "-c", "-o", obj, src]
+            procs.append((lib_path, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                                          text=True)))
+    objs, errors = {}, []
+    for lib_path, obj, p in procs:
         out, _ = p.communicate()
         if p.returncode != 0:
             errors.append(out)
         elif verbose:
             print(out)
-        objs.append(obj)
+        objs.setdefault(lib_path, []).append(obj)
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
-    tmp = LIB_PATH + ".tmp"
-    # only the ph_* entry points of the header are exported (exports.map)
-    link = ["-pthread", "-Wl,--version-script=" + os.path.join(CSRC, "exports.map"), "-o", tmp]
-    res = subprocess.run([hipcc] + HIPCC_FLAGS + link + objs, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("hipcc link failed:\n" + res.stdout + res.stderr)
-    os.replace(tmp, LIB_PATH)
+    for lib_path, _, _ in variants:
+        tmp = lib_path + ".tmp"
+        # only the ph_* entry points of the header are exported (exports.map)
+        link = ["-pthread", "-Wl,--version-script=" + os.path.join(CSRC, "exports.map"), "-o", tmp]
+        res = subprocess.run([hipcc] + HIPCC_FLAGS + link + objs[lib_path], capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("hipcc link failed:\n" + res.stdout + res.stderr)
+        os.replace(tmp, lib_path)
     return LIB_PATH
 
 
@@ -91,6 +100,22 @@ def lib():
     return _lib
 
 
+_rc_lib = None
+
+
+def lib_rc():
+    """The range-checking debug variant, libprimus_hip_rc.so (same entry points)."""
+    global _rc_lib
+    if _rc_lib is None:
+        if not os.path.exists(RC_LIB_PATH):
+            raise RuntimeError("libprimus_hip_rc.so is not built (%s); run __graft_entry__.build() first"
+                               % RC_LIB_PATH)
+        L = ctypes.CDLL(RC_LIB_PATH)
+        _declare(L)
+        _rc_lib = L
+    return _rc_lib
+
+
 def check(code):
     if code != 0:
         msg = lib().ph_last_error().decode(errors="replace")
@@ -104,6 +129,7 @@ def _declare(L):
     sigs = {
         "ph_status_string": (ctypes.c_char_p, [i]),
         "ph_last_error": (ctypes.c_char_p, []),
+        "ph_debug_range_report": (i, [p64, i]),
         "ph_ntt_table_new": (i, [u32, u64, pvp]),
         "ph_ntt_table_new_ex": (i, [u32, u64, u32, pvp]),
         "ph_ntt_table_arith": (i, [vp]),

@@ -323,6 +323,24 @@ const char *ph_status_string(int status) {
 
 const char *ph_last_error(void) { return g_last_error.c_str(); }
 
+int ph_debug_range_report(uint64_t report[4], int clear) {
+    if (!report) return fail(PH_ERR_INVALID, "null report");
+#if PH_RANGE_CHECK
+    uint64_t a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    hipError_t e = range_report_ntt(a, clear != 0);
+    if (e == hipSuccess) e = range_report_fhe(b, clear != 0);
+    if (e != hipSuccess) return hip_fail(e, "range-check report");
+    const uint64_t *first = a[0] ? a : b;
+    report[0] = a[0] + b[0];
+    for (int i = 1; i < 4; ++i) report[i] = first[i];
+    return PH_OK;
+#else
+    (void)clear;
+    for (int i = 0; i < 4; ++i) report[i] = 0;
+    return fail(PH_ERR_UNSUPPORTED, "library built without PH_RANGE_CHECK");
+#endif
+}
+
 int ph_ntt_table_new(uint32_t log_n, uint64_t modulus, ph_ntt_table **out) {
     return ph_ntt_table_new_ex(log_n, modulus, 0, out);
 }

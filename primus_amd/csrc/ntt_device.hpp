@@ -160,6 +160,12 @@ struct IntArith {
     __device__ __forceinline__ static T mul_pair(T x, T y, const DevTable &t) {
         return barrett_mul(canon_fwd(x, t), canon_fwd(y, t), t.q, t.br0, t.br1);
     }
+    // Value ranges at round boundaries, checked in builds with PH_RANGE_CHECK
+    // (the reference's debug_assert'ed input ranges, hexl/mod.rs:452-465 and
+    // 546-558): forward below 4q, inverse below 2q, inverse outputs canonical.
+    __device__ __forceinline__ static bool in_fwd_range(T v, const DevTable &t) { return v < t.q4; }
+    __device__ __forceinline__ static bool in_inv_range(T v, const DevTable &t) { return v < t.q2; }
+    __device__ __forceinline__ static bool in_out_range(T v, const DevTable &t) { return v < t.q; }
 };
 
 // IntArith with HexlNttTable's last inverse stage (the radix-2 path,
@@ -173,6 +179,8 @@ struct HexlLazyArith : IntArith {
         a = shoup_lazy(t.inv_n, t.inv_n_q, sub_if_ge(u + v, t.q2), t.q);
         b = shoup_lazy(t.sr, t.sr_q, u + t.q2 - v, t.q);
     }
+    // outputs in [0, 2q) (output_mod_factor 2)
+    __device__ __forceinline__ static bool in_out_range(T v, const DevTable &t) { return v < t.q2; }
 };
 
 // ---------------------------------------------------------------------------
@@ -238,6 +246,10 @@ struct IntCanonArith {
     __device__ __forceinline__ static T mul_pair(T x, T y, const DevTable &t) {
         return barrett_mul(canon_fwd(x, t), canon_fwd(y, t), t.q, t.br0, t.br1);
     }
+    // value ranges (as IntArith's): forward below 16q, inverse below 4q
+    __device__ __forceinline__ static bool in_fwd_range(T v, const DevTable &t) { return v < t.q8 + t.q8; }
+    __device__ __forceinline__ static bool in_inv_range(T v, const DevTable &t) { return v < t.q4; }
+    __device__ __forceinline__ static bool in_out_range(T v, const DevTable &t) { return v < t.q; }
 };
 
 // ---------------------------------------------------------------------------
@@ -358,6 +370,12 @@ struct F64Arith {
         const double yr = reduce(y, t);
         return mulmod(reduce(x, t), yr, yr * t.qinv, t.qd);
     }
+    // value ranges (as IntArith's), from the bounds above: forward |x| < 4q
+    // (loaded values, round outputs below 3.2 q), inverse |x| <= 1.5 q, inverse
+    // outputs |x| < 0.75 q before their conversion to [0, q)
+    __device__ __forceinline__ static bool in_fwd_range(T v, const DevTable &t) { return fabs(v) < 4.0 * t.qd; }
+    __device__ __forceinline__ static bool in_inv_range(T v, const DevTable &t) { return fabs(v) <= 1.5 * t.qd; }
+    __device__ __forceinline__ static bool in_out_range(T v, const DevTable &t) { return fabs(v) < 0.75 * t.qd; }
 };
 
 // ---------------------------------------------------------------------------

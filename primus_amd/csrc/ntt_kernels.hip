@@ -602,4 +602,21 @@ hipError_t launch_pointwise(const DevTable *tabs, uint32_t L, uint32_t log_n, ui
     return hipGetLastError();
 }
 
+// Range-check report of this translation unit's kernels (PH_RANGE_CHECK
+// builds): {violations, first site, first value's bits, its modulus},
+// cleared afterwards when asked.  Blocking copies on the null stream.
+hipError_t range_report_ntt(uint64_t *out, bool clear) {
+#if PH_RANGE_CHECK
+    hipError_t e = hipMemcpyFromSymbol(out, HIP_SYMBOL(g_range_report), 4 * sizeof(uint64_t));
+    if (e == hipSuccess && clear) {
+        const uint64_t zero[4] = {0, 0, 0, 0};
+        e = hipMemcpyToSymbol(HIP_SYMBOL(g_range_report), zero, sizeof(zero));
+    }
+    return e;
+#else
+    (void)out, (void)clear;
+    return hipErrorNotSupported;
+#endif
+}
+
 }  // namespace ph

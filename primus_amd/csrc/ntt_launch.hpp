@@ -47,6 +47,10 @@ enum class Op : int {
 constexpr int kOpCount = 6;
 
 bool needs_scratch(Op op, uint32_t log_n);
+// Range-check reports (ntt_rounds.hpp, PH_RANGE_CHECK builds) of the kernels in
+// ntt_kernels.hip and fhe_kernels.hip; hipErrorNotSupported in default builds.
+hipError_t range_report_ntt(uint64_t *out, bool clear);
+hipError_t range_report_fhe(uint64_t *out, bool clear);
 // Launch shape of the single-pass kernels (N <= 2^13): log2 of the
 // coefficients per thread (3 or 4) in the low bits, kShapeStream set for the
 // persistent streaming kernel (forward-first operations, 2^10 <= N <= 2^13).

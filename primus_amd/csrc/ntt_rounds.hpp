@@ -5,7 +5,54 @@
 
 #include "ntt_device.hpp"
 
+// PH_RANGE_CHECK=1 builds (a debug variant of the library) check every value
+// at the register rounds' boundaries against its arithmetic's contract
+// (A::in_fwd_range / in_inv_range / in_out_range) and count violations in a
+// device array of the translation unit: the kernels neither trap nor change
+// their results.  Default builds compile the checks out.
+#ifndef PH_RANGE_CHECK
+#define PH_RANGE_CHECK 0
+#endif
+
 namespace ph {
+
+#if PH_RANGE_CHECK
+// {violations, first site, first value's bits, its modulus}; one copy per
+// translation unit (no relocatable device code), read by its range_report_*.
+static __device__ unsigned long long g_range_report[4];
+#endif
+
+// Check sites.
+enum RangeSite : uint32_t {
+    kRangeFwdIn = 1,    // values entering forward rounds
+    kRangeFwdRound = 2, // values leaving a forward round
+    kRangeInvIn = 3,    // values entering inverse rounds
+    kRangeInvRound = 4, // values leaving an inverse round
+    kRangeInvOut = 5,   // inverse outputs
+};
+
+template <class A>
+__device__ __forceinline__ void range_check(const typename A::T *x, int count, uint32_t site, const DevTable &t) {
+#if PH_RANGE_CHECK
+    for (int j = 0; j < count; ++j) {
+        const bool ok = site == kRangeFwdIn || site == kRangeFwdRound ? A::in_fwd_range(x[j], t)
+                        : site == kRangeInvOut                       ? A::in_out_range(x[j], t)
+                                                                     : A::in_inv_range(x[j], t);
+        if (!ok) {
+            const unsigned long long prev = atomicAdd(&g_range_report[0], 1ull);
+            if (prev == 0) {
+                uint64_t bits;
+                __builtin_memcpy(&bits, &x[j], sizeof(bits));
+                g_range_report[1] = site;
+                g_range_report[2] = bits;
+                g_range_report[3] = t.q;
+            }
+        }
+    }
+#else
+    (void)x, (void)count, (void)site, (void)t;
+#endif
+}
 
 template <int LOGC, int LOGT>
 struct Geo {
@@ -177,6 +224,8 @@ struct FwdRounds {
             FwdRounds<A, LOGC, LOGT, I - 1, FENCE, PERM, SYNC0>::store(lds, x, t);
             X::before_load();
             load(lds, x, t);
+        } else {
+            range_check<A>(x, G::E, kRangeFwdIn, tab);
         }
 #pragma unroll
         for (int m = 0; m < (G::E >> R); ++m) {
@@ -193,6 +242,7 @@ struct FwdRounds {
             else
                 ct_round<A, R, FENCE, false, LANE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
         }
+        range_check<A>(x, G::E, kRangeFwdRound, tab);
         if constexpr (I + 1 < G::NR) FwdRounds<A, LOGC, LOGT, I + 1, FENCE, PERM, SYNC0>::run(x, lds, t, st0, b, tab);
     }
 };
@@ -257,6 +307,8 @@ struct InvRounds {
             InvRounds<A, LOGC, LOGT, I - 1, LAST, PERM, SYNC0, SCALED>::store(lds, x, t);
             X::before_load();
             load(lds, x, t);
+        } else {
+            range_check<A>(x, G::E, kRangeInvIn, tab);
         }
         constexpr int HR = G::first_scalar_inv() < G::NR ? G::first_scalar_inv() : G::NR - 1;
         if constexpr (I == HR) {
@@ -279,6 +331,7 @@ struct InvRounds {
                 gs_round<A, R, last_here, true, false, LANE, SCALED>(x + (m << R), P, (b << (LOGC - P - R)) + cl,
                                                                      n, A::inv(tab), tab);
         }
+        range_check<A>(x, G::E, last_here ? kRangeInvOut : kRangeInvRound, tab);
         if constexpr (I + 1 < G::NR)
             InvRounds<A, LOGC, LOGT, I + 1, LAST, PERM, SYNC0, SCALED>::run(x, lds, t, n, b, tab, hook);
     }

@@ -164,7 +164,8 @@ class HexlNttTable(UintNttTable):
     compute_forward / compute_inverse with HEXL's range factors, on the same
     device tables and kernels.  Full-range outputs and monomials are identical
     to UintNttTable's (the reference asserts it in primus_ntt/tests/hexl.rs);
-    lazy outputs lie in HEXL's ranges without being HEXL's own bit patterns."""
+    lazy outputs (output_mod_factor 4 forward, 2 inverse) are HEXL's own
+    radix-2 values (hexl/radix2.rs), bit for bit."""
 
     def compute_forward(self, operand, input_mod_factor, output_mod_factor):
         _check(lib().ph_hexl_compute_forward(self._h, _slice_ptr(operand), operand.size, input_mod_factor,
