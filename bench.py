@@ -592,6 +592,8 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="counter runs: only the fused batch kernel (--steps launches) and one 256 MiB "
                          "device-to-device copy as the byte-count calibration")
+    ap.add_argument("--profile-int", action="store_true",
+                    help="with --profile: the same batch on the integer-arithmetic table (IntCanonArith kernel)")
     ap.add_argument("--shape", type=int, default=None,
                     help="launch shape of the fused kernel instead of the autotuner's choice "
                          "(3/4: 8/16 coefficients per thread; +16: streaming)")
@@ -706,6 +708,8 @@ def main():
 
     K = args.steps
     if args.profile:
+        if args.profile_int:
+            step = int_step
         _, total = timer.run(step, K, args.warmup, latency=False)
         src = torch.empty(32 << 20, dtype=torch.int64, device=dev)
         dst = torch.empty_like(src)
@@ -713,7 +717,9 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)  # calibration: 256 MiB read + 256 MiB written
         torch.cuda.synchronize()
-        print(json.dumps({"profile": True, "kernel": kernel_signature(shape), "kernel_ms": total / K,
+        print(json.dumps({"profile": True, "kernel": kernel_signature(shape) if not args.profile_int
+                          else "chunk_kernel<ph::IntCanonArith, ...> (%d per thread)" % tuning.describe(int_shape)[0],
+                          "kernel_ms": total / K,
                           "launches": K + args.warmup, "calibration_copy_bytes": src.numel() * 8}))
         return 0
 
@@ -769,12 +775,32 @@ def main():
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated),
     # used only when they were taken on the kernel this run selected
     kernel_sig = kernel_signature(shape)
-    traffic, traffic_note = None, "no counter profile of %s" % kernel_sig
+    traffic, traffic_note, prof = None, "no counter profile of %s" % kernel_sig, None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f).get(kernel_sig)
         if prof:
             traffic, traffic_note = prof.get("traffic_bytes"), "counter profile of %s" % kernel_sig
+    # VALU issue time of one launch: SQ_INSTS_VALU of the counter profile, one
+    # wave instruction per 4 cycles on each of the 1024 SIMDs, at the engine
+    # clock the clock-probe variant measured inside the same kernel (the chip
+    # runs this binary64-dense kernel below its 2.4 GHz peak)
+    valu = {"valu_frac": None, "valu_note": "no SQ_INSTS_VALU / clock profile of %s" % kernel_sig}
+    if prof and prof.get("valu_insts"):
+        insts = prof["valu_insts"]
+        peak_us = insts * 4 / (1024 * 2.4e9) * 1e6
+        valu = {"valu_insts": insts, "valu_floor_us_peak_clock": peak_us,
+                "valu_frac_peak_clock": peak_us / (kernel_ms * 1e3)}
+        if prof.get("sclk_ghz"):
+            at_clk = insts * 4 / (1024 * prof["sclk_ghz"] * 1e9) * 1e6
+            valu.update({"sclk_ghz": prof["sclk_ghz"], "valu_issue_us": at_clk,
+                         "valu_frac": at_clk / (kernel_ms * 1e3),
+                         "valu_note": "valu_frac: SIMD cycles issuing VALU / launch cycles, at the measured "
+                                      "engine clock sclk_ghz; valu_frac_peak_clock prices the same "
+                                      "instructions at 2.4 GHz"})
+        else:
+            valu["valu_frac"] = valu["valu_frac_peak_clock"]
+            valu["valu_note"] = "valu_frac priced at the 2.4 GHz peak clock (no clock-probe profile)"
 
     out = {
         "metric": "NTTs/sec + HBM GB/s at N=4096x4 limbs; p50/p99 latency, batched vs plain",
@@ -842,6 +868,7 @@ def main():
             "kernel": kernel_sig + " (MODE 39: fused forward -> Shoup multiply -> inverse)",
             "kernel_ms": kernel_ms,
             "bytes_per_launch": BYTES_PER_STEP,
+            **valu,
         },
     }
     if not args.no_secondary:

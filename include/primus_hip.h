@@ -80,6 +80,21 @@ const char *ph_last_error(void);
  * zeroes the counters.  Synchronise the streams that ran the kernels first.
  * Default builds return PH_ERR_UNSUPPORTED. */
 int ph_debug_range_report(uint64_t report[4], int clear);
+/* Measurement builds only (no reference counterpart).  The library variant
+ * built with PH_CLOCK_PROBE=1 (libprimus_hip_probe.so) logs every wave of the
+ * transform kernels' single-pass chunk launches as it ends, 4 words per wave:
+ * {shader-clock counter, 100 MHz counter, HW_ID register | XCC_ID register <<
+ * 32, workgroup index}.  Each of PH_CLOCK_PROBE_SLOTS hardware wave slots
+ * keeps its last PH_CLOCK_PROBE_DEPTH entries as a ring (wave c of the slot
+ * at index c mod depth) and counts its waves.  entries (slots x depth x 4
+ * words) and counts (slots) receive them when not null; clear != 0 restarts
+ * the counts.  Successive entries of one slot over back-to-back launches give
+ * the engine clock the kernel ran at: the ratio of the two counters' advances
+ * times 100 MHz.  Synchronise first.  Default builds return
+ * PH_ERR_UNSUPPORTED. */
+#define PH_CLOCK_PROBE_SLOTS 65536
+#define PH_CLOCK_PROBE_DEPTH 32
+int ph_debug_clock_report(uint64_t *entries, uint32_t *counts, int clear);
 
 /* ---- NttTable ---- */
 

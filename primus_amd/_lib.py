@@ -12,6 +12,10 @@ LIB_PATH = os.path.join(PKG_DIR, "libprimus_hip.so")
 # debug variant: every transform kernel checks its values' ranges at round
 # boundaries (PH_RANGE_CHECK=1, read back with ph_debug_range_report)
 RC_LIB_PATH = os.path.join(PKG_DIR, "libprimus_hip_rc.so")
+# measurement variant: the single-pass chunk kernels log each wave's end with
+# the shader and 100 MHz clocks (PH_CLOCK_PROBE=1, read back with
+# ph_debug_clock_report)
+PROBE_LIB_PATH = os.path.join(PKG_DIR, "libprimus_hip_probe.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "primus_hip.h")
 CSRC = os.path.join(PKG_DIR, "csrc")
 SOURCES = ["capi.hip", "ntt_kernels.hip", "fhe_kernels.hip", "rns_kernels.hip", "copy_kernels.hip", "serving.cpp"]
@@ -32,7 +36,9 @@ HIPCC_FLAGS = [
 
 def build(verbose=False):
     """Compile every HIP source into primus_amd/libprimus_hip.so for gfx950,
-    and the same sources with PH_RANGE_CHECK=1 into libprimus_hip_rc.so.
+    the same sources with PH_RANGE_CHECK=1 into libprimus_hip_rc.so, and
+    ntt_kernels.hip with PH_CLOCK_PROBE=1 (linked with the default build's
+    other objects) into libprimus_hip_probe.so.
 
     Each (variant, source) is compiled to an object in parallel (one hipcc
     each), then each variant's objects are linked; every call recompiles
@@ -40,12 +46,17 @@ def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
-    variants = [(LIB_PATH, os.path.join(PKG_DIR, "_build"), []),
-                (RC_LIB_PATH, os.path.join(PKG_DIR, "_build", "rc"), ["-DPH_RANGE_CHECK=1"])]
+    objdir0 = os.path.join(PKG_DIR, "_build")
+    # (library, object folder, defines, sources compiled for it; the rest
+    # come from the default build)
+    variants = [(LIB_PATH, objdir0, [], srcs),
+                (RC_LIB_PATH, os.path.join(objdir0, "rc"), ["-DPH_RANGE_CHECK=1"], srcs),
+                (PROBE_LIB_PATH, os.path.join(objdir0, "probe"), ["-DPH_CLOCK_PROBE=1"],
+                 [os.path.join(CSRC, "ntt_kernels.hip")])]
     procs = []
-    for lib_path, objdir, defs in variants:
+    for lib_path, objdir, defs, own in variants:
         os.makedirs(objdir, exist_ok=True)
-        for src in srcs:
+        for src in own:
             obj = os.path.join(objdir, os.path.basename(src) + ".o")
             cmd = [hipcc] + compile_flags + defs + ["-pthread", "-c", "-o", obj, src]
             procs.append((lib_path, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
@@ -60,11 +71,14 @@ def build(verbose=False):
         objs.setdefault(lib_path, []).append(obj)
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
-    for lib_path, _, _ in variants:
+    for lib_path, _, _, _ in variants:
+        names = {os.path.basename(o) for o in objs[lib_path]}
+        shared = [os.path.join(objdir0, os.path.basename(s) + ".o") for s in srcs
+                  if os.path.basename(s) + ".o" not in names]
         tmp = lib_path + ".tmp"
         # only the ph_* entry points of the header are exported (exports.map)
         link = ["-pthread", "-Wl,--version-script=" + os.path.join(CSRC, "exports.map"), "-o", tmp]
-        res = subprocess.run([hipcc] + HIPCC_FLAGS + link + objs[lib_path], capture_output=True, text=True)
+        res = subprocess.run([hipcc] + HIPCC_FLAGS + link + objs[lib_path] + shared, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError("hipcc link failed:\n" + res.stdout + res.stderr)
         os.replace(tmp, lib_path)
@@ -100,6 +114,21 @@ def lib():
     return _lib
 
 
+def use_probe():
+    """Routes this process's entry points to the clock-probe variant
+    (measurement runs only; call before any other use of the package)."""
+    global _lib
+    if _lib is not None:
+        raise RuntimeError("the library is already loaded in this process")
+    if not os.path.exists(PROBE_LIB_PATH):
+        raise RuntimeError("libprimus_hip_probe.so is not built (%s); run __graft_entry__.build() first"
+                           % PROBE_LIB_PATH)
+    L = ctypes.CDLL(PROBE_LIB_PATH)
+    _declare(L)
+    _lib = L
+    return L
+
+
 _rc_lib = None
 
 
@@ -130,6 +159,7 @@ def _declare(L):
         "ph_status_string": (ctypes.c_char_p, [i]),
         "ph_last_error": (ctypes.c_char_p, []),
         "ph_debug_range_report": (i, [p64, i]),
+        "ph_debug_clock_report": (i, [p64, ctypes.POINTER(ctypes.c_uint32), i]),
         "ph_ntt_table_new": (i, [u32, u64, pvp]),
         "ph_ntt_table_new_ex": (i, [u32, u64, u32, pvp]),
         "ph_ntt_table_arith": (i, [vp]),

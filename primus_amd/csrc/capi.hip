@@ -323,6 +323,13 @@ const char *ph_status_string(int status) {
 
 const char *ph_last_error(void) { return g_last_error.c_str(); }
 
+int ph_debug_clock_report(uint64_t *entries, uint32_t *counts, int clear) {
+    const hipError_t e = clock_probe_report(entries, counts, clear != 0);
+    if (e == hipErrorNotSupported) return fail(PH_ERR_UNSUPPORTED, "library built without PH_CLOCK_PROBE");
+    if (e != hipSuccess) return hip_fail(e, "clock-probe report");
+    return PH_OK;
+}
+
 int ph_debug_range_report(uint64_t report[4], int clear) {
     if (!report) return fail(PH_ERR_INVALID, "null report");
 #if PH_RANGE_CHECK
@@ -1698,6 +1705,8 @@ int ph_rlwe_plan_autotune(ph_rlwe_plan *p, const uint64_t *d_a, const int8_t *d_
 uint32_t ph_rlwe_plan_grid(const ph_rlwe_plan *p) { return p ? p->grid : 0; }
 
 static_assert(PH_SHAPE_STREAM == kShapeStream, "the header's shape flag is the launcher's");
+static_assert(PH_CLOCK_PROBE_SLOTS == kClockProbeSlots && PH_CLOCK_PROBE_DEPTH == kClockProbeDepth,
+              "the header's probe log shape is the kernels'");
 
 int ph_tuning_set(uint32_t log_n, int op, int arith, int shape) {
     if (log_n > kMaxLogN || op < 0 || op > 4 || arith < 0 || arith > 1 || !valid_shape(shape))

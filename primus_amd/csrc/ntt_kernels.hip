@@ -130,6 +130,51 @@ __device__ __forceinline__ void after_forward(const KArgs &a, typename A::T *x, 
     }
 }
 
+// PH_CLOCK_PROBE=1 builds (a measurement variant of the library) log every
+// wave of every chunk_kernel launch as it ends: the shader-clock counter
+// (s_memtime), the constant 100 MHz counter (s_memrealtime), its placement
+// (HW_ID, XCC_ID) and its workgroup.  Successive launches' entries of one
+// hardware wave slot give the engine clock the kernel ran at (the ratio of
+// the two counters' advances times 100 MHz), and one launch's entries the
+// spread of its workgroups' end times (clock_probe_report).
+// * Only the end is stamped, after the wave's last load: a counter read or a
+//   store earlier in the kernel counts as a possible write to the tables it
+//   reads afterwards, which turns the scalar twiddle loads into vector loads
+//   and changes the kernel being measured.
+// * Each hardware wave slot (XCC, shader engine, CU, SIMD, wave slot) keeps a
+//   ring of its last kClockProbeDepth entries and a count.  The waves of one
+//   slot run one after another, so a plain load and store of the count
+//   suffice; one counter shared by every wave would serialise their ends.
+// Default builds compile the log out.
+#ifndef PH_CLOCK_PROBE
+#define PH_CLOCK_PROBE 0
+#endif
+#if PH_CLOCK_PROBE
+// entry: {shader clock, 100 MHz clock, HW_ID | XCC_ID << 32, blockIdx.x}
+static __device__ unsigned long long g_clock_log[(uint64_t)kClockProbeSlots * kClockProbeDepth * 4];
+static __device__ unsigned int g_clock_count[kClockProbeSlots];
+
+__device__ __forceinline__ void clock_probe_end() {
+    const uint64_t clk = clock64(), wall = wall_clock64();
+    const uint32_t hw = (uint32_t)__builtin_amdgcn_s_getreg(4 | (31 << 11));    // HW_ID
+    const uint32_t xcc = (uint32_t)__builtin_amdgcn_s_getreg(20 | (31 << 11));  // XCC_ID
+    const uint32_t wave = hw & 15u, simd = (hw >> 4) & 3u, cu = (hw >> 8) & 15u, se = (hw >> 13) & 7u;
+    const uint32_t slot = ((((xcc & 7u) * 8u + se) * 16u + cu) * 4u + simd) * 16u + wave;
+    if (__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)) == 0) {  // the wave's first active lane
+        const uint32_t c = g_clock_count[slot];
+        g_clock_count[slot] = c + 1;
+        unsigned long long *p = g_clock_log + ((uint64_t)slot * kClockProbeDepth + c % kClockProbeDepth) * 4u;
+        p[0] = clk;
+        p[1] = wall;
+        p[2] = hw | ((uint64_t)xcc << 32);
+        p[3] = blockIdx.x;
+    }
+}
+#define PH_PROBE_END() clock_probe_end()
+#else
+#define PH_PROBE_END() ((void)0)
+#endif
+
 template <class A, int LOGC, int LOGT, int MODE>
 __global__ __launch_bounds__(1 << LOGT, (Occupancy<LOGC, LOGT, MODE>::value))
 __attribute__((amdgpu_waves_per_eu(1, Occupancy<LOGC, LOGT, MODE>::value))) void chunk_kernel(KArgs a) {
@@ -144,7 +189,10 @@ __attribute__((amdgpu_waves_per_eu(1, Occupancy<LOGC, LOGT, MODE>::value))) void
     uint32_t limb;
     if constexpr ((MODE & M_MUL_POLY) != 0) {
         limb = a.limb_of ? a.limb_of[poly] : (uint32_t)(poly % a.L);
-        if (limb >= a.L) return;  // a row with no table is left as it is (whole workgroup)
+        if (limb >= a.L) {  // a row with no table is left as it is (whole workgroup)
+            PH_PROBE_END();
+            return;
+        }
     } else {
         limb = (uint32_t)(poly % a.L);
     }
@@ -181,6 +229,7 @@ __attribute__((amdgpu_waves_per_eu(1, Occupancy<LOGC, LOGT, MODE>::value))) void
         InvRounds<A, LOGC, LOGT, 0, LAST, A::kRegisterOrder, false>::run(x, lds, t, n, b, tab);
         InvRounds<A, LOGC, LOGT, G::NR - 1, LAST, A::kRegisterOrder>::store_global(data, x, t, tab);
     }
+    PH_PROBE_END();
 }
 
 // Waves per SIMD of the streaming kernel: as the chunk kernel's, but two at
@@ -615,6 +664,26 @@ hipError_t range_report_ntt(uint64_t *out, bool clear) {
     return e;
 #else
     (void)out, (void)clear;
+    return hipErrorNotSupported;
+#endif
+}
+
+// Clock-probe log (PH_CLOCK_PROBE builds): every slot's ring of entries
+// (kClockProbeSlots x kClockProbeDepth x 4 words) into out and its count of
+// waves logged into counts (either may be null); zeroed afterwards when
+// asked.  Blocking copies on the null stream.
+hipError_t clock_probe_report(uint64_t *out, uint32_t *counts, bool clear) {
+#if PH_CLOCK_PROBE
+    hipError_t e = hipSuccess;
+    if (out) e = hipMemcpyFromSymbol(out, HIP_SYMBOL(g_clock_log), sizeof(g_clock_log));
+    if (e == hipSuccess && counts) e = hipMemcpyFromSymbol(counts, HIP_SYMBOL(g_clock_count), sizeof(g_clock_count));
+    if (e == hipSuccess && clear) {
+        static const unsigned int zero[kClockProbeSlots] = {};
+        e = hipMemcpyToSymbol(HIP_SYMBOL(g_clock_count), zero, sizeof(zero));
+    }
+    return e;
+#else
+    (void)out, (void)counts, (void)clear;
     return hipErrorNotSupported;
 #endif
 }

@@ -50,6 +50,13 @@ bool needs_scratch(Op op, uint32_t log_n);
 // Range-check reports (ntt_rounds.hpp, PH_RANGE_CHECK builds) of the kernels in
 // ntt_kernels.hip and fhe_kernels.hip; hipErrorNotSupported in default builds.
 hipError_t range_report_ntt(uint64_t *out, bool clear);
+// Clock-probe log of chunk_kernel waves (ntt_kernels.hip, PH_CLOCK_PROBE
+// builds): per hardware wave slot (8 XCCs x 8 shader engines x 16 CUs x 4
+// SIMDs x 16 waves) a ring of its last kClockProbeDepth entries of 4 words
+// and a count; hipErrorNotSupported in default builds.
+constexpr uint32_t kClockProbeSlots = 8u * 8u * 16u * 4u * 16u;
+constexpr uint32_t kClockProbeDepth = 32;
+hipError_t clock_probe_report(uint64_t *out, uint32_t *counts, bool clear);
 hipError_t range_report_fhe(uint64_t *out, bool clear);
 // Launch shape of the single-pass kernels (N <= 2^13): log2 of the
 // coefficients per thread (3 or 4) in the low bits, kShapeStream set for the

@@ -80,12 +80,27 @@ struct Geo {
     static_assert(LOGE >= 1, "a thread needs at least two coefficients");
 };
 
+// Chunk index of a round whose groups span 2^R slots at stride 2^B (B = OB
+// for the forward, P for the inverse) as c = gi >> B.  With B = 3 and R = 3 a
+// 16-lane quarter of a wave covers two chunks of 8 consecutive slots 64 slots
+// apart, and lds_pos puts the second run 68 words after the first: four of
+// its 64-bit accesses then share banks with the first run's (32 four-byte
+// banks, 16 lanes per cycle).  Swapping bits 0 and 1 of c pairs chunks two
+// apart instead (136 words, a disjoint half of the banks).  Bits 0 and 1 of c
+// are bits B and B+1 of the thread index (LOGT >= B + 2), so the slot stays a
+// thread part plus a register part and every round's map stays a bijection.
+template <int LOGT, int B, int R>
+__host__ __device__ __forceinline__ constexpr uint32_t chunk_perm(uint32_t c) {
+    if constexpr (B == 3 && R == 3 && LOGT >= B + 2) return (c & ~3u) | ((c & 1u) << 1) | ((c >> 1) & 1u);
+    else return c;
+}
+
 // Local slot of register (m, j) in the forward round covering local stages [S, S+R).
 template <int LOGC, int LOGT, int S, int R>
 __host__ __device__ __forceinline__ constexpr uint32_t fwd_slot(uint32_t t, int m, int j) {
     constexpr int OB = LOGC - S - R;
     const uint32_t gi = t + ((uint32_t)m << LOGT);
-    const uint32_t c = gi >> OB;
+    const uint32_t c = chunk_perm<LOGT, OB, R>(gi >> OB);
     const uint32_t o = gi & ((1u << OB) - 1u);
     return (c << (LOGC - S)) + ((uint32_t)j << OB) + o;
 }
@@ -94,7 +109,7 @@ __host__ __device__ __forceinline__ constexpr uint32_t fwd_slot(uint32_t t, int 
 template <int LOGC, int LOGT, int P, int R>
 __host__ __device__ __forceinline__ constexpr uint32_t inv_slot(uint32_t t, int m, int j) {
     const uint32_t gi = t + ((uint32_t)m << LOGT);
-    const uint32_t c = gi >> P;
+    const uint32_t c = chunk_perm<LOGT, P, R>(gi >> P);
     const uint32_t o = gi & ((1u << P) - 1u);
     return (c << (P + R)) + ((uint32_t)j << P) + o;
 }
@@ -232,7 +247,7 @@ struct FwdRounds {
             constexpr int OB = LOGC - S - R;
             uint32_t cl;
             if constexpr (OB >= LOGT) cl = (uint32_t)m >> (OB - LOGT);  // workgroup-uniform
-            else cl = (t + ((uint32_t)m << LOGT)) >> OB;
+            else cl = chunk_perm<LOGT, OB, R>((t + ((uint32_t)m << LOGT)) >> OB);
             // a wave's 64 lanes share cl: scalar twiddle loads
             if constexpr (OB >= 6 && OB < LOGT) cl = __builtin_amdgcn_readfirstlane(cl);
             constexpr bool LANE = OB < 6 && OB < LOGT;  // cl differs between lanes
@@ -321,7 +336,7 @@ struct InvRounds {
         for (int m = 0; m < (G::E >> R); ++m) {
             uint32_t cl;
             if constexpr (P >= LOGT) cl = (uint32_t)m >> (P - LOGT);  // workgroup-uniform
-            else cl = (t + ((uint32_t)m << LOGT)) >> P;
+            else cl = chunk_perm<LOGT, P, R>((t + ((uint32_t)m << LOGT)) >> P);
             if constexpr (P >= 6 && P < LOGT) cl = __builtin_amdgcn_readfirstlane(cl);  // as in FwdRounds
             constexpr bool LANE = P < 6 && P < LOGT;
             if constexpr (PERM)

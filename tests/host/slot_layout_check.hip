@@ -6,7 +6,12 @@
 //     position: lds_pos(slot(t,m,j)) == lds_pos(slot(t,0,0)) + lds_pos(slot(0,m,j)).
 // The kernels address global rows and the LDS exchanges with one thread base
 // per round plus constant offsets, which is valid only under the second
-// property.  Built for the host only; prints "ok <checks>" and exits 0.
+// property.
+// It also counts LDS bank conflicts of every round's 64-bit accesses under
+// the model the hardware's counters agree with (32 four-byte banks, a wave's
+// access served 16 lanes per cycle): the single-pass shapes of 8 and 16
+// coefficients per thread (2^6 <= C <= 2^13, full waves) must have none.
+// Built for the host only; prints "ok <checks>" and exits 0.
 #include <cstdio>
 #include <vector>
 
@@ -16,8 +21,38 @@ using namespace ph;
 
 static long checks = 0, failures = 0;
 
+static long conflict_cycles = 0, conflict_failures = 0;
+
+// Extra cycles of one round's accesses: for every wave, register and 16-lane
+// quarter, the largest number of lanes meeting in one bank, less one.
+template <class F>
+static long bank_conflicts(int logt, int E, int R, F slot) {
+    long extra = 0;
+    for (uint32_t w = 0; w < (1u << logt) / 64; ++w)
+        for (int m = 0; m < (E >> R); ++m)
+            for (int j = 0; j < (1 << R); ++j)
+                for (uint32_t q = 0; q < 64; q += 16) {
+                    int use[32] = {0}, worst = 0;
+                    for (uint32_t l = q; l < q + 16; ++l) {
+                        const uint32_t p = lds_pos(slot(64 * w + l, m, j));
+                        for (uint32_t d = 0; d < 2; ++d) {
+                            const int b = (int)((2 * p + d) % 32);
+                            worst = ++use[b] > worst ? use[b] : worst;
+                        }
+                    }
+                    extra += worst - 1;
+                }
+    return extra;
+}
+
 template <class F>
 static void check_round(int logc, int logt, int I, char kind, int E, int R, F slot) {
+    if (logt >= 6 && logc >= 6 && (E == 8 || E == 16)) {
+        const long c = bank_conflicts(logt, E, R, slot);
+        conflict_cycles += c;
+        if (c && conflict_failures++ < 10)
+            std::printf("BANK CONFLICTS C=2^%d T=2^%d round %c%d: %ld\n", logc, logt, kind, I, c);
+    }
     std::vector<int> seen(1u << logc, 0);
     for (uint32_t t = 0; t < (1u << logt); ++t) {
         const uint32_t base = lds_pos(slot(t, 0, 0));
@@ -66,8 +101,8 @@ static void all() {
 
 int main() {
     all<1>();
-    if (failures) {
-        std::printf("%ld of %ld checks failed\n", failures, checks);
+    if (failures || conflict_failures) {
+        std::printf("%ld of %ld checks failed; %ld rounds with bank conflicts\n", failures, checks, conflict_failures);
         return 1;
     }
     std::printf("ok %ld\n", checks);
