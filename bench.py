@@ -855,8 +855,8 @@ def main():
         },
         "batched_vs_plain": plain_ms / ms_per_step,
         "arith": table.arith(),
-        "autotune": {"coefficients_per_thread": tuning.describe(shape)[0], "streaming": tuning.describe(shape)[1],
-                     "best_ms": shape_ms},
+        "autotune": dict(tuning.fields(shape), shape=shape, best_ms=shape_ms,
+                         candidates=len(tuning.shapes(LOG_N, "fwd_mul_inv"))),
         "int_path": {
             "desc": "same fused kernel on 64-bit integer Shoup arithmetic (32-bit multiply chains)",
             "coefficients_per_thread": tuning.describe(int_shape)[0],

@@ -489,18 +489,35 @@ int ph_server_run_poisson(ph_server *server, const ph_serve_request *requests, s
 #define PH_OP_NEGACYCLIC_MUL 4
 
 /* Process-wide tuning table: for (log_n, op, arith: 0 integer / 1 binary64) the
- * launch shape of the single-pass kernels (N <= 2^13): log2 of coefficients
- * per thread (3 or 4), plus PH_SHAPE_STREAM for the persistent streaming
- * kernel (forward-first operations, N >= 2^10); 0 = the default (4).  Results
- * never depend on it. */
+ * launch shape of the transform kernels, as bit fields (0 = the default shape):
+ *   - log2 of the coefficients each thread holds, 3 or 4 (radix-8 or radix-16
+ *     register rounds; 0: 4);
+ *   - PH_SHAPE_STREAM: the persistent streaming kernel (forward-first
+ *     operations, 2^10 <= N <= 2^13), with PH_SHAPE_LPB(k), k = 0..3, the
+ *     limb-polynomials each workgroup walks: k = 0 sizes the grid to one
+ *     resident generation of workgroups, k > 0 to ceil(count / 2^k) of them;
+ *   - PH_SHAPE_TILE(c), c = 11..13 (N >= 2^14, N / 2^c <= 32): the two-pass
+ *     schedule's LDS tile, chunks of 2^c coefficients after columns of
+ *     N / 2^c (0: 4096).
+ * ph_tuning_set rejects a shape whose fields do not apply at that degree.
+ * Results never depend on the shape. */
 #define PH_SHAPE_STREAM 16
+#define PH_SHAPE_LPB(k) ((k) << 5)
+#define PH_SHAPE_TILE(c) ((c) << 8)
 int ph_tuning_set(uint32_t log_n, int op, int arith, int shape);
 int ph_tuning_get(uint32_t log_n, int op, int arith);
+/* The shapes ph_autotune tries for (log_n, op): writes up to `max` of them to
+ * shapes and returns how many there are (negative status on a bad query). */
+int ph_tuning_shapes(uint32_t log_n, int op, int *shapes, int max);
+/* Save / load the table as text lines "log_n op arith shape".  The file the
+ * environment variable PH_TUNING_FILE names, if it exists, is loaded once
+ * when the process creates its first table; an invalid entry in it makes
+ * table creation fail with PH_ERR_INVALID. */
 int ph_tuning_save(const char *path);
 int ph_tuning_load(const char *path);
-/* Times every launch shape of `op` for this table on `count` limb-polynomials
- * (reps timed launches each), stores the fastest in the tuning table and
- * reports it. */
+/* Times every launch shape of `op` for this table (ph_tuning_shapes) on
+ * `count` limb-polynomials (reps timed launches each), stores the fastest in
+ * the tuning table and reports it. */
 int ph_autotune(const ph_dcrt_table *table, int op, size_t count, int reps, int *chosen, float *best_ms);
 
 /* ---- device memory helpers (for callers without their own HIP allocator) ---- */

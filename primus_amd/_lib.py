@@ -160,6 +160,7 @@ def _declare(L):
         "ph_last_error": (ctypes.c_char_p, []),
         "ph_debug_range_report": (i, [p64, i]),
         "ph_debug_clock_report": (i, [p64, ctypes.POINTER(ctypes.c_uint32), i]),
+        "ph_tuning_shapes": (i, [u32, i, ctypes.POINTER(ctypes.c_int), i]),
         "ph_ntt_table_new": (i, [u32, u64, pvp]),
         "ph_ntt_table_new_ex": (i, [u32, u64, u32, pvp]),
         "ph_ntt_table_arith": (i, [vp]),

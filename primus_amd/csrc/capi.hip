@@ -352,9 +352,29 @@ int ph_ntt_table_new(uint32_t log_n, uint64_t modulus, ph_ntt_table **out) {
     return ph_ntt_table_new_ex(log_n, modulus, 0, out);
 }
 
+// The tuning table a deployment keeps in the file PH_TUNING_FILE names
+// (ph_tuning_save's format), loaded once, when the first table is created.  A
+// file that does not exist yet is no error (nothing has been tuned); one that
+// holds an invalid entry fails every table creation, so that a damaged table
+// is not silently ignored.
+static int load_env_tuning() {
+    static std::once_flag once;
+    static int rc = PH_OK;
+    std::call_once(once, [] {
+        const char *path = getenv("PH_TUNING_FILE");
+        if (!path || !*path) return;
+        FILE *f = fopen(path, "r");
+        if (!f) return;
+        fclose(f);
+        rc = ph_tuning_load(path);
+    });
+    return rc == PH_OK ? PH_OK : fail(rc, "the tuning file PH_TUNING_FILE names holds an invalid entry");
+}
+
 int ph_ntt_table_new_ex(uint32_t log_n, uint64_t modulus, uint32_t flags, ph_ntt_table **out) {
     if (!out) return fail(PH_ERR_INVALID, "null output");
     *out = nullptr;
+    if (const int rc = load_env_tuning()) return rc;
     return build_table(log_n, modulus, flags, out);
 }
 
@@ -447,6 +467,7 @@ int ph_dcrt_table_new_ex(uint32_t log_n, const uint64_t *moduli, uint32_t moduli
                          ph_dcrt_table **out) {
     if (!out || !moduli || moduli_count == 0) return fail(PH_ERR_INVALID, "null moduli or output");
     *out = nullptr;
+    if (const int rc = load_env_tuning()) return rc;
     ph_dcrt_table *d = new (std::nothrow) ph_dcrt_table();
     if (!d) return fail(PH_ERR_NO_MEMORY, "host allocation failed");
     d->log_n = log_n;
@@ -1704,15 +1725,26 @@ int ph_rlwe_plan_autotune(ph_rlwe_plan *p, const uint64_t *d_a, const int8_t *d_
 
 uint32_t ph_rlwe_plan_grid(const ph_rlwe_plan *p) { return p ? p->grid : 0; }
 
-static_assert(PH_SHAPE_STREAM == kShapeStream, "the header's shape flag is the launcher's");
+static_assert(PH_SHAPE_STREAM == kShapeStream && PH_SHAPE_LPB(1) == (1 << kShapeLpbShift) &&
+                  PH_SHAPE_TILE(1) == (1 << kShapeTileShift),
+              "the header's shape fields are the launcher's");
 static_assert(PH_CLOCK_PROBE_SLOTS == kClockProbeSlots && PH_CLOCK_PROBE_DEPTH == kClockProbeDepth,
               "the header's probe log shape is the kernels'");
 
 int ph_tuning_set(uint32_t log_n, int op, int arith, int shape) {
-    if (log_n > kMaxLogN || op < 0 || op > 4 || arith < 0 || arith > 1 || !valid_shape(shape))
+    if (log_n > kMaxLogN || op < 0 || op > 4 || arith < 0 || arith > 1 || !valid_shape_for(shape, log_n))
         return fail(PH_ERR_INVALID, "bad tuning key or value");
     g_tuning[log_n][op][arith].store(shape, std::memory_order_relaxed);
     return PH_OK;
+}
+
+int ph_tuning_shapes(uint32_t log_n, int op, int *shapes, int max) {
+    if (log_n < kMinLogN || log_n > kMaxLogN || op < 0 || op > 4 || max < 0 || (max > 0 && !shapes))
+        return -fail(PH_ERR_INVALID, "bad tuning-shape query");
+    int tmp[64];
+    const int n = tuning_candidates(log_n, (Op)op, tmp, 64);
+    for (int i = 0; i < n && i < max; ++i) shapes[i] = tmp[i];
+    return n;
 }
 
 int ph_tuning_get(uint32_t log_n, int op, int arith) {
@@ -1775,10 +1807,11 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
     int best_shape = 0;
     // the shape in force before the trials: put back if they fail part-way
     const int prior_shape = g_tuning[d->log_n][op][f64 ? 1 : 0].load();
-    // 16 / 8 coefficients per thread, each as one workgroup per polynomial
-    // and as the persistent streaming kernel (forward-first operations)
-    const int candidates[4] = {4, 3, 4 | kShapeStream, 3 | kShapeStream};
-    const bool can_stream = o == Op::FwdLazy || o == Op::FwdCanon || o == Op::FwdMulInv;
+    // every shape of this degree and operation (tuning_candidates): radix of
+    // the register rounds, one workgroup per polynomial or the streaming kernel
+    // with its limb-polynomials per workgroup, the two-pass LDS tile
+    int candidates[64];
+    const int ncand = tuning_candidates(d->log_n, o, candidates, 64);
     if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
         hipEventCreate(&e1) != hipSuccess) {
         rc = fail(PH_ERR_DEVICE, "autotune set-up failed");
@@ -1816,10 +1849,8 @@ int ph_autotune(const ph_dcrt_table *d, int op, size_t count, int reps, int *cho
             }
         }
     }
-    for (int shape : candidates) {
-        const bool single = d->log_n >= 10 && d->log_n <= kMaxSinglePassLogN;
-        if ((shape & 15) == 3 && !single) continue;
-        if ((shape & kShapeStream) && !(single && can_stream)) continue;
+    for (int ci = 0; ci < ncand; ++ci) {
+        const int shape = candidates[ci];
         g_tuning[d->log_n][op][f64 ? 1 : 0].store(shape);
         // warm-up, then timed repetitions
         for (int r = 0; r <= reps; ++r) {

@@ -457,14 +457,18 @@ static uint32_t stream_slots() {
     return v;
 }
 
+// lpb_log: 0 sizes the grid to one resident generation (or the batch, if
+// smaller); k > 0 to ceil(count / 2^k) workgroups, each walking about 2^k
+// limb-polynomials, at most one generation.
 template <class A, int LOGC, int LOGT, int MODE>
-static hipError_t launch_stream(const KArgs &a0, uint64_t count, hipStream_t s) {
+static hipError_t launch_stream(const KArgs &a0, uint64_t count, int lpb_log, hipStream_t s) {
     if (count == 0) return hipSuccess;
     const uint32_t slots = stream_slots<A, LOGC, LOGT, MODE>();
     if (slots == 0) return hipErrorInvalidDevice;
     KArgs a = a0;
     a.count = count;
-    const uint64_t grid = std::min<uint64_t>(count, slots);
+    const uint64_t want = lpb_log > 0 ? (count + (1ull << lpb_log) - 1) >> lpb_log : count;
+    const uint64_t grid = std::min<uint64_t>(want, slots);
     stream_kernel<A, LOGC, LOGT, MODE><<<dim3((uint32_t)grid), dim3(1u << LOGT), 0, s>>>(a);
     return hipGetLastError();
 }
@@ -479,18 +483,18 @@ static hipError_t launch_column(const KArgs &a, uint32_t log_n, uint64_t count, 
 
 // Chunk or streaming launch of one forward-first mode.
 template <class A, int LOGN, int LOGT, int MODE>
-static hipError_t launch_fwd_first(const KArgs &a, uint64_t count, bool stream, hipStream_t s) {
+static hipError_t launch_fwd_first(const KArgs &a, uint64_t count, int shape, hipStream_t s) {
     if constexpr (LOGN >= 10) {
-        if (stream) return launch_stream<A, LOGN, LOGT, MODE>(a, count, s);
+        if (shape & kShapeStream) return launch_stream<A, LOGN, LOGT, MODE>(a, count, shape_lpb_log(shape), s);
     }
     return launch_chunk<A, LOGN, LOGT, MODE>(a, count, s);
 }
 
 // Single-pass launch for log_n = LOGN <= 13, E = 2^LOGE coefficients per
-// thread, the mode chosen at run time; `stream` selects the persistent
-// streaming kernel where the mode allows it.
+// thread, the mode chosen at run time; kShapeStream in `shape` selects the
+// persistent streaming kernel where the mode allows it.
 template <int LOGN, int LOGE>
-static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, int arith, bool stream, hipStream_t s) {
+static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, int arith, int stream, hipStream_t s) {
     constexpr int LOGT = LOGN > LOGE ? LOGN - LOGE : 0;
     a.blocks_per_poly = 1;
     if constexpr (LOGN >= 6) {
@@ -539,17 +543,18 @@ static hipError_t single_pass_e(Op op, KArgs a, uint64_t count, int arith, bool 
 // round) for 2^10 <= N <= 2^13 when the tuning asks for it.
 template <int LOGN>
 static hipError_t single_pass(Op op, KArgs a, uint64_t count, int arith, int shape, hipStream_t s) {
-    const bool stream = (shape & kShapeStream) != 0;
+    const int stream = shape & (kShapeStream | (3 << kShapeLpbShift));
     if constexpr (LOGN >= 10) {
-        if ((shape & 15) == 3) return single_pass_e<LOGN, 3>(op, a, count, arith, stream, s);
+        if (shape_loge(shape) == 3) return single_pass_e<LOGN, 3>(op, a, count, arith, stream, s);
     }
     return single_pass_e<LOGN, 4>(op, a, count, arith, stream, s);
 }
 
-// Two-pass schedule for log_n = 12 + LOGR: columns of 2^LOGR, chunks of 4096.
-template <int LOGR>
+// Two-pass schedule for log_n = LOGC + LOGR: columns of 2^LOGR, then chunks of
+// 2^LOGC in LDS with 2^LOGE coefficients per thread.
+template <int LOGC, int LOGR, int LOGE>
 static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStream_t s) {
-    constexpr int LOGC = 12, LOGT = 8;
+    constexpr int LOGT = LOGC - LOGE;
     using A = IntArith;
     const uint32_t log_n = LOGC + LOGR;
     a.blocks_per_poly = 1u << LOGR;
@@ -600,6 +605,22 @@ static hipError_t two_pass(Op op, KArgs a, uint64_t count, void *scratch, hipStr
     return hipErrorInvalidValue;
 }
 
+// The two-pass launch of ring degree 2^LOGN in the tile and radix the shape
+// names (valid_shape_for); any other shape runs the default tile of 4096.
+template <int LOGN>
+static hipError_t two_pass_shape(Op op, KArgs a, uint64_t count, void *scratch, int shape, hipStream_t s) {
+    const int tile = valid_shape_for(shape, LOGN) ? shape_tile_log(shape) : 12;
+    const bool e8 = shape_loge(shape) == 3;
+    if constexpr (LOGN - 11 <= 5) {
+        if (tile == 11) return e8 ? two_pass<11, LOGN - 11, 3>(op, a, count, scratch, s)
+                                  : two_pass<11, LOGN - 11, 4>(op, a, count, scratch, s);
+    }
+    if (tile == 13) return e8 ? two_pass<13, LOGN - 13, 3>(op, a, count, scratch, s)
+                              : two_pass<13, LOGN - 13, 4>(op, a, count, scratch, s);
+    return e8 ? two_pass<12, LOGN - 12, 3>(op, a, count, scratch, s)
+              : two_pass<12, LOGN - 12, 4>(op, a, count, scratch, s);
+}
+
 bool needs_scratch(Op op, uint32_t log_n) { return op == Op::PolyMul && log_n > kMaxSinglePassLogN; }
 
 hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t count, void *scratch, int arith,
@@ -618,12 +639,36 @@ hipError_t launch_transform(Op op, const KArgs &a, uint32_t log_n, uint64_t coun
         case 11: return single_pass<11>(op, a, count, arith, shape, s);
         case 12: return single_pass<12>(op, a, count, arith, shape, s);
         case 13: return single_pass<13>(op, a, count, arith, shape, s);
-        case 14: return two_pass<2>(op, a, count, scratch, s);
-        case 15: return two_pass<3>(op, a, count, scratch, s);
-        case 16: return two_pass<4>(op, a, count, scratch, s);
-        case 17: return two_pass<5>(op, a, count, scratch, s);
+        case 14: return two_pass_shape<14>(op, a, count, scratch, shape, s);
+        case 15: return two_pass_shape<15>(op, a, count, scratch, shape, s);
+        case 16: return two_pass_shape<16>(op, a, count, scratch, shape, s);
+        case 17: return two_pass_shape<17>(op, a, count, scratch, shape, s);
     }
     return hipErrorInvalidValue;
+}
+
+int tuning_candidates(uint32_t log_n, Op op, int *out, int max) {
+    int n = 0;
+    const auto add = [&](int shape) {
+        if (n < max) out[n] = shape;
+        ++n;
+    };
+    if (log_n > kMaxSinglePassLogN) {
+        // LDS tile x radix of the chunk pass
+        for (int tile = 11; tile <= 13; ++tile)
+            for (int e : {4, 3})
+                if (valid_shape_for(e | (tile << kShapeTileShift), log_n)) add(e | (tile << kShapeTileShift));
+    } else if (log_n >= 10) {
+        const bool fwd_first = op == Op::FwdLazy || op == Op::FwdCanon || op == Op::FwdMulInv;
+        for (int e : {4, 3}) {
+            add(e);
+            if (fwd_first)
+                for (int lpb = 0; lpb <= 3; ++lpb) add(e | kShapeStream | (lpb << kShapeLpbShift));
+        }
+    } else {
+        add(4);
+    }
+    return std::min(n, max);
 }
 
 hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, uint64_t coeff,

@@ -58,12 +58,42 @@ constexpr uint32_t kClockProbeSlots = 8u * 8u * 16u * 4u * 16u;
 constexpr uint32_t kClockProbeDepth = 32;
 hipError_t clock_probe_report(uint64_t *out, uint32_t *counts, bool clear);
 hipError_t range_report_fhe(uint64_t *out, bool clear);
-// Launch shape of the single-pass kernels (N <= 2^13): log2 of the
-// coefficients per thread (3 or 4) in the low bits, kShapeStream set for the
-// persistent streaming kernel (forward-first operations, 2^10 <= N <= 2^13).
-// 0 is the default: 16 coefficients per thread, one workgroup per polynomial.
+// Launch shape of the transform kernels, as bit fields (0 everywhere = the
+// default shape):
+//   bits 0-3  log2 of the coefficients per thread of the chunk kernels, 3 or 4
+//             (0: 4), i.e. radix-8 or radix-16 register rounds;
+//   bit 4     kShapeStream: the persistent streaming kernel (forward-first
+//             operations, 2^10 <= N <= 2^13);
+//   bits 5-6  with kShapeStream, log2 of the limb-polynomials each workgroup
+//             walks: 0 sizes the grid to one resident generation, k = 1..3 to
+//             ceil(count / 2^k) workgroups (at most one generation);
+//   bits 8-11 two-pass schedule (N >= 2^14): log2 of the chunk the chunk pass
+//             holds in LDS, 11..13 (0: 12), with N / chunk <= 32.
 constexpr int kShapeStream = 16;
-inline bool valid_shape(int shape) { return shape == 0 || shape == 3 || shape == 4 || shape == 19 || shape == 20; }
+constexpr int kShapeLpbShift = 5;
+constexpr int kShapeTileShift = 8;
+constexpr int kShapeMask = 0xF7F;
+inline int shape_loge(int shape) { return (shape & 15) ? (shape & 15) : 4; }
+inline int shape_lpb_log(int shape) { return (shape >> kShapeLpbShift) & 3; }
+inline int shape_tile_log(int shape) { return ((shape >> kShapeTileShift) & 15) ? ((shape >> kShapeTileShift) & 15) : 12; }
+inline bool valid_shape(int shape) {
+    const int e = shape & 15, tile = (shape >> kShapeTileShift) & 15;
+    if (shape < 0 || (shape & ~kShapeMask) != 0) return false;
+    if (e != 0 && e != 3 && e != 4) return false;
+    if (shape_lpb_log(shape) != 0 && !(shape & kShapeStream)) return false;
+    return tile == 0 || (tile >= 11 && tile <= 13);
+}
+// A shape's fields that apply at ring degree 2^log_n are within range there:
+// a two-pass chunk only for N >= 2^14, and at most 32 columns per chunk.
+inline bool valid_shape_for(int shape, uint32_t log_n) {
+    if (!valid_shape(shape)) return false;
+    const int tile = (shape >> kShapeTileShift) & 15;
+    if (tile == 0) return true;
+    return log_n > kMaxSinglePassLogN && (int)log_n - tile >= 1 && (int)log_n - tile <= 5;
+}
+// Every shape the autotuner tries at ring degree 2^log_n for an operation;
+// returns how many it wrote to out (at most max).
+int tuning_candidates(uint32_t log_n, Op op, int *out, int max);
 
 // arith: kArithF64 / kArithCanonInt bits.  F64Arith where it applies
 // (kArithF64, canonical outputs, 2^6 <= N <= 2^13), else IntCanonArith for

@@ -367,7 +367,7 @@ def test_launch_shapes_agree(log_n, bits):
     opv = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in moduli])
     op = pa.Operand(tab, opv)
     try:
-        for shape in tuning.SHAPES:
+        for shape in tuning.shapes(log_n, "fwd_mul_inv"):
             for name in ("forward", "inverse", "fwd_mul_inv", "negacyclic_mul", "forward_lazy"):
                 tuning.set_shape(log_n, name, arith, shape)
                 tuning.set_shape(log_n, name, "int", shape)
@@ -400,7 +400,7 @@ def test_launch_shapes_agree(log_n, bits):
             pa.synchronize()
             np.testing.assert_array_equal(d.download((count, n)), a)
         shape, ms = tuning.autotune(tab, "fwd_mul_inv", 64, reps=3)
-        assert shape in tuning.SHAPES and ms > 0
+        assert shape in tuning.shapes(log_n, "fwd_mul_inv") and ms > 0
     finally:
         for name in ("forward", "inverse", "fwd_mul_inv", "negacyclic_mul", "forward_lazy"):
             tuning.set_shape(log_n, name, "f64", 0)
@@ -430,7 +430,7 @@ def test_streaming_shapes_long_batch(log_n, bits):
     d = pa.DeviceBuffer.from_array(a)
     print("streaming", log_n, bits, count, flush=True)
     try:
-        for shape in (3 | tuning.STREAM, 4 | tuning.STREAM):
+        for shape in [e | tuning.STREAM | tuning.lpb(k) for e in (3, 4) for k in range(4)]:
             for name in ("forward", "fwd_mul_inv", "forward_lazy"):
                 tuning.set_shape(log_n, name, tab.arith(), shape)
                 tuning.set_shape(log_n, name, "int", shape)
@@ -454,6 +454,64 @@ def test_streaming_shapes_long_batch(log_n, bits):
     finally:
         for name in ("forward", "fwd_mul_inv", "forward_lazy"):
             tuning.set_shape(log_n, name, tab.arith(), 0)
+            tuning.set_shape(log_n, name, "int", 0)
+
+
+@pytest.mark.parametrize("log_n", [14, 15, 16, 17])
+def test_two_pass_tiles_agree(log_n):
+    """Every two-pass shape the autotuner tries (LDS tile of 2^11, 2^12 or
+    2^13 coefficients, 8 or 16 per thread in the chunk pass) gives the
+    oracle's outputs: lazy and canonical forward, inverse, fused forward ->
+    multiply -> inverse and the negacyclic product."""
+    from primus_amd import tuning
+
+    n = 1 << log_n
+    L, count = 2, 3
+    moduli = pa.ntt_primes_below(59, 2 * n, L)
+    tab = pa.UintCrtNttTable.new(log_n, moduli)
+    tables = [O.Table(log_n, q) for q in moduli]
+    rng = np.random.default_rng(2000 + log_n)
+    a = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    b = np.stack([rng.integers(0, moduli[i % L], size=n, dtype=np.uint64) for i in range(count)])
+    opv = np.stack([rng.integers(0, q, size=n, dtype=np.uint64) for q in moduli])
+    op = pa.Operand(tab, opv)
+    fwd = [tables[i % L].transform(a[i]) for i in range(count)]
+    lazy = [tables[i % L].lazy_transform(a[i]) for i in range(count)]
+    fmi = [tables[i % L].inverse_transform(O.pointwise_mul(fwd[i], opv[i % L], moduli[i % L])) for i in range(count)]
+    prod = [tables[i % L].inverse_transform(O.pointwise_mul(fwd[i], tables[i % L].transform(b[i]), moduli[i % L]))
+            for i in range(count)]
+    shapes = tuning.shapes(log_n, "fwd_mul_inv")
+    assert len(shapes) == (6 if log_n < 17 else 4)
+    d, db = pa.DeviceBuffer.from_array(a), pa.DeviceBuffer.from_array(b)
+    names = ("forward", "inverse", "fwd_mul_inv", "negacyclic_mul", "forward_lazy")
+    try:
+        for shape in shapes:
+            print("two-pass", log_n, "shape", shape, flush=True)
+            for name in names:
+                tuning.set_shape(log_n, name, tab.arith(), shape)
+                tuning.set_shape(log_n, name, "int", shape)
+            for kind, want in (("forward", fwd), ("forward_lazy", lazy)):
+                d.upload(a)
+                pa.transform_device(tab, d.ptr, count, lazy=(kind == "forward_lazy"))
+                pa.synchronize()
+                got = d.download((count, n))
+                for i in range(count):
+                    np.testing.assert_array_equal(got[i], want[i], err_msg="%s shape %d" % (kind, shape))
+            d.upload(np.stack(fwd))
+            pa.inverse_transform_device(tab, d.ptr, count)
+            pa.synchronize()
+            np.testing.assert_array_equal(d.download((count, n)), a, err_msg="inverse shape %d" % shape)
+            d.upload(a)
+            pa.fwd_mul_inv_device(tab, d.ptr, count, op)
+            pa.synchronize()
+            np.testing.assert_array_equal(d.download((count, n)), np.stack(fmi), err_msg="fused shape %d" % shape)
+            d.upload(a)
+            pa.negacyclic_mul_device(tab, d.ptr, db.ptr, count)
+            pa.synchronize()
+            np.testing.assert_array_equal(d.download((count, n)), np.stack(prod), err_msg="product shape %d" % shape)
+    finally:
+        for name in names:
+            tuning.set_shape(log_n, name, "f64", 0)
             tuning.set_shape(log_n, name, "int", 0)
 
 
