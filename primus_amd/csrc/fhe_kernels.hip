@@ -37,6 +37,22 @@ __device__ __forceinline__ uint64_t gadget_digit(uint64_t adj, uint32_t d, uint3
     return digit < g.half ? digit : g.q_minus_basis + digit;
 }
 
+// The same digit as a signed value for F64Arith: the digit below half the
+// basis, digit - basis from half up.  It is congruent to the residue
+// gadget_digit gives (q - B + digit), and |value| <= B/2 < q, so the forward
+// transform's exact arithmetic takes it as it is.  `bits` is the coefficient
+// shifted to the digit's window; 32-bit arithmetic, valid for bases up to 2^30
+// (the digit plus its carry stays below 2^31).
+__device__ __forceinline__ double signed_digit_f64(uint32_t bits, uint32_t mask, uint32_t carry_mask, uint32_t half,
+                                                   bool two, uint32_t &carry) {
+    const uint32_t temp = (bits & mask) + carry;
+    carry = (temp & carry_mask) != 0;
+    const uint32_t digit = temp & mask;
+    const int32_t v = (two || digit < half) ? (int32_t)digit : (int32_t)digit - (int32_t)(mask + 1u);
+    return (double)v;
+}
+constexpr uint32_t kSignedDigitMaxLogBasis = 30;
+
 // acc += x * key (mod q), one term of DcrtPolynomial::add_mul_assign
 // (primus_poly/src/dcrt/mod.rs:94-114); canonical for IntArith, centred and
 // reduced every third term for F64Arith (|acc| <= q/2 + 1 + 3 * 0.744 q).
@@ -96,8 +112,22 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
             }
         for (uint32_t d = 0; d < g.l; ++d) {
             T x[G::E];
+            if constexpr (std::is_same<A, F64Arith>::value) {
+                if (g.log_basis <= kSignedDigitMaxLogBasis) {
+                    const uint32_t shift = g.drop_bits + d * g.log_basis;
 #pragma unroll
-            for (int i = 0; i < G::E; ++i) x[i] = A::load_fwd(gadget_digit(adj[i], d, carry[i], g), tab);
+                    for (int i = 0; i < G::E; ++i)
+                        x[i] = signed_digit_f64((uint32_t)(adj[i] >> shift), (uint32_t)g.basis_minus_one,
+                                                (uint32_t)g.carry_mask, (uint32_t)g.half, g.basis_is_two != 0,
+                                                carry[i]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < G::E; ++i) x[i] = A::load_fwd(gadget_digit(adj[i], d, carry[i], g), tab);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < G::E; ++i) x[i] = A::load_fwd(gadget_digit(adj[i], d, carry[i], g), tab);
+            }
             // The twiddles are the same for every digit; without this the
             // compiler hoists all their loads out of the digit loop and keeps
             // them live in registers for the whole kernel.
@@ -505,9 +535,23 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 3) void ext_rns_fused_kerne
             }
         for (uint32_t d = 0; d < g.l; ++d) {
             T x[G::E];
+            if constexpr (std::is_same<A, F64Arith>::value) {
+                if (g.log_basis <= kSignedDigitMaxLogBasis) {
+                    const uint32_t shift = first + d * g.log_basis;
 #pragma unroll
-            for (int e = 0; e < G::E; ++e)
-                x[e] = A::load_fwd(ext_rns_digit(g, win[e], first, d, carry[e], tab.q), tab);
+                    for (int e = 0; e < G::E; ++e)
+                        x[e] = signed_digit_f64((uint32_t)(win[e] >> shift), (uint32_t)g.basis_minus_one,
+                                                (uint32_t)g.carry_mask, (uint32_t)g.half, g.basis == 2, carry[e]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < G::E; ++e)
+                        x[e] = A::load_fwd(ext_rns_digit(g, win[e], first, d, carry[e], tab.q), tab);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < G::E; ++e)
+                    x[e] = A::load_fwd(ext_rns_digit(g, win[e], first, d, carry[e], tab.q), tab);
+            }
             // as in ext_product_kernel: keep the twiddle pointers and slot
             // addresses from being hoisted out of the digit loop
             DevTable td = tab;
