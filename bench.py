@@ -612,8 +612,9 @@ def main():
                     help="requests in ordinary (pageable) host memory, staged by the server")
     ap.add_argument("--serving-saturation-requests", type=int, default=20000)
     ap.add_argument("--serving-leg", action="store_true",
-                    help="the bounded serving pass of the default run: 1k and 10k req/s and saturation, batched "
-                         "and plain on equal request counts (1000, 5000, 5000), a pool of 96 requests")
+                    help="the bounded serving pass of the default run: 1k and 10k req/s for 2 s each and "
+                         "saturation, batched and plain on equal request counts (2000, 20000, 20000), a pool "
+                         "of 96 requests")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving pass of the default run")
     ap.add_argument("--secondary-only", choices=sorted(SECONDARY), default=None,
                     help="run one secondary workload alone and print its JSON (for kernel profiles)")
@@ -626,10 +627,10 @@ def main():
     if args.serving_leg:
         args.serving = True
         args.serving_rates = [1000.0, 10000.0]
-        args.serving_seconds = 1.0  # 1000 requests at 1k req/s, 5000 at 10k
-        args.serving_requests = 5000
-        args.serving_plain_requests = 5000
-        args.serving_saturation_requests = 5000
+        args.serving_seconds = 2.0  # 2000 requests at 1k req/s, 20000 at 10k
+        args.serving_requests = 20000
+        args.serving_plain_requests = 20000
+        args.serving_saturation_requests = 20000
         args.serving_pool = 96
     if args.serving:
         # one hardware queue per server stream (slots, input copies, result
