@@ -548,7 +548,7 @@ def serving_main(args):
         b = run(batched, rate, count, seed=11 + k)
         p = run(plain, rate, int(min(count, args.serving_plain_requests)), seed=11 + k)
         out["rates"].append({"batched": b, "plain": p, "p99_ratio_plain_over_batched": p["p99_ms"] / b["p99_ms"]})
-        print(json.dumps({"progress": "rate done", "rate": rate}), file=sys.stderr, flush=True)
+        print("bench: serving rate %g req/s done" % rate, file=sys.stderr, flush=True)
     # saturation: everything offered at once, the highest throughput each path reaches
     count = int(min(args.serving_requests, args.serving_saturation_requests))
     b = run(batched, 1e7, count, seed=99)
@@ -567,7 +567,7 @@ def serving_leg():
     import subprocess
 
     cmd = [sys.executable, os.path.abspath(__file__), "--serving-leg"]
-    print(json.dumps({"progress": "serving pass starts"}), file=sys.stderr, flush=True)
+    print("bench: serving pass starts", file=sys.stderr, flush=True)
     try:
         res = subprocess.run(cmd, capture_output=True, text=True, timeout=400)
     except subprocess.TimeoutExpired:

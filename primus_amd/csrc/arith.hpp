@@ -29,17 +29,45 @@ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __u
 // middle products (together below 3 * 2^64, so the floor loses at most 2).
 // Two multiply-highs and one multiply-add instead of four multiplies and
 // their carries.  The sum stays below 2^64: (2^32-1)^2 + 2 (2^32-1) < 2^64.
-__device__ __forceinline__ uint64_t mulhi64_approx(uint64_t a, uint64_t b) {
+__host__ __device__ __forceinline__ uint64_t mulhi64_approx(uint64_t a, uint64_t b) {
     const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-    const uint32_t h1 = __umulhi(a1, b0), h2 = __umulhi(a0, b1);
+    const uint32_t h1 = (uint32_t)(((uint64_t)a1 * b0) >> 32), h2 = (uint32_t)(((uint64_t)a0 * b1) >> 32);
     return (uint64_t)a1 * b1 + h1 + h2;
 }
 
-__device__ __forceinline__ uint64_t sub_if_ge(uint64_t x, uint64_t m) { return x >= m ? x - m : x; }
+// x >= m ? x - m : x.  The subtraction's own borrow selects (sub, sub-with-
+// borrow, two selects) where a 64-bit comparison would add a fifth
+// instruction.
+__host__ __device__ __forceinline__ uint64_t sub_if_ge(uint64_t x, uint64_t m) {
+    unsigned c0, c1;
+    const unsigned d0 = __builtin_subc((unsigned)x, (unsigned)m, 0u, &c0);
+    const unsigned d1 = __builtin_subc((unsigned)(x >> 32), (unsigned)(m >> 32), c0, &c1);
+    return c1 ? x : (((uint64_t)d1 << 32) | d0);
+}
+
+// w*b - k*q (mod 2^64), given nq = 2^64 - q: w*b + k*nq, whose two low
+// products chain through one 64-bit multiply-add (w0*b0, then k0*nq0 added)
+// and whose four cross products only reach the high word.  Two multiply-adds,
+// four multiply-lows and two adds, against two full 64-bit products and a
+// 64-bit subtraction.
+__host__ __device__ __forceinline__ uint64_t mul_sub_lo(uint64_t w, uint64_t b, uint64_t k, uint64_t nq) {
+    const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint32_t k0 = (uint32_t)k, k1 = (uint32_t)(k >> 32), n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
+    const uint32_t cross = w1 * b0 + w0 * b1 + k1 * n0 + k0 * n1;
+    const uint64_t low = (uint64_t)k0 * n0 + (uint64_t)w0 * b0;
+    return low + ((uint64_t)cross << 32);
+}
+
+// x - k*q (mod 2^64), given nq = 2^64 - q: the low product k0*nq0 as a
+// multiply-add onto x, the two cross products into the high word.
+__host__ __device__ __forceinline__ uint64_t sub_mul_lo(uint64_t x, uint64_t k, uint64_t nq) {
+    const uint32_t k0 = (uint32_t)k, k1 = (uint32_t)(k >> 32), n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
+    return (uint64_t)k0 * n0 + x + ((uint64_t)(k1 * n0 + k0 * n1) << 32);
+}
 
 // w*b - floor(wq*b / 2^64)*q (mod 2^64), in [0, 2q); wq = floor(w*2^64/q).
 __device__ __forceinline__ uint64_t shoup_lazy(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
-    return w * b - mulhi64(wq, b) * q;
+    return mul_sub_lo(w, b, mulhi64(wq, b), 0 - q);
 }
 
 __device__ __forceinline__ uint64_t shoup_full(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
@@ -49,8 +77,8 @@ __device__ __forceinline__ uint64_t shoup_full(uint64_t w, uint64_t wq, uint64_t
 // shoup_lazy with the approximate quotient: the quotient is at most two below
 // Shoup's, so the result is congruent to w*b and lies in [0, 4q) for any
 // 64-bit b (q < 2^62).  Used only where no caller sees lazy values.
-__device__ __forceinline__ uint64_t shoup_approx(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
-    return w * b - mulhi64_approx(wq, b) * q;
+__host__ __device__ __forceinline__ uint64_t shoup_approx(uint64_t w, uint64_t wq, uint64_t b, uint64_t q) {
+    return mul_sub_lo(w, b, mulhi64_approx(wq, b), 0 - q);
 }
 
 // Canonical (a*b + c) mod q by the 2-limb Barrett quotient with ratio

@@ -669,7 +669,7 @@ __device__ __forceinline__ uint64_t mac59(uint64_t a, uint64_t s, uint64_t x, co
     const uint32_t sh = t.bar_sh;  // b - 1, 1 <= sh <= 58
     const uint64_t tq = (lo >> sh) | (hi << (64 - sh));
     const uint64_t k = mulhi64(tq, t.bar_mu);
-    const uint64_t r = lo - k * t.q;
+    const uint64_t r = sub_mul_lo(lo, k, 0 - t.q);
     return sub_if_ge(sub_if_ge(r, t.q2), t.q);
 }
 
