@@ -322,7 +322,8 @@ typedef struct ph_ext_key_rns ph_ext_key_rns;
 int ph_ext_key_rns_new(const ph_dcrt_table *table, uint32_t log_basis, uint32_t decompose_length,
                        const uint64_t *key_ntt, ph_ext_key_rns **out);
 void ph_ext_key_rns_free(ph_ext_key_rns *key);
-/* d_in [count][2][L][N] residues, d_out [count][2][L][N]: per input polynomial
+/* d_in [count][2][L][N] residues, each below its limb's modulus (as the
+ * reference's DCRT ciphertexts hold them), d_out [count][2][L][N]: per input polynomial
  * each coefficient's residues composed into one integer below Q
  * (primus_rns/src/base.rs:396-450), its gadget digits, each digit wrapped
  * into every limb (base.rs:201-236), transformed, and multiply-accumulated

@@ -1119,7 +1119,7 @@ struct ph_ext_key_rns {
     // (slot t*E + j at j*(N/E) + t), when it applies
     bool fused = false;
     int arith = 0;
-    uint32_t words = 0;
+    ExtRnsGarner garner{};  // the fused kernel's composition constants
     Shoup *d_key_perm = nullptr;
     ShoupF *d_key_perm_f = nullptr;
 };
@@ -1248,7 +1248,20 @@ int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t deco
                  (uint64_t)decompose_length * log_basis + (g.drop_bits > 0 ? 1 : 0) <= 64;
     if (key->fused) {
         key->arith = arith_of(d);
-        key->words = (bits + 63) / 64;
+        // Garner's mixed-radix constants over the limbs in table order
+        ExtRnsGarner &gk = key->garner;
+        for (uint32_t i = 1; i < d->L; ++i) {
+            const uint64_t qi = d->tables[i]->q;
+            uint64_t prefix = 1;  // q_0 ... q_{i-1} mod q_i
+            for (uint32_t j = 0; j < i; ++j) prefix = (uint64_t)((u128)prefix * (d->tables[j]->q % qi) % qi);
+            gk.c[i] = inv_mod(prefix, qi);
+            gk.c_q[i] = h_shoup_quotient(gk.c[i], qi);
+            gk.off[i] = qi * (((1ull << 63) + qi - 1) / qi);
+            for (uint32_t j = 0; j + 1 < i; ++j) {
+                gk.qm[i][j] = d->tables[j]->q % qi;
+                gk.qm_q[i][j] = h_shoup_quotient(gk.qm[i][j], qi);
+            }
+        }
         const uint64_t E = 1ull << kExtLogE, T = n / E, rows = 4ull * decompose_length * d->L;
         std::vector<Shoup> hp(words);
         std::vector<ShoupF> hpf;
@@ -1299,7 +1312,8 @@ int ph_ext_product_rns_device(const ph_ext_key_rns *key, const uint64_t *d_in, u
         f.key_f = key->d_key_perm_f;
         f.tabs = d->d_tabs;
         f.ntt_output = ntt_output ? 1u : 0u;
-        hipError_t e = launch_ext_rns_fused(f, key->arith, key->words, s);
+        f.garner = key->garner;
+        hipError_t e = launch_ext_rns_fused(f, key->arith, s);
         return e == hipSuccess ? PH_OK : hip_fail(e, "fused external product kernel");
     }
     const uint64_t n = 1ull << d->log_n, L = d->L, l = key->proto.l;

@@ -394,83 +394,79 @@ __device__ __forceinline__ gptr<ShoupF> ext_rns_key(const ExtRnsFusedArgs &a, co
     return as_global(a.key_f);
 }
 
-// One coefficient of one input polynomial composed into W words (the value
-// below Q, compose_inplace, primus_rns/src/base.rs:396-420) and shifted by
-// the split adjustment (init_value_carry, primus_decompose/src/big_integer/
-// basis.rs:307-331).  Returns its digit window: the 64 bits from bit
-// drop_bits - 1 (the carry into the first digit, when drop_bits > 0) or from
-// bit 0, which hold every digit when l * log_basis + 1 <= 64.  src: the
-// coefficient in limb 0, limbs N words apart.
-template <int W>
-__device__ __forceinline__ uint64_t ext_rns_window(const ExtRnsArgs &g, const uint64_t *src, uint64_t n) {
-    uint64_t v[W];
+// Borrow out of the L-word subtraction x - y (y uniform): 0 when x >= y.
+// 32-bit halves through the hardware borrow chain; the differences go unused.
+template <int L>
+__device__ __forceinline__ unsigned words_borrow(const uint64_t (&x)[L], const uint64_t *y) {
+    unsigned c = 0;
 #pragma unroll
-    for (int w = 0; w < W; ++w) v[w] = 0;
-    for (uint32_t i = 0; i < g.L; ++i) {
-        const uint64_t prod = shoup_full(g.inv[i], g.inv_q[i], src[(uint64_t)i * n], g.q[i]);
-        uint64_t carry = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            // Q/q_i has at most punct_words nonzero words (uniform): the
-            // words above them only take the carry
-            const bool mul = (uint32_t)w < g.punct_words;
-            const uint64_t lo = mul ? prod * g.punct[i][w] : 0, hi = mul ? mulhi64(prod, g.punct[i][w]) : 0;
-            const uint64_t s1 = v[w] + lo;
-            uint64_t c = s1 < lo;
-            const uint64_t s2 = s1 + carry;
-            c += s2 < carry;
-            v[w] = s2;
-            carry = hi + c;
-        }
-        // v >= Q (or a carry out of the W words): subtract Q
-        bool ge = carry != 0;
-        if (!ge) {
-            ge = true;
-#pragma unroll
-            for (int w = W - 1; w >= 0; --w)
-                if (v[w] != g.Q[w]) {
-                    ge = v[w] > g.Q[w];
-                    break;
-                }
-        }
-        if (ge) {
-            uint64_t borrow = 0;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                const uint64_t d = v[w] - g.Q[w];
-                const uint64_t b1 = v[w] < g.Q[w];
-                v[w] = d - borrow;
-                borrow = b1 | (d < borrow);
-            }
-        }
+    for (int w = 0; w < L; ++w) {
+        (void)__builtin_subc((unsigned)x[w], (unsigned)y[w], c, &c);
+        (void)__builtin_subc((unsigned)(x[w] >> 32), (unsigned)(y[w] >> 32), c, &c);
     }
-    if (g.has_split) {
-        bool ge = true;
+    return c;
+}
+
+// One coefficient of one input polynomial composed into L words: the CRT
+// value below Q of residues r_i < q_i (what compose_inplace,
+// primus_rns/src/base.rs:396-420, sums) in Garner's mixed-radix form, one
+// Shoup product per limb above the first plus the Horner steps of the
+// partial value; then shifted by the split adjustment (init_value_carry,
+// primus_decompose/src/big_integer/basis.rs:307-331: from split up the value
+// stands for value - Q, taken mod 2^bits).  Returns its digit window: the 64
+// bits from bit drop_bits - 1 (the carry into the first digit, when
+// drop_bits > 0) or from bit 0, which hold every digit when
+// l * log_basis + 1 <= 64.  src: the coefficient in limb 0, limbs N words
+// apart.
+template <int L>
+__device__ __forceinline__ uint64_t ext_rns_window(const ExtRnsFusedArgs &a, const uint64_t *src, uint64_t n) {
+    const ExtRnsArgs &g = a.g;
+    const ExtRnsGarner &k = a.garner;
+    uint64_t v[L];
+    v[0] = src[0];
 #pragma unroll
-        for (int w = W - 1; w >= 0; --w)
-            if (v[w] != g.split[w]) {
-                ge = v[w] > g.split[w];
-                break;
-            }
-        if (ge) {
-            uint64_t carry = 0;
+    for (int i = 1; i < L; ++i) {
+        // (v_0 + v_1 q_0 + ... + v_{i-1} q_0..q_{i-2}) mod q_i, below q_i + q_0
+        uint64_t part = v[i - 1];
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                const uint64_t s1 = v[w] + g.pow2_sub[w];
-                const uint64_t c1 = s1 < g.pow2_sub[w];
-                v[w] = s1 + carry;
-                carry = c1 | (v[w] < carry);
-            }
+        for (int j = i - 2; j >= 0; --j) part = shoup_full(k.qm[i][j], k.qm_q[i][j], part, g.q[i]) + v[j];
+        v[i] = shoup_full(k.c[i], k.c_q[i], src[(uint64_t)i * n] + k.off[i] - part, g.q[i]);
+    }
+    // x = v_{L-1}; x = x q_j + v_j for j = L-2 .. 0 (x grows by one word a step)
+    uint64_t x[L];
+    x[0] = v[L - 1];
+#pragma unroll
+    for (int w = 1; w < L; ++w) x[w] = 0;
+#pragma unroll
+    for (int j = L - 2; j >= 0; --j) {
+        uint64_t carry = v[j];
+#pragma unroll
+        for (int w = 0; w < L - 1 - j; ++w) {
+            const uint64_t lo = x[w] * g.q[j] + carry;
+            carry = mulhi64(x[w], g.q[j]) + (lo < carry);
+            x[w] = lo;
+        }
+        x[L - 1 - j] = carry;
+    }
+    if (g.has_split && words_borrow<L>(x, g.split) == 0) {
+        // x - Q mod 2^(64 L): the same bits below bit `bits` as x + pow2_sub
+        unsigned c = 0;
+#pragma unroll
+        for (int w = 0; w < L; ++w) {
+            const unsigned lo = __builtin_subc((unsigned)x[w], (unsigned)g.Q[w], c, &c);
+            const unsigned hi = __builtin_subc((unsigned)(x[w] >> 32), (unsigned)(g.Q[w] >> 32), c, &c);
+            x[w] = ((uint64_t)hi << 32) | lo;
         }
     }
     const uint32_t ws = g.drop_bits > 0 ? g.drop_bits - 1 : 0;
     const uint32_t wi = ws >> 6, off = ws & 63;
-    uint64_t lo = v[0], hi = 0;
+    uint64_t lo = x[0], hi = L > 1 ? x[1] : 0;
 #pragma unroll
-    for (int w = 1; w < W; ++w) {
-        if (wi == (uint32_t)w) lo = v[w];
-        if (wi + 1 == (uint32_t)w) hi = v[w];
-    }
+    for (int w = 1; w < L; ++w)
+        if (wi == (uint32_t)w) {
+            lo = x[w];
+            hi = w + 1 < L ? x[(w + 1) % L] : 0;
+        }
     return off ? (lo >> off) | (hi << (64 - off)) : lo;
 }
 
@@ -495,7 +491,7 @@ __device__ __forceinline__ uint64_t ext_rns_digit(const ExtRnsArgs &g, uint64_t 
 // Requires l * log_basis + (drop_bits > 0) <= 64 (the digits fit one window).
 // Three waves per SIMD: the two accumulators, the windows, a transform and
 // the key batch need about 140 VGPRs, and 128 spills.
-template <class A, int LOGN, int W>
+template <class A, int LOGN, int L>
 __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 3) void ext_rns_fused_kernel(ExtRnsFusedArgs a) {
     constexpr int LOGT = LOGN - kExtLogE;
     using G = Geo<LOGN, LOGT>;
@@ -530,7 +526,7 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 3) void ext_rns_fused_kerne
 #pragma unroll
             for (int j = 0; j < (1 << R0); ++j) {
                 const int e = (m << R0) + j;
-                win[e] = ext_rns_window<W>(g, src + fwd_slot<LOGN, LOGT, 0, R0>(t, m, j), n);
+                win[e] = ext_rns_window<L>(a, src + fwd_slot<LOGN, LOGT, 0, R0>(t, m, j), n);
                 carry[e] = (uint32_t)(win[e] & first);
             }
         for (uint32_t d = 0; d < g.l; ++d) {
@@ -603,14 +599,13 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 3) void ext_rns_fused_kerne
 }
 
 template <class A, int LOGN>
-static hipError_t launch_ext_rns_fused_n(const ExtRnsFusedArgs &a, uint32_t words, hipStream_t s) {
+static hipError_t launch_ext_rns_fused_n(const ExtRnsFusedArgs &a, hipStream_t s) {
     const uint64_t groups = (a.g.count + 7) / 8;
     const dim3 grid((uint32_t)(groups * 8 * a.g.L)), block(1u << (LOGN - kExtLogE));
-    // three-word moduli products run on four words (the fourth is zero)
-    switch (words) {
+    switch (a.g.L) {
         case 1: ext_rns_fused_kernel<A, LOGN, 1><<<grid, block, 0, s>>>(a); break;
         case 2: ext_rns_fused_kernel<A, LOGN, 2><<<grid, block, 0, s>>>(a); break;
-        case 3:
+        case 3: ext_rns_fused_kernel<A, LOGN, 3><<<grid, block, 0, s>>>(a); break;
         case 4: ext_rns_fused_kernel<A, LOGN, 4><<<grid, block, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;
     }
@@ -618,22 +613,22 @@ static hipError_t launch_ext_rns_fused_n(const ExtRnsFusedArgs &a, uint32_t word
 }
 
 template <int LOGN>
-static hipError_t launch_ext_rns_fused_a(const ExtRnsFusedArgs &a, int arith, uint32_t words, hipStream_t s) {
-    if (arith & kArithF64) return launch_ext_rns_fused_n<F64Arith, LOGN>(a, words, s);
-    if (arith & kArithCanonInt) return launch_ext_rns_fused_n<IntCanonArith, LOGN>(a, words, s);
-    return launch_ext_rns_fused_n<IntArith, LOGN>(a, words, s);
+static hipError_t launch_ext_rns_fused_a(const ExtRnsFusedArgs &a, int arith, hipStream_t s) {
+    if (arith & kArithF64) return launch_ext_rns_fused_n<F64Arith, LOGN>(a, s);
+    if (arith & kArithCanonInt) return launch_ext_rns_fused_n<IntCanonArith, LOGN>(a, s);
+    return launch_ext_rns_fused_n<IntArith, LOGN>(a, s);
 }
 
-hipError_t launch_ext_rns_fused(const ExtRnsFusedArgs &a, int arith, uint32_t words, hipStream_t s) {
+hipError_t launch_ext_rns_fused(const ExtRnsFusedArgs &a, int arith, hipStream_t s) {
     if (a.g.count == 0) return hipSuccess;
     switch (a.g.log_n) {
-        case 6: return launch_ext_rns_fused_a<6>(a, arith, words, s);
-        case 7: return launch_ext_rns_fused_a<7>(a, arith, words, s);
-        case 8: return launch_ext_rns_fused_a<8>(a, arith, words, s);
-        case 9: return launch_ext_rns_fused_a<9>(a, arith, words, s);
-        case 10: return launch_ext_rns_fused_a<10>(a, arith, words, s);
-        case 11: return launch_ext_rns_fused_a<11>(a, arith, words, s);
-        case 12: return launch_ext_rns_fused_a<12>(a, arith, words, s);
+        case 6: return launch_ext_rns_fused_a<6>(a, arith, s);
+        case 7: return launch_ext_rns_fused_a<7>(a, arith, s);
+        case 8: return launch_ext_rns_fused_a<8>(a, arith, s);
+        case 9: return launch_ext_rns_fused_a<9>(a, arith, s);
+        case 10: return launch_ext_rns_fused_a<10>(a, arith, s);
+        case 11: return launch_ext_rns_fused_a<11>(a, arith, s);
+        case 12: return launch_ext_rns_fused_a<12>(a, arith, s);
     }
     return hipErrorInvalidValue;
 }

@@ -74,8 +74,27 @@ hipError_t launch_ext_rns_mac(const ExtRnsMacArgs &a, hipStream_t s);
 // for its limb, multiply-accumulates them with the key and inverse-transforms
 // the two accumulators, with no intermediate in device memory.  Ring degrees
 // kExtMinLogN .. kExtMaxLogN; E = 2^kExtLogE coefficients per thread.
+//
+// Its composition is Garner's mixed-radix form of the CRT value, which for
+// residues r_i < q_i is the same integer below Q that compose_inplace
+// (primus_rns/src/base.rs:396-420) sums:
+//   x = v_0 + v_1 q_0 + v_2 q_0 q_1 + ...,  v_0 = r_0,
+//   v_i = (r_i - (v_0 + v_1 q_0 + ... + v_{i-1} q_0..q_{i-2})) (q_0..q_{i-1})^-1 mod q_i,
+// each v_i below q_i, so x < Q with no reduction by Q.
+struct ExtRnsGarner {
+    uint64_t c[kExtRnsMaxLimbs], c_q[kExtRnsMaxLimbs];  // (q_0 ... q_{i-1})^-1 mod q_i, Shoup pair (i >= 1)
+    // q_i ceil(2^63 / q_i): added before the partial value (below 2^63) is
+    // subtracted, so the Shoup operand r_i + off_i - s stays a 64-bit value
+    // congruent to r_i - s
+    uint64_t off[kExtRnsMaxLimbs];
+    // q_j mod q_i with its Shoup quotient, for the partial value's Horner
+    // steps mod q_i (j <= i - 2)
+    uint64_t qm[kExtRnsMaxLimbs][kExtRnsMaxLimbs], qm_q[kExtRnsMaxLimbs][kExtRnsMaxLimbs];
+};
+
 struct ExtRnsFusedArgs {
     ExtRnsArgs g;              // gadget constants and the input (g.digits unused)
+    ExtRnsGarner garner;
     uint64_t *out;             // [count][2][L][N]
     const Shoup *key;          // [2][l][2][L] rows of N, slot t*E + j at j*(N/E) + t (integer arithmetic)
     const ShoupF *key_f;       // the same as centred binary64 pairs (every limb below 2^50)
@@ -84,9 +103,10 @@ struct ExtRnsFusedArgs {
     uint32_t pad_;
 };
 
-// arith: kArithF64 / kArithCanonInt bits (ntt_launch.hpp) of the limbs; words:
-// 64-bit words of Q (1..kExtRnsWords).
-hipError_t launch_ext_rns_fused(const ExtRnsFusedArgs &a, int arith, uint32_t words, hipStream_t s);
+// arith: kArithF64 / kArithCanonInt bits (ntt_launch.hpp) of the limbs; the
+// kernel is instantiated per limb count a.g.L (1..kExtRnsMaxLimbs), Q held in
+// L words.  Input residues must lie below their limb's modulus.
+hipError_t launch_ext_rns_fused(const ExtRnsFusedArgs &a, int arith, hipStream_t s);
 
 // RLWE encryption: ring degrees 2^10 .. 2^16 in one persistent launch.
 constexpr uint32_t kRlweMinLogN = 10;
