@@ -489,10 +489,15 @@ __device__ __forceinline__ uint64_t ext_rns_digit(const ExtRnsArgs &g, uint64_t 
 // turn) and start close together, so the L - 1 re-reads of its input rows are
 // served by that XCD's L2.
 // Requires l * log_basis + (drop_bits > 0) <= 64 (the digits fit one window).
-// Three waves per SIMD: the two accumulators, the windows, a transform and
-// the key batch need about 140 VGPRs, and 128 spills.
+// Four waves per SIMD (128 VGPRs: the two accumulators, the windows, a
+// transform and the key batch); integer arithmetic above N = 2^10 needs more
+// and runs three.
+template <class A>
+constexpr int ext_rns_waves(int log_n) {
+    return std::is_same<A, F64Arith>::value || log_n <= 10 ? 4 : 3;
+}
 template <class A, int LOGN, int L>
-__global__ __launch_bounds__(1 << (LOGN - kExtLogE), 3) void ext_rns_fused_kernel(ExtRnsFusedArgs a) {
+__global__ __launch_bounds__(1 << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) void ext_rns_fused_kernel(ExtRnsFusedArgs a) {
     constexpr int LOGT = LOGN - kExtLogE;
     using G = Geo<LOGN, LOGT>;
     using T = typename A::T;
