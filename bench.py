@@ -332,12 +332,18 @@ def secondary_ext_product_rns(pa, timer, sp):
 
         per, tot = timer.run(step, 200, 10)
         ms = tot / 200
+        try:  # the fused launch allocates nothing: replayed from a HIP graph of rot batches
+            graph_ms = timer.run_graph(step, rot, 200, 8)
+        except Exception:  # capture unsupported: stream launches only
+            graph_ms = None
+        best = min(ms, graph_ms) if graph_ms else ms
         out["%d_bit" % bits] = {
             "moduli": moduli, "arith": table.arith() if bits == 50 else "integer (approximate-quotient Shoup)",
-            "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
-            "ntt_per_s": (2 * l + 2) * L * count / (ms * 1e-3),
-            "algorithmic_gbps": 4 * L * n * 8 * count / (ms * 1e-3) / 1e9,
-            "hbm_frac": 4 * L * n * 8 * count / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, "rotating_buffers": rot}
+            "ms_per_batch": best, "stream_launch_ms": ms, "graph_replay_ms": graph_ms,
+            "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+            "ntt_per_s": (2 * l + 2) * L * count / (best * 1e-3),
+            "algorithmic_gbps": 4 * L * n * 8 * count / (best * 1e-3) / 1e9,
+            "hbm_frac": 4 * L * n * 8 * count / (best * 1e-3) / 1e9 / HBM_PEAK_GBPS, "rotating_buffers": rot}
     out["workload"] = ("1024 external products over 2 limbs, N=1024, B=2^8, l=3: one fused launch per batch "
                        "(composition, digits, digit transforms, multiply-accumulate, inverse)")
     return out

@@ -134,6 +134,62 @@ def test_gpu_ext_product_rns(L, bits, log_b, l, log_n, count):
     np.testing.assert_array_equal(din.download((count, 2, L, n)), x)
 
 
+def _edge_values(b, n, rng):
+    """n composed values below Q: the ends of [0, Q), the split point and its
+    neighbours, the first limb's modulus (where Garner's first mixed-radix
+    digit wraps), the boundaries of every digit's bit window and of the
+    rounding carry, then uniform values."""
+    Q = b.Q
+    vals = [0, 1, 2, Q - 1, Q - 2, b.moduli[0] - 1, b.moduli[0], b.moduli[0] + 1, Q // 2, Q // 2 + 1]
+    if len(b.moduli) > 2:
+        vals += [b.moduli[0] * b.moduli[1] - 1, b.moduli[0] * b.moduli[1]]
+    if b.split is not None:
+        vals += [b.split - 1, b.split, b.split + 1]
+    for d in range(b.l + 1):
+        edge = 1 << (b.drop_bits + d * b.log_basis)
+        half = edge >> 1
+        vals += [edge - 1, edge, edge + 1, edge + half - 1, edge + half, Q - edge, Q - half]
+    vals = [v % Q for v in vals]
+    while len(vals) < n:
+        vals.append(int.from_bytes(rng.bytes(40), "little") % Q)
+    return vals[:n]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,bits,log_b,l,log_n", [(2, 59, 8, 3, 10), (2, 50, 8, 3, 10), (3, 50, 16, 3, 11),
+                                                  (4, 62, 20, 3, 12), (3, 59, 7, 9, 9), (2, 59, 2, 29, 8)])
+def test_gpu_ext_product_rns_composition_edges(L, bits, log_b, l, log_n):
+    """The fused kernel's composition (Garner's mixed-radix form, then the
+    split adjustment and the digit window) on inputs whose composed values sit
+    on every boundary they can cross, against the oracle's limb-by-limb sum
+    (compose_inplace, primus_rns/src/base.rs:396-420)."""
+    from primus_amd.fhe import ExtKeyRns
+    n = 1 << log_n
+    moduli = pa.ntt_primes_below(bits, 2 * n, L)
+    print("ext rns edges", L, bits, log_b, l, log_n, flush=True)
+    basis = O.RnsBasis(moduli, log_b, l)
+    rng = _rng(77 * L + bits)
+    x = np.zeros((1, 2, L, n), dtype=np.uint64)
+    for p in range(2):
+        vals = _edge_values(basis, n, rng)
+        if p:
+            vals = vals[::-1]
+        for i, q in enumerate(moduli):
+            x[0, p, i] = [v % q for v in vals]
+    assert basis.compose(x[0, 0])[:3] == [0, 1, 2]
+    qcol = np.array(moduli, dtype=np.uint64).reshape(-1, 1)
+    key = np.stack([rng.integers(0, 1 << 63, size=(L, n), dtype=np.uint64) % qcol for _ in range(4 * l)])
+    key = key.reshape(2, l, 2, L, n)
+    k = ExtKeyRns(pa.UintCrtNttTable.new(log_n, moduli), log_b, l, key)
+    din, dout = pa.DeviceBuffer.from_array(x), pa.DeviceBuffer(x.size)
+    tables = [O.Table(log_n, q) for q in moduli]
+    for ntt_output in (True, False):
+        k.product_device(din.ptr, dout.ptr, 1, ntt_output=ntt_output)
+        pa.synchronize()
+        want = O.ext_product_rns(tables, basis, key, x[0], ntt_output=ntt_output)
+        np.testing.assert_array_equal(dout.download((1, 2, L, n))[0], want, err_msg="ntt_output %s" % ntt_output)
+
+
 @pytest.mark.gpu
 def test_gpu_ext_key_rns_rejects():
     from primus_amd.fhe import ExtKeyRns
