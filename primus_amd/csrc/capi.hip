@@ -1021,6 +1021,7 @@ int ph_ext_key_new(const ph_ntt_table *t, uint32_t log_basis, uint32_t decompose
     g.split = (uint64_t)split;
     g.pow2_sub_q = (uint64_t)((((u128)1) << bits) - q);
     g.tab = t->d_tab;
+    g.mac_reduce_every = ext_mac_reduce_interval(q);
 
     // key rows [2][l][2][N] in NTT form, permuted so that slot t*E + j of a row
     // sits at j*(N/E) + t (the register layout of the kernel's last round)
@@ -1248,6 +1249,9 @@ int ph_ext_key_rns_new(const ph_dcrt_table *d, uint32_t log_basis, uint32_t deco
                  (uint64_t)decompose_length * log_basis + (g.drop_bits > 0 ? 1 : 0) <= 64;
     if (key->fused) {
         key->arith = arith_of(d);
+        uint64_t q_max = 0;
+        for (uint32_t i = 0; i < d->L; ++i) q_max = std::max(q_max, d->tables[i]->q);
+        g.mac_reduce_every = ext_mac_reduce_interval(q_max);
         // Garner's mixed-radix constants over the limbs in table order
         ExtRnsGarner &gk = key->garner;
         for (uint32_t i = 1; i < d->L; ++i) {

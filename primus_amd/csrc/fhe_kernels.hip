@@ -54,14 +54,33 @@ __device__ __forceinline__ double signed_digit_f64(uint32_t bits, uint32_t mask,
 constexpr uint32_t kSignedDigitMaxLogBasis = 30;
 
 // acc += x * key (mod q), one term of DcrtPolynomial::add_mul_assign
-// (primus_poly/src/dcrt/mod.rs:94-114); canonical for IntArith, centred and
-// reduced every third term for F64Arith (|acc| <= q/2 + 1 + 3 * 0.744 q).
-__device__ __forceinline__ void ext_mac(uint64_t &acc, uint64_t x, const Shoup &key, const DevTable &t, bool) {
+// (primus_poly/src/dcrt/mod.rs:94-114); canonical for IntArith.  For F64Arith
+// the exact centred products are summed unreduced: the caller reduces every
+// accumulator after mac_reduce_every terms (ext_mac_reduce_interval) and at
+// the end.
+__device__ __forceinline__ void ext_mac(uint64_t &acc, uint64_t x, const Shoup &key, const DevTable &t) {
     acc = add_mod(acc, shoup_full(key.w, key.wq, x, t.q), t.q);
 }
-__device__ __forceinline__ void ext_mac(double &acc, double x, const ShoupF &key, const DevTable &t, bool reduce) {
+__device__ __forceinline__ void ext_mac(double &acc, double x, const ShoupF &key, const DevTable &t) {
     acc += F64Arith::mulmod(x, key.w, key.wp, t.qd);
-    if (reduce) acc = F64Arith::reduce(acc, t);
+}
+// After a digit's multiply-accumulate: binary64 accumulators are reduced
+// (|acc| <= q/2 + 1) once `every` key products have been added since the last
+// reduction; one uniform branch for all of them.  Integer accumulators are
+// canonical already.
+template <class A, int E>
+__device__ __forceinline__ void ext_mac_settle(typename A::T (&acc)[2][E], uint32_t &terms, uint32_t every,
+                                               const DevTable &t) {
+    if constexpr (std::is_same<A, F64Arith>::value) {
+        if (every != 0 && ++terms >= every) {
+            terms = 0;
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                acc[0][j] = F64Arith::reduce(acc[0][j], t);
+                acc[1][j] = F64Arith::reduce(acc[1][j], t);
+            }
+        }
+    }
 }
 __device__ __forceinline__ gptr<Shoup> ext_key(const ExtArgs &g, const IntArith *) { return as_global(g.key); }
 __device__ __forceinline__ gptr<ShoupF> ext_key(const ExtArgs &g, const F64Arith *) { return as_global(g.key_f); }
@@ -95,7 +114,7 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
 #pragma unroll
     for (int j = 0; j < G::E; ++j) acc[0][j] = acc[1][j] = (T)0;
 
-    uint32_t term = 0;
+    uint32_t terms = 0;  // key products since the last reduction (binary64)
     for (uint32_t p = 0; p < 2; ++p) {
         const uint64_t *src = g.in + (prod * 2 + p) * n;
         // init_value_carry_slice_inplace (primus_decompose/src/big_integer/basis.rs:307-331)
@@ -143,7 +162,6 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
             // loads of KB registers' key entries are all issued before their
             // products: left to itself the scheduler waits for each load in
             // turn.
-            const bool red = (++term % 3) == 0;
             using W = typename A::Tw;
             const auto krs = table_rsrc(key + ((uint64_t)(p * g.l + d) * 2) * n);
             constexpr int KB = kExtKeyBatch < G::E ? kExtKeyBatch : G::E;
@@ -160,11 +178,12 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int jj = 0; jj < KB; ++jj) {
-                    ext_mac(acc[0][j0 + jj], x[j0 + jj], k0[jj], tab, red);
-                    ext_mac(acc[1][j0 + jj], x[j0 + jj], k1[jj], tab, red);
+                    ext_mac(acc[0][j0 + jj], x[j0 + jj], k0[jj], tab);
+                    ext_mac(acc[1][j0 + jj], x[j0 + jj], k1[jj], tab);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
+            ext_mac_settle<A, G::E>(acc, terms, g.mac_reduce_every, tab);
         }
     }
 #pragma unroll
@@ -354,13 +373,13 @@ hipError_t launch_ext_rns_mac(const ExtRnsMacArgs &a, hipStream_t s) {
 
 // Multiply-accumulate of one transformed digit with a key entry, per
 // arithmetic: canonical (IntArith), below 4q (IntCanonArith: products in
-// [0, 4q), one subtraction of 4q per term), centred and reduced every third
-// term (F64Arith, as ext_mac).
+// [0, 4q), one subtraction of 4q per term), centred and summed unreduced
+// between ext_mac_settle's reductions (F64Arith, as ext_mac).
 template <class A>
 struct ExtRnsOps;
 template <>
 struct ExtRnsOps<IntArith> {
-    __device__ __forceinline__ static void mac(uint64_t &acc, uint64_t x, const Shoup &k, const DevTable &t, bool) {
+    __device__ __forceinline__ static void mac(uint64_t &acc, uint64_t x, const Shoup &k, const DevTable &t) {
         acc = add_mod(acc, shoup_full(k.w, k.wq, x, t.q), t.q);
     }
     __device__ __forceinline__ static uint64_t finish(uint64_t acc, const DevTable &) { return acc; }
@@ -368,7 +387,7 @@ struct ExtRnsOps<IntArith> {
 };
 template <>
 struct ExtRnsOps<IntCanonArith> {
-    __device__ __forceinline__ static void mac(uint64_t &acc, uint64_t x, const Shoup &k, const DevTable &t, bool) {
+    __device__ __forceinline__ static void mac(uint64_t &acc, uint64_t x, const Shoup &k, const DevTable &t) {
         acc = sub_if_ge(acc + shoup_approx(k.w, k.wq, x, t.q), t.q4);
     }
     __device__ __forceinline__ static uint64_t finish(uint64_t acc, const DevTable &) { return acc; }  // < 4q
@@ -378,8 +397,8 @@ struct ExtRnsOps<IntCanonArith> {
 };
 template <>
 struct ExtRnsOps<F64Arith> {
-    __device__ __forceinline__ static void mac(double &acc, double x, const ShoupF &k, const DevTable &t, bool red) {
-        ext_mac(acc, x, k, t, red);
+    __device__ __forceinline__ static void mac(double &acc, double x, const ShoupF &k, const DevTable &t) {
+        ext_mac(acc, x, k, t);
     }
     __device__ __forceinline__ static double finish(double acc, const DevTable &t) { return F64Arith::reduce(acc, t); }
     __device__ __forceinline__ static uint64_t canon(double acc, const DevTable &t) { return F64Arith::to_canon(acc, t); }
@@ -519,7 +538,7 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) voi
 #pragma unroll
     for (int j = 0; j < G::E; ++j) acc[0][j] = acc[1][j] = (T)0;
 
-    uint32_t term = 0;
+    uint32_t terms = 0;  // key products since the last reduction (binary64)
     for (uint32_t p = 0; p < 2; ++p) {
         // the thread's first-round coefficients, composed
         const uint64_t *src = g.in + (prod * 2 + p) * g.L * n;
@@ -561,7 +580,6 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) voi
             asm volatile("" : "+v"(tl));
             FwdRounds<A, LOGN, LOGT, 0, true, A::kRegisterOrder>::run(x, lds, tl, 0, 0, td);
             // thread t holds slots t*E + j; key row (p, d, c, limb) stores them at j*T + t
-            const bool red = (++term % 3) == 0;
             using Wt = typename A::Tw;
             const uint64_t row0 = (((uint64_t)p * g.l + d) * 2) * g.L + limb;
             const auto k0 = table_rsrc(key + row0 * n);
@@ -578,11 +596,12 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) voi
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int jj = 0; jj < kExtKeyBatch; ++jj) {
-                    Ops::mac(acc[0][j0 + jj], x[j0 + jj], w0[jj], td, red);
-                    Ops::mac(acc[1][j0 + jj], x[j0 + jj], w1[jj], td, red);
+                    Ops::mac(acc[0][j0 + jj], x[j0 + jj], w0[jj], td);
+                    Ops::mac(acc[1][j0 + jj], x[j0 + jj], w1[jj], td);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
+            ext_mac_settle<A, G::E>(acc, terms, g.mac_reduce_every, td);
         }
     }
 #pragma unroll

@@ -30,9 +30,23 @@ struct ExtArgs {
     uint64_t basis_minus_one, carry_mask, half, q_minus_basis;
     uint32_t drop_bits, log_basis, l;
     uint32_t has_split, basis_is_two, ntt_output;
+    // binary64 keys: the accumulators are reduced after this many key
+    // products (ext_mac_reduce_interval); integer keys ignore it
+    uint32_t mac_reduce_every;
 };
 
 hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64_t count, hipStream_t s);
+
+// Key products a binary64 accumulator takes between reductions.  Each product
+// is an exact integer below 0.7503 q in magnitude (a forward output below
+// 3.2 q times a centred key entry, F64Arith::mulmod), a reduced accumulator is
+// at most q/2 + 1, and the sum stays exact while below 2^53; 0.76 q per product
+// and q for the reduced start leave a margin.  At least 9 for q < 2^50, so an
+// external product with 2 l <= 9 terms never reduces before its end.
+inline uint32_t ext_mac_reduce_interval(uint64_t q) {
+    const double r = (9007199254740992.0 - (double)q) / (0.76 * (double)q);
+    return r >= 1048576.0 ? 1048576u : (uint32_t)r;
+}
 
 // External product over several limbs (CrtGlwe::mul_dcrt_ggsw_inplace): the
 // gadget basis is over Q = q_0 ... q_{L-1}, at most kExtRnsMaxLimbs limbs, so
@@ -53,7 +67,7 @@ struct ExtRnsArgs {
     uint64_t count;
     uint32_t L, l, log_basis, drop_bits, has_split, log_n;
     uint32_t punct_words;  // the most nonzero words of any Q/q_i
-    uint32_t pad_;
+    uint32_t mac_reduce_every;  // as ExtArgs (the fused kernel's binary64 accumulators)
 };
 
 // The multiply-accumulate of the transformed digits with the key.

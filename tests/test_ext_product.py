@@ -110,6 +110,7 @@ def test_oracle_ext_product_schoolbook(q, log_n, log_basis, l):
     (EXT_Q32, 10, 8, 4, 7),
     (132120577, 8, 7, 3, 5),
     (1125899906826241, 12, 10, 4, 3),
+    (1125899906826241, 10, 5, 9, 2),          # 18 key products: binary64 accumulators reduced twice
     (576460752303421441, 10, 20, 2, 4),       # > 2^50: integer path either way
     (12289, 6, 1, 13, 3),                     # binary basis
 ])
