@@ -93,9 +93,14 @@ __device__ __forceinline__ void ext_finish(uint64_t &, const DevTable &) {}
 __device__ __forceinline__ void ext_finish(double &acc, const DevTable &t) { acc = F64Arith::reduce(acc, t); }
 
 // Registers whose key entries (two rows each) the multiply-accumulate loads
-// in one batch.  Two fit beside the accumulators at 4 waves per SIMD; four
-// spill there, and 3 waves per SIMD with eight measured slower.
-constexpr int kExtKeyBatch = 2;
+// in one batch.  Binary64: four, which fit beside the accumulators at 4 waves
+// per SIMD once the key products are summed without a per-term reduction, and
+// keep twice the key loads of two in flight.  Integer arithmetic: two (four
+// spill there, and 3 waves per SIMD with eight measured slower).
+template <class A>
+constexpr int ext_key_batch() {
+    return std::is_same<A, F64Arith>::value ? 4 : 2;
+}
 
 template <class A, int LOGN>
 __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(ExtArgs g) {
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
             // turn.
             using W = typename A::Tw;
             const auto krs = table_rsrc(key + ((uint64_t)(p * g.l + d) * 2) * n);
-            constexpr int KB = kExtKeyBatch < G::E ? kExtKeyBatch : G::E;
+            constexpr int KB = ext_key_batch<A>() < G::E ? ext_key_batch<A>() : G::E;
             static_assert(G::E % KB == 0, "key batches tile the register group");
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -586,16 +591,16 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) voi
             const auto k1 = table_rsrc(key + (row0 + g.L) * n);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int j0 = 0; j0 < G::E; j0 += kExtKeyBatch) {
-                Wt w0[kExtKeyBatch], w1[kExtKeyBatch];
+            for (int j0 = 0; j0 < G::E; j0 += ext_key_batch<A>()) {
+                Wt w0[ext_key_batch<A>()], w1[ext_key_batch<A>()];
 #pragma unroll
-                for (int jj = 0; jj < kExtKeyBatch; ++jj) {
+                for (int jj = 0; jj < ext_key_batch<A>(); ++jj) {
                     w0[jj] = ld_lane<Wt>(k0, tl * (uint32_t)sizeof(Wt), (j0 + jj) * G::T * (uint32_t)sizeof(Wt));
                     w1[jj] = ld_lane<Wt>(k1, tl * (uint32_t)sizeof(Wt), (j0 + jj) * G::T * (uint32_t)sizeof(Wt));
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int jj = 0; jj < kExtKeyBatch; ++jj) {
+                for (int jj = 0; jj < ext_key_batch<A>(); ++jj) {
                     Ops::mac(acc[0][j0 + jj], x[j0 + jj], w0[jj], td);
                     Ops::mac(acc[1][j0 + jj], x[j0 + jj], w1[jj], td);
                 }

@@ -114,8 +114,10 @@ struct IntArith {
     __device__ __forceinline__ static gptr<Tw> fwd(const DevTable &t) { return as_global(t.fwd); }
     __device__ __forceinline__ static gptr<Tw> inv(const DevTable &t) { return as_global(t.inv); }
     static constexpr bool kRegisterOrder = false;  // integer tables keep the CPU order
-    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t, int) { return fwd(t); }
-    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t, int) { return inv(t); }
+    template <int LOGE>
+    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t) { return fwd(t); }
+    template <int LOGE>
+    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t) { return inv(t); }
 
     // Harvey forward butterfly, primitive.rs:225-228 (every stage reduces:
     // REDUCE is for IntCanonArith's schedule).
@@ -205,8 +207,10 @@ struct IntCanonArith {
     __device__ __forceinline__ static gptr<Tw> fwd(const DevTable &t) { return as_global(t.fwd); }
     __device__ __forceinline__ static gptr<Tw> inv(const DevTable &t) { return as_global(t.inv); }
     static constexpr bool kRegisterOrder = false;
-    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t, int) { return fwd(t); }
-    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t, int) { return inv(t); }
+    template <int LOGE>
+    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t) { return fwd(t); }
+    template <int LOGE>
+    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t) { return inv(t); }
 
     // forward, see above: u < 12q, v < 4q, outputs below 16q
     template <bool REDUCE = true>
@@ -265,11 +269,16 @@ struct F64Arith {
     __device__ __forceinline__ static gptr<Tw> inv(const DevTable &t) { return as_global(t.inv_f); }
     // register-order tables of the single-pass shape with 2^loge coefficients per thread
     static constexpr bool kRegisterOrder = true;
-    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t, int loge) {
-        return as_global(t.fwd_p[loge - 3]);
+    // (built for 8 and 16 coefficients per thread only: fwd_p / inv_p hold two)
+    template <int LOGE>
+    __device__ __forceinline__ static gptr<Tw> fwd_reg(const DevTable &t) {
+        static_assert(LOGE == 3 || LOGE == 4, "register-order tables exist for 8 and 16 coefficients per thread");
+        return as_global(t.fwd_p[LOGE - 3]);
     }
-    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t, int loge) {
-        return as_global(t.inv_p[loge - 3]);
+    template <int LOGE>
+    __device__ __forceinline__ static gptr<Tw> inv_reg(const DevTable &t) {
+        static_assert(LOGE == 3 || LOGE == 4, "register-order tables exist for 8 and 16 coefficients per thread");
+        return as_global(t.inv_p[LOGE - 3]);
     }
 
     // x*w mod q, centred.  Twiddles are stored centred (|w| <= q/2, |wp| < 1/2

@@ -252,7 +252,7 @@ struct FwdRounds {
             if constexpr (OB >= 6 && OB < LOGT) cl = __builtin_amdgcn_readfirstlane(cl);
             constexpr bool LANE = OB < 6 && OB < LOGT;  // cl differs between lanes
             if constexpr (PERM)
-                ct_round<A, R, FENCE, true, LANE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd_reg(tab, G::LOGE),
+                ct_round<A, R, FENCE, true, LANE>(x + (m << R), st0 + S, (b << S) + cl, A::template fwd_reg<G::LOGE>(tab),
                                                   tab);
             else
                 ct_round<A, R, FENCE, false, LANE>(x + (m << R), st0 + S, (b << S) + cl, A::fwd(tab), tab);
@@ -341,7 +341,7 @@ struct InvRounds {
             constexpr bool LANE = P < 6 && P < LOGT;
             if constexpr (PERM)
                 gs_round<A, R, last_here, true, true, LANE, SCALED>(x + (m << R), P, (b << (LOGC - P - R)) + cl, n,
-                                                                    A::inv_reg(tab, G::LOGE), tab);
+                                                                    A::template inv_reg<G::LOGE>(tab), tab);
             else
                 gs_round<A, R, last_here, true, false, LANE, SCALED>(x + (m << R), P, (b << (LOGC - P - R)) + cl,
                                                                      n, A::inv(tab), tab);
