@@ -508,10 +508,20 @@ __device__ __forceinline__ uint64_t ext_rns_digit(const ExtRnsArgs &g, uint64_t 
     return (g.basis == 2 || digit < g.half) ? digit : q - g.basis + digit;
 }
 
-// Workgroup b handles product 8*(b / (8L)) + b % 8 and limb (b % (8L)) / 8:
-// the L workgroups of a product sit on one XCD (workgroups go to the XCDs in
-// turn) and start close together, so the L - 1 re-reads of its input rows are
-// served by that XCD's L2.
+// Two launch forms.
+// * Per (product, limb) (COOP false): workgroup b handles product
+//   8*(b / (8L)) + b % 8 and limb (b % (8L)) / 8.  The L workgroups of a
+//   product sit on one XCD (workgroups go to the XCDs in turn) and start close
+//   together, so the L - 1 re-reads of its input rows are served by that XCD's
+//   L2; each composes every coefficient itself.
+// * Per product (COOP true): one workgroup of L x 2^LOGT threads, thread
+//   group h (2^LOGT threads, whole waves) running limb h's transforms in its
+//   own slice of the LDS.  Each coefficient is composed once: all threads
+//   compose the N coefficients of an input polynomial in natural order
+//   (coalesced residue loads) and leave their digit windows in the LDS, which
+//   the transforms have not used yet; each thread then reads the windows of
+//   its first-round slots.  The limbs' transforms run in lockstep, so the
+//   workgroup barriers of their LDS exchanges are reached by all threads.
 // Requires l * log_basis + (drop_bits > 0) <= 64 (the digits fit one window).
 // Four waves per SIMD (128 VGPRs: the two accumulators, the windows, a
 // transform and the key batch); integer arithmetic above N = 2^10 needs more
@@ -520,22 +530,33 @@ template <class A>
 constexpr int ext_rns_waves(int log_n) {
     return std::is_same<A, F64Arith>::value || log_n <= 10 ? 4 : 3;
 }
-template <class A, int LOGN, int L>
-__global__ __launch_bounds__(1 << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) void ext_rns_fused_kernel(ExtRnsFusedArgs a) {
+template <class A, int LOGN, int L, bool COOP>
+__global__ __launch_bounds__((COOP ? L : 1) << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) void ext_rns_fused_kernel(
+    ExtRnsFusedArgs a) {
     constexpr int LOGT = LOGN - kExtLogE;
     using G = Geo<LOGN, LOGT>;
     using T = typename A::T;
     using Ops = ExtRnsOps<A>;
     static_assert(G::E == (1 << kExtLogE), "key permutation assumes this register layout");
+    static_assert(!COOP || LOGT >= 6, "a limb's threads must be whole waves");
     constexpr int R0 = G::fr(0);
-    __shared__ T lds[G::LDS];
+    __shared__ T lds_all[(COOP ? L : 1) * G::LDS];
     const ExtRnsArgs &g = a.g;
-    const uint32_t t = threadIdx.x;
-    const uint32_t group = 8u * g.L;
-    const uint32_t r = blockIdx.x % group;
-    const uint64_t prod = (uint64_t)(blockIdx.x / group) * 8u + (r & 7u);
-    const uint32_t limb = r >> 3;
-    if (prod >= g.count) return;  // the whole workgroup: no barrier reached
+    uint32_t t, limb;
+    uint64_t prod;
+    if constexpr (COOP) {
+        t = threadIdx.x & ((1u << LOGT) - 1);
+        limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> LOGT);  // the same on a wave's lanes
+        prod = blockIdx.x;  // the grid is the product count
+    } else {
+        t = threadIdx.x;
+        const uint32_t group = 8u * g.L;
+        const uint32_t r = blockIdx.x % group;
+        prod = (uint64_t)(blockIdx.x / group) * 8u + (r & 7u);
+        limb = r >> 3;
+        if (prod >= g.count) return;  // the whole workgroup: no barrier reached
+    }
+    T *lds = lds_all + (COOP ? limb * G::LDS : 0);
     const DevTable tab = a.tabs[limb];
     const uint64_t n = 1ull << LOGN;
     const auto key = ext_rns_key(a, (const A *)nullptr);
@@ -550,14 +571,35 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) voi
         const uint32_t first = g.drop_bits > 0 ? 1u : 0u;
         uint64_t win[G::E];
         uint32_t carry[G::E];
+        if constexpr (COOP) {
+            // the previous polynomial's last exchange is read by every thread
+            // before the windows overwrite the LDS, and the windows are all
+            // written before any thread reads its own
+            uint64_t *wl = reinterpret_cast<uint64_t *>(lds_all);
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < (uint32_t)n; k += (uint32_t)L << LOGT)
+                wl[k] = ext_rns_window<L>(a, src + k, n);
+            __syncthreads();
 #pragma unroll
-        for (int m = 0; m < (G::E >> R0); ++m)
+            for (int m = 0; m < (G::E >> R0); ++m)
 #pragma unroll
-            for (int j = 0; j < (1 << R0); ++j) {
-                const int e = (m << R0) + j;
-                win[e] = ext_rns_window<L>(a, src + fwd_slot<LOGN, LOGT, 0, R0>(t, m, j), n);
-                carry[e] = (uint32_t)(win[e] & first);
-            }
+                for (int j = 0; j < (1 << R0); ++j) {
+                    const int e = (m << R0) + j;
+                    win[e] = wl[fwd_slot<LOGN, LOGT, 0, R0>(t, m, j)];
+                    carry[e] = (uint32_t)(win[e] & first);
+                }
+            // the first transform's first exchange stores after a workgroup
+            // barrier (SYNC0), so these reads finish before the LDS is reused
+        } else {
+#pragma unroll
+            for (int m = 0; m < (G::E >> R0); ++m)
+#pragma unroll
+                for (int j = 0; j < (1 << R0); ++j) {
+                    const int e = (m << R0) + j;
+                    win[e] = ext_rns_window<L>(a, src + fwd_slot<LOGN, LOGT, 0, R0>(t, m, j), n);
+                    carry[e] = (uint32_t)(win[e] & first);
+                }
+        }
         for (uint32_t d = 0; d < g.l; ++d) {
             T x[G::E];
             if constexpr (std::is_same<A, F64Arith>::value) {
@@ -627,18 +669,29 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), ext_rns_waves<A>(LOGN)) voi
     }
 }
 
-template <class A, int LOGN>
-static hipError_t launch_ext_rns_fused_n(const ExtRnsFusedArgs &a, hipStream_t s) {
-    const uint64_t groups = (a.g.count + 7) / 8;
-    const dim3 grid((uint32_t)(groups * 8 * a.g.L)), block(1u << (LOGN - kExtLogE));
-    switch (a.g.L) {
-        case 1: ext_rns_fused_kernel<A, LOGN, 1><<<grid, block, 0, s>>>(a); break;
-        case 2: ext_rns_fused_kernel<A, LOGN, 2><<<grid, block, 0, s>>>(a); break;
-        case 3: ext_rns_fused_kernel<A, LOGN, 3><<<grid, block, 0, s>>>(a); break;
-        case 4: ext_rns_fused_kernel<A, LOGN, 4><<<grid, block, 0, s>>>(a); break;
-        default: return hipErrorInvalidValue;
+template <class A, int LOGN, int L>
+static hipError_t launch_ext_rns_fused_l(const ExtRnsFusedArgs &a, hipStream_t s) {
+    constexpr int LOGT = LOGN - (int)kExtLogE;
+    // one workgroup per product when the limbs' threads are whole waves and
+    // fit one workgroup
+    if constexpr (L >= 2 && LOGT >= 6 && (L << LOGT) <= 1024) {
+        ext_rns_fused_kernel<A, LOGN, L, true><<<dim3((uint32_t)a.g.count), dim3(L << LOGT), 0, s>>>(a);
+    } else {
+        const uint64_t groups = (a.g.count + 7) / 8;
+        ext_rns_fused_kernel<A, LOGN, L, false><<<dim3((uint32_t)(groups * 8 * L)), dim3(1u << LOGT), 0, s>>>(a);
     }
     return hipGetLastError();
+}
+
+template <class A, int LOGN>
+static hipError_t launch_ext_rns_fused_n(const ExtRnsFusedArgs &a, hipStream_t s) {
+    switch (a.g.L) {
+        case 1: return launch_ext_rns_fused_l<A, LOGN, 1>(a, s);
+        case 2: return launch_ext_rns_fused_l<A, LOGN, 2>(a, s);
+        case 3: return launch_ext_rns_fused_l<A, LOGN, 3>(a, s);
+        case 4: return launch_ext_rns_fused_l<A, LOGN, 4>(a, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <int LOGN>
