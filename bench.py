@@ -653,6 +653,9 @@ def main():
     if not (args.no_serving or args.no_secondary or args.profile or args.secondary_only):
         serving = serving_leg()
 
+    # one hardware queue per stream of the pipelined pass (up to 6 streams),
+    # read when HIP starts, which has not happened yet in this process
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
     import torch
 
     import primus_amd as pa
@@ -738,11 +741,12 @@ def main():
                           "launches": K + args.warmup, "calibration_copy_bytes": src.numel() * 8}))
         return 0
 
-    # pipelined: consecutive batches on 2, 3 and 4 streams (ROTATE is a multiple
-    # of each, so a buffer is only ever reused on the stream that last wrote it)
+    # pipelined: consecutive batches on 2, 3, 4 and 6 streams (ROTATE is a
+    # multiple of each, so a buffer is only ever reused on the stream that last
+    # wrote it)
     pipelined = {}
-    extra = [torch.cuda.Stream(device=dev) for _ in range(3)]
-    for S in (2, 3, 4):
+    extra = [torch.cuda.Stream(device=dev) for _ in range(5)]
+    for S in (2, 3, 4, 6):
         streams = [stream] + extra[:S - 1]
         steps = [launcher(table, op, ctypes.c_void_p(s.cuda_stream)) for s in streams]
         pipelined[S] = timer.run_pipelined(steps, streams, K, args.warmup)
