@@ -213,6 +213,7 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
 
 template <class A, int LOGN>
 static hipError_t launch_ext_n(const ExtArgs &g, uint64_t count, hipStream_t s) {
+    if (!grid_fits(count, 1u << (LOGN - kExtLogE))) return hipErrorInvalidConfiguration;
     ext_product_kernel<A, LOGN><<<dim3((uint32_t)count), dim3(1u << (LOGN - kExtLogE)), 0, s>>>(g);
     return hipGetLastError();
 }
@@ -361,6 +362,7 @@ __global__ __launch_bounds__(256) void ext_rns_mac_kernel(ExtRnsMacArgs a) {
 hipError_t launch_ext_rns_digits(const ExtRnsArgs &a, hipStream_t s) {
     const uint64_t threads = a.count * 2 * (1ull << a.log_n);
     if (threads == 0) return hipSuccess;
+    if (!grid_fits((threads + 255) / 256, 256)) return hipErrorInvalidConfiguration;
     ext_rns_digits_kernel<<<dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s>>>(a);
     return hipGetLastError();
 }
@@ -368,6 +370,7 @@ hipError_t launch_ext_rns_digits(const ExtRnsArgs &a, hipStream_t s) {
 hipError_t launch_ext_rns_mac(const ExtRnsMacArgs &a, hipStream_t s) {
     const uint64_t threads = a.count * a.L * (1ull << a.log_n);
     if (threads == 0) return hipSuccess;
+    if (!grid_fits((threads + 255) / 256, 256)) return hipErrorInvalidConfiguration;
     ext_rns_mac_kernel<<<dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s>>>(a);
     return hipGetLastError();
 }
@@ -675,9 +678,11 @@ static hipError_t launch_ext_rns_fused_l(const ExtRnsFusedArgs &a, hipStream_t s
     // one workgroup per product when the limbs' threads are whole waves and
     // fit one workgroup
     if constexpr (L >= 2 && LOGT >= 6 && (L << LOGT) <= 1024) {
+        if (!grid_fits(a.g.count, L << LOGT)) return hipErrorInvalidConfiguration;
         ext_rns_fused_kernel<A, LOGN, L, true><<<dim3((uint32_t)a.g.count), dim3(L << LOGT), 0, s>>>(a);
     } else {
         const uint64_t groups = (a.g.count + 7) / 8;
+        if (!grid_fits(groups * 8 * L, 1u << LOGT)) return hipErrorInvalidConfiguration;
         ext_rns_fused_kernel<A, LOGN, L, false><<<dim3((uint32_t)(groups * 8 * L)), dim3(1u << LOGT), 0, s>>>(a);
     }
     return hipGetLastError();
@@ -968,6 +973,7 @@ __global__ __launch_bounds__(256) void rlwe_sample_kernel(SampleArgs a) {
 
 hipError_t launch_rlwe_sample(const SampleArgs &a, hipStream_t st) {
     if (a.n_chunks == 0) return hipSuccess;
+    if (!grid_fits(a.n_chunks, 256)) return hipErrorInvalidConfiguration;
     rlwe_sample_kernel<<<dim3(a.n_chunks), dim3(256), 0, st>>>(a);
     return hipGetLastError();
 }

@@ -41,6 +41,13 @@
 
 namespace ph {
 
+// A launch's grid in one dimension is a 32-bit count of work-items (the
+// dispatch packet's grid size): launchers refuse larger batches with an error
+// instead of truncating the workgroup count.
+inline bool grid_fits(uint64_t groups, uint64_t threads_per_group) {
+    return groups <= 0xffffffffull / threads_per_group;
+}
+
 // Pointer into the constant address space (read-only global memory).  Table
 // pointers read out of a DevTable are generic otherwise, and generic loads are
 // FLAT instructions that count against both memory counters (every LDS wait

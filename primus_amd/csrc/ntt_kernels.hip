@@ -405,6 +405,7 @@ __global__ __launch_bounds__(256) void mul_factor_kernel(MulFactorArgs a) {
 
 hipError_t launch_mul_factor(const MulFactorArgs &a, uint32_t bit_shift, hipStream_t s) {
     if (a.total == 0) return hipSuccess;
+    if (!grid_fits((a.total + 255) / 256, 256)) return hipErrorInvalidConfiguration;
     const dim3 grid((uint32_t)((a.total + 255) / 256)), block(256);
     switch (bit_shift) {
         case 32: mul_factor_kernel<32><<<grid, block, 0, s>>>(a); break;
@@ -436,6 +437,7 @@ template <class A, int LOGC, int LOGT, int MODE>
 static hipError_t launch_chunk(const KArgs &a, uint64_t count, hipStream_t s) {
     const uint64_t blocks = count * a.blocks_per_poly;
     if (blocks == 0) return hipSuccess;
+    if (!grid_fits(blocks, 1u << LOGT)) return hipErrorInvalidConfiguration;
     chunk_kernel<A, LOGC, LOGT, MODE><<<dim3((uint32_t)blocks), dim3(1u << LOGT), 0, s>>>(a);
     return hipGetLastError();
 }
@@ -477,6 +479,7 @@ template <int LOGR, int MODE, class CA = IntArith>
 static hipError_t launch_column(const KArgs &a, uint32_t log_n, uint64_t count, hipStream_t s) {
     const uint64_t threads = count << (log_n - LOGR);
     if (threads == 0) return hipSuccess;
+    if (!grid_fits(threads / 256u, 256)) return hipErrorInvalidConfiguration;
     column_kernel<LOGR, MODE, CA><<<dim3((uint32_t)(threads / 256u)), dim3(256), 0, s>>>(a);
     return hipGetLastError();
 }
@@ -683,6 +686,7 @@ hipError_t launch_automorphism(const uint64_t *in, uint64_t *out, uint32_t log_n
                                hipStream_t s) {
     if (rows == 0) return hipSuccess;
     const uint64_t blocks = log_n <= 12 ? rows : ((rows << log_n) + 255) / 256;
+    if (!grid_fits(blocks, 256)) return hipErrorInvalidConfiguration;
     automorphism_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(in, out, log_n, k, rows);
     return hipGetLastError();
 }
@@ -692,6 +696,7 @@ hipError_t launch_pointwise(const DevTable *tabs, uint32_t L, uint32_t log_n, ui
                             hipStream_t s) {
     if (total == 0) return hipSuccess;
     const uint64_t blocks = (total + 255) / 256;
+    if (!grid_fits(blocks, 256)) return hipErrorInvalidConfiguration;
     pointwise_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(tabs, L, log_n, total, a, b, out, mac);
     return hipGetLastError();
 }

@@ -151,6 +151,7 @@ hipError_t launch_base_convert(const BaseConvDev &c, const uint64_t *in, uint64_
                                uint64_t polys, bool exact, hipStream_t s) {
     const uint64_t threads = polys << log_n;
     if (threads == 0) return hipSuccess;
+    if (!grid_fits((threads + 255) / 256, 256)) return hipErrorInvalidConfiguration;
     const dim3 grid((uint32_t)((threads + 255) / 256)), block(256);
     if (c.f64) {
         if (exact) base_convert_f64_kernel<true><<<grid, block, 0, s>>>(c, in, out, log_n, polys);

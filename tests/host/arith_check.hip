@@ -6,12 +6,15 @@
 //   mulhi64_approx(a, b)          in [floor(a*b/2^64) - 2, floor(a*b/2^64)]
 //   shoup_approx(w, wq, b, q)     == w*b (mod q) and < 4q, for q < 2^62, w < q,
 //                                    wq = floor(w*2^64/q) and any 64-bit b
+//   grid_fits(groups, threads)    == (groups * threads < 2^32), the launchers'
+//                                    bound on a grid's work-items
 // on random operands and on the edges (0, 1, all-ones, x == m, q near 2^62).
 // Prints "ok <cases>" and exits 0, or the first mismatch and exits 1.
 #include <cstdio>
 #include <random>
 
 #include "../../primus_amd/csrc/arith.hpp"
+#include "../../primus_amd/csrc/ntt_device.hpp"
 
 using namespace ph;
 typedef unsigned __int128 u128x;
@@ -53,6 +56,15 @@ int main() {
         const uint64_t r = shoup_approx(w, wq, b, q);
         if (r >= 4 * (u128x)q || r % q != (uint64_t)(((u128x)w * b) % q)) return fail("shoup_approx", w, b, q, r);
         ++cases;
+    }
+    for (uint64_t threads : {1ull, 64ull, 256ull, 512ull, 1024ull}) {
+        const uint64_t edge[] = {0, 1, (1ull << 32) / threads - 1, (1ull << 32) / threads,
+                                 (1ull << 32) / threads + 1, 1ull << 40, ~0ull / threads};
+        for (uint64_t groups : edge) {
+            const bool want = (u128x)groups * threads < ((u128x)1 << 32);
+            if (grid_fits(groups, threads) != want) return fail("grid_fits", groups, threads, want, 0);
+            ++cases;
+        }
     }
     std::printf("ok %ld\n", cases);
     return 0;
