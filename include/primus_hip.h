@@ -308,6 +308,18 @@ uint32_t ph_ext_key_log_n(const ph_ext_key *key);
  * fused launch. */
 int ph_ext_product_device(const ph_ext_key *key, const uint64_t *d_in, uint64_t *d_out, size_t count,
                           int ntt_output, void *stream);
+/* The key of ph_ext_key_new with a choice of digits.  PH_DIGITS_APPROX_SIGNED
+ * is the reference's approximate signed decomposition, as ph_ext_key_new.
+ * PH_DIGITS_UNSIGNED is the literal reading of "l unsigned digits of base
+ * 2^log_basis": digit d of a coefficient v < q is
+ * (v >> (drop_bits + d * log_basis)) & (2^log_basis - 1), with no rounding
+ * carry, no split adjustment and no signed mapping.  It has no counterpart in
+ * the reference, whose only external product uses the approximate signed
+ * digits (primus_decompose/src/big_integer/common.rs:150-201). */
+#define PH_DIGITS_APPROX_SIGNED 0
+#define PH_DIGITS_UNSIGNED 1
+int ph_ext_key_new_digits(const ph_ntt_table *table, uint32_t log_basis, uint32_t decompose_length, int digits,
+                          const uint64_t *key_ntt, ph_ext_key **out);
 /* The same on host memory, synchronous. */
 int ph_ext_product(const ph_ext_key *key, const uint64_t *in, uint64_t *out, size_t count, int ntt_output);
 

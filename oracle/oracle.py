@@ -73,6 +73,8 @@ def lib():
             ("po_basis_new", ctypes.c_int, [_u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
             ("po_decompose", None, [ctypes.c_void_p, _p64, ctypes.c_size_t, _p64]),
             ("po_ext_product", None, [ctypes.c_void_p, ctypes.c_void_p, _p64, _p64, _p64, ctypes.c_int]),
+            ("po_decompose_unsigned", None, [ctypes.c_void_p, _p64, ctypes.c_size_t, _p64]),
+            ("po_ext_product_unsigned", None, [ctypes.c_void_p, ctypes.c_void_p, _p64, _p64, _p64, ctypes.c_int]),
             ("po_ntt_automorphism", None, [_p64, _p64, ctypes.c_size_t, _u64]),
             ("po_base_converter_new", ctypes.c_int,
              [_p64, ctypes.c_uint32, _p64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
@@ -266,13 +268,24 @@ class Basis(ctypes.Structure):
         lib().po_decompose(ctypes.byref(self), ptr(v), v.size, ptr(out))
         return out
 
+    def decompose_unsigned(self, values):
+        """The literal unsigned digits (po_decompose_unsigned): no carry, no
+        split, no signed mapping; [l][n] residues."""
+        v = np.ascontiguousarray(values, dtype=np.uint64)
+        out = np.zeros((self.l, v.size), dtype=np.uint64)
+        lib().po_decompose_unsigned(ctypes.byref(self), ptr(v), v.size, ptr(out))
+        return out
 
-def ext_product(table, basis, key, rlwe, ntt_output=False):
-    """External product of one RLWE ciphertext [2][N] with a GGSW key [2][l][2][N] (NTT form)."""
+
+def ext_product(table, basis, key, rlwe, ntt_output=False, unsigned_digits=False):
+    """External product of one RLWE ciphertext [2][N] with a GGSW key [2][l][2][N]
+    (NTT form); unsigned_digits: the literal unsigned digits instead of the
+    reference's approximate signed ones."""
     key = np.ascontiguousarray(key, dtype=np.uint64)
     x = np.ascontiguousarray(rlwe, dtype=np.uint64)
     out = np.zeros(2 * table.n, dtype=np.uint64)
-    lib().po_ext_product(table.handle, ctypes.byref(basis), ptr(key), ptr(x), ptr(out), int(bool(ntt_output)))
+    fn = lib().po_ext_product_unsigned if unsigned_digits else lib().po_ext_product
+    fn(table.handle, ctypes.byref(basis), ptr(key), ptr(x), ptr(out), int(bool(ntt_output)))
     return out.reshape(2, table.n)
 
 

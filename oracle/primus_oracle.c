@@ -627,13 +627,21 @@ void po_decompose(const po_basis *b, const uint64_t *values, size_t n, uint64_t 
     }
 }
 
-void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, const uint64_t *in,
-                    uint64_t *out, int ntt_output) {
+void po_decompose_unsigned(const po_basis *b, const uint64_t *values, size_t n, uint64_t *residues) {
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t d = 0; d < b->l; ++d)
+            residues[(size_t)d * n + i] =
+                ((values[i] >> (b->drop_bits + d * b->log_basis)) & b->basis_minus_one) % b->q;
+}
+
+static void ext_product_digits(const po_table *t, const po_basis *b, const uint64_t *key, const uint64_t *in,
+                               uint64_t *out, int ntt_output, int unsigned_digits) {
     const size_t n = t->n;
     uint64_t *res = (uint64_t *)malloc(b->l * n * 8);
     memset(out, 0, 2 * n * 8);
     for (int p = 0; p < 2; ++p) {
-        po_decompose(b, in + p * n, n, res);
+        if (unsigned_digits) po_decompose_unsigned(b, in + p * n, n, res);
+        else po_decompose(b, in + p * n, n, res);
         for (uint32_t d = 0; d < b->l; ++d) {
             uint64_t *x = res + (size_t)d * n;
             po_transform(t, x); /* table.transform_slice(multi_residues), dcrt.rs:238 */
@@ -644,6 +652,16 @@ void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, c
     if (!ntt_output)
         for (int j = 0; j < 2; ++j) po_inverse_transform(t, out + j * n);
     free(res);
+}
+
+void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, const uint64_t *in,
+                    uint64_t *out, int ntt_output) {
+    ext_product_digits(t, b, key, in, out, ntt_output, 0);
+}
+
+void po_ext_product_unsigned(const po_table *t, const po_basis *b, const uint64_t *key, const uint64_t *in,
+                             uint64_t *out, int ntt_output) {
+    ext_product_digits(t, b, key, in, out, ntt_output, 1);
 }
 
 /* ------------------------------------------------------------------------- */

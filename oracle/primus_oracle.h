@@ -116,6 +116,15 @@ void po_decompose(const po_basis *b, const uint64_t *values, size_t n, uint64_t 
  * in: [2][N] coefficients < q; key: [2][l][2][N] NTT form < q; out: [2][N]. */
 void po_ext_product(const po_table *t, const po_basis *b, const uint64_t *key, const uint64_t *in,
                     uint64_t *out, int ntt_output);
+/* The literal reading of "l unsigned digits of base B": digit d of a value
+ * v < q is (v >> (drop_bits + d*log_basis)) & (B - 1), with no rounding carry,
+ * no split adjustment and no signed mapping (no reference counterpart: the
+ * reference's only external product uses the approximate signed digits
+ * above).  residues[l][n], each digit reduced mod q. */
+void po_decompose_unsigned(const po_basis *b, const uint64_t *values, size_t n, uint64_t *residues);
+/* po_ext_product with these digits. */
+void po_ext_product_unsigned(const po_table *t, const po_basis *b, const uint64_t *key, const uint64_t *in,
+                             uint64_t *out, int ntt_output);
 
 #ifdef __cplusplus
 }

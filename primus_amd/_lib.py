@@ -220,6 +220,7 @@ def _declare(L):
         "ph_memcpy_dtoh": (i, [vp, vp, sz]),
         "ph_stream_synchronize": (i, [vp]),
         "ph_ext_key_new": (i, [vp, u32, u32, p64, pvp]),
+        "ph_ext_key_new_digits": (i, [vp, u32, u32, i, p64, pvp]),
         "ph_ext_key_free": (None, [vp]),
         "ph_ext_product_device": (i, [vp, vp, vp, sz, i, vp]),
         "ph_ext_product": (i, [vp, p64, p64, sz, i]),

@@ -981,8 +981,15 @@ struct ph_ext_key {
 
 int ph_ext_key_new(const ph_ntt_table *t, uint32_t log_basis, uint32_t decompose_length, const uint64_t *key_ntt,
                    ph_ext_key **out) {
+    return ph_ext_key_new_digits(t, log_basis, decompose_length, PH_DIGITS_APPROX_SIGNED, key_ntt, out);
+}
+
+int ph_ext_key_new_digits(const ph_ntt_table *t, uint32_t log_basis, uint32_t decompose_length, int digits,
+                          const uint64_t *key_ntt, ph_ext_key **out) {
     if (!t || !key_ntt || !out) return fail(PH_ERR_INVALID, "null argument");
     *out = nullptr;
+    if (digits != PH_DIGITS_APPROX_SIGNED && digits != PH_DIGITS_UNSIGNED)
+        return fail(PH_ERR_INVALID, "digits: PH_DIGITS_APPROX_SIGNED or PH_DIGITS_UNSIGNED");
     if (t->log_n < kExtMinLogN || t->log_n > kExtMaxLogN)
         return fail(PH_ERR_UNSUPPORTED, "external product supports 2^6 <= N <= 2^12");
     const uint64_t q = t->q, n = t->n;
@@ -1022,6 +1029,10 @@ int ph_ext_key_new(const ph_ntt_table *t, uint32_t log_basis, uint32_t decompose
     g.pow2_sub_q = (uint64_t)((((u128)1) << bits) - q);
     g.tab = t->d_tab;
     g.mac_reduce_every = ext_mac_reduce_interval(q);
+    if (digits == PH_DIGITS_UNSIGNED) {
+        g.unsigned_digits = 1;
+        g.has_split = 0;
+    }
 
     // key rows [2][l][2][N] in NTT form, permuted so that slot t*E + j of a row
     // sits at j*(N/E) + t (the register layout of the kernel's last round)

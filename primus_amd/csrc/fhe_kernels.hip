@@ -136,7 +136,12 @@ __global__ __launch_bounds__(1 << (LOGN - kExtLogE), 4) void ext_product_kernel(
             }
         for (uint32_t d = 0; d < g.l; ++d) {
             T x[G::E];
-            if constexpr (std::is_same<A, F64Arith>::value) {
+            if (g.unsigned_digits) {
+                // below the basis, so below 2q: a forward input as it is
+                const uint32_t shift = g.drop_bits + d * g.log_basis;
+#pragma unroll
+                for (int i = 0; i < G::E; ++i) x[i] = A::load_fwd((adj[i] >> shift) & g.basis_minus_one, tab);
+            } else if constexpr (std::is_same<A, F64Arith>::value) {
                 if (g.log_basis <= kSignedDigitMaxLogBasis) {
                     const uint32_t shift = g.drop_bits + d * g.log_basis;
 #pragma unroll

@@ -33,6 +33,10 @@ struct ExtArgs {
     // binary64 keys: the accumulators are reduced after this many key
     // products (ext_mac_reduce_interval); integer keys ignore it
     uint32_t mac_reduce_every;
+    // 1: the literal unsigned digits (v >> (drop_bits + d*log_basis)) & (B - 1)
+    // of each coefficient, no carry, no split, no signed mapping
+    // (PH_DIGITS_UNSIGNED; has_split is 0 then)
+    uint32_t unsigned_digits;
 };
 
 hipError_t launch_ext_product(const ExtArgs &g, uint32_t log_n, bool f64, uint64_t count, hipStream_t s);

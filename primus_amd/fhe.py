@@ -12,17 +12,24 @@ from .ntt import _check, _stream
 _p64 = ctypes.POINTER(ctypes.c_uint64)
 
 
-class ExtKey:
-    """Device-resident GGSW key [2][l][2][N] (NTT form) with its gadget basis."""
+DIGITS_APPROX_SIGNED = 0  # PH_DIGITS_APPROX_SIGNED: the reference's digits
+DIGITS_UNSIGNED = 1  # PH_DIGITS_UNSIGNED: the literal unsigned digits
 
-    def __init__(self, table, log_basis, decompose_length, key_ntt):
+
+class ExtKey:
+    """Device-resident GGSW key [2][l][2][N] (NTT form) with its gadget basis;
+    unsigned_digits selects the literal unsigned digits (ph_ext_key_new_digits)
+    instead of the reference's approximate signed ones."""
+
+    def __init__(self, table, log_basis, decompose_length, key_ntt, unsigned_digits=False):
         k = np.ascontiguousarray(key_ntt, dtype=np.uint64)
         n = table.poly_length()
         if k.size != 4 * decompose_length * n:
             raise ValueError("key must hold 2 x l x 2 x N values (%d), got %d" % (4 * decompose_length * n, k.size))
         h = ctypes.c_void_p()
-        _check(lib().ph_ext_key_new(table.handle, log_basis, decompose_length, k.ctypes.data_as(_p64),
-                                    ctypes.byref(h)))
+        digits = DIGITS_UNSIGNED if unsigned_digits else DIGITS_APPROX_SIGNED
+        _check(lib().ph_ext_key_new_digits(table.handle, log_basis, decompose_length, digits,
+                                           k.ctypes.data_as(_p64), ctypes.byref(h)))
         self._h = h
         self.table = table  # keeps the table alive
         self.n = n

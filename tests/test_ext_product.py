@@ -101,7 +101,88 @@ def test_oracle_ext_product_schoolbook(q, log_n, log_basis, l):
     np.testing.assert_array_equal(got_ntt, t.transform(got))
 
 
+@pytest.mark.parametrize("q,log_basis,l", [(EXT_Q32, 8, 3), (EXT_Q32, 8, 4), (EXT_Q32, 1, 32), (12289, 14, 1),
+                                           (12289, 4, 3), (1125899906826241, 10, 5), (576460752303421441, 20, 2)])
+def test_unsigned_digits_reconstruct(q, log_basis, l):
+    """The literal unsigned digits of v < q: each below the basis, and their
+    weighted sum is v with its low drop_bits cleared, exactly (no carry, no
+    split); residues are the digits mod q."""
+    b = O.Basis.new(q, log_basis, l)
+    rng = np.random.default_rng(q % 101 + l)
+    v = rng.integers(0, q, size=400, dtype=np.uint64)
+    v[:6] = [0, 1, q - 1, q - 2, q // 2, (1 << b.drop_bits) - 1 if b.drop_bits else 3]
+    res = b.decompose_unsigned(v)
+    raw = [[(int(x) >> (b.drop_bits + d * log_basis)) & ((1 << log_basis) - 1) for x in v] for d in range(l)]
+    for k, x in enumerate(v):
+        total = sum(raw[d][k] << (b.drop_bits + d * log_basis) for d in range(l))
+        assert total == int(x) >> b.drop_bits << b.drop_bits
+        for d in range(l):
+            assert raw[d][k] < (1 << log_basis)
+            assert int(res[d, k]) == raw[d][k] % q
+
+
+@pytest.mark.parametrize("q,log_n,log_basis,l", [(12289, 4, 4, 3), (132120577, 5, 7, 3), (EXT_Q32, 5, 8, 3)])
+def test_oracle_ext_product_unsigned_schoolbook(q, log_n, log_basis, l):
+    """The unsigned-digit external product equals the sum of schoolbook
+    negacyclic products of each digit polynomial with its key row."""
+    n = 1 << log_n
+    t = O.Table(log_n, q)
+    b = O.Basis.new(q, log_basis, l)
+    rng = np.random.default_rng(log_n + 7)
+    key = rng.integers(0, q, size=(2, l, 2, n), dtype=np.uint64)
+    x = rng.integers(0, q, size=(2, n), dtype=np.uint64)
+    got = O.ext_product(t, b, key, x, unsigned_digits=True)
+    want = np.zeros((2, n), dtype=np.uint64)
+    for p in range(2):
+        for d in range(l):
+            digits = np.array([(int(v) >> (b.drop_bits + d * log_basis)) & ((1 << log_basis) - 1) for v in x[p]],
+                              dtype=np.uint64)
+            for j in range(2):
+                kc = t.inverse_transform(key[p, d, j])
+                want[j] = (want[j] + _negacyclic(digits, kc, q)) % np.uint64(q)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(O.ext_product(t, b, key, x, ntt_output=True, unsigned_digits=True),
+                                  t.transform(got))
+
+
 # ---------------------------------------------------------------- GPU parity
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integer_only", [False, True])
+@pytest.mark.parametrize("q,log_n,log_basis,l,count", [
+    (EXT_Q32, 10, 8, 3, 64),                  # cfg 4's parameters, read as literal unsigned digits
+    (EXT_Q32, 10, 8, 4, 5),                   # every bit a digit bit (drop_bits 0)
+    (12289, 6, 14, 1, 3),                     # basis above q: digits up to 2q, reduced by the transform
+    (1125899906826241, 12, 10, 4, 3),
+    (576460752303421441, 10, 20, 2, 4),       # > 2^50: integer path either way
+])
+def test_gpu_ext_product_unsigned_digits(q, log_n, log_basis, l, count, integer_only):
+    """PH_DIGITS_UNSIGNED against the oracle's restatement of the literal
+    unsigned digits, coefficient and NTT-form outputs."""
+    import primus_amd as pa
+    from primus_amd.fhe import ExtKey
+
+    n = 1 << log_n
+    print("ext_product unsigned", q, log_n, log_basis, l, count, integer_only, flush=True)
+    tab = pa.UintNttTable.new(log_n, q, integer_only=integer_only)
+    rng = np.random.default_rng(q % 977 + l)
+    key = rng.integers(0, q, size=(2, l, 2, n), dtype=np.uint64)
+    x = rng.integers(0, q, size=(count, 2, n), dtype=np.uint64)
+    x[0, 0, :6] = [0, 1, q - 1, q - 2, q // 2, q // 2 + 1]
+    ek = ExtKey(tab, log_basis, l, key, unsigned_digits=True)
+    got = ek.product(x)
+    got_ntt = ek.product(x, ntt_output=True)
+    ot = O.Table(log_n, q)
+    ob = O.Basis.new(q, log_basis, l)
+    for i in range(count):
+        np.testing.assert_array_equal(got[i], O.ext_product(ot, ob, key, x[i], unsigned_digits=True), err_msg=str(i))
+        np.testing.assert_array_equal(got_ntt[i], O.ext_product(ot, ob, key, x[i], ntt_output=True,
+                                                                unsigned_digits=True), err_msg=str(i))
+    # the reference's digits give a different product on the same inputs,
+    # except where both represent every value exactly (one digit, no bits dropped)
+    if l > 1 or ob.drop_bits > 0:
+        assert not np.array_equal(ExtKey(tab, log_basis, l, key).product(x[:1])[0], got[0])
+
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("integer_only", [False, True])
