@@ -612,7 +612,7 @@ def main():
     ap.add_argument("--serving-plain-requests", type=int, default=3000)
     ap.add_argument("--serving-pool", type=int, default=256)
     ap.add_argument("--serving-max-batch", type=int, default=256)
-    ap.add_argument("--serving-streams", type=int, default=4,
+    ap.add_argument("--serving-streams", type=int, default=6,
                     help="stream slots of the batched server (batches in flight)")
     ap.add_argument("--serving-pageable", action="store_true",
                     help="requests in ordinary (pageable) host memory, staged by the server")

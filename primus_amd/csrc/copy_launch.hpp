@@ -18,12 +18,15 @@ struct CopySpan {
 constexpr uint64_t kCopySpanMax = 256u << 10;
 
 // Workgroups of one span-copy launch (fewer when there are fewer spans).
-// Inputs: 64 keep one direction of the host link at its full rate (57 GB/s
-// measured on MI355X, the same as a DMA copy), and near 44 GB/s while the
-// other direction runs too.  Results: 16, enough for the serving mix's result
-// traffic (about 0.6 of its input bytes) with fewer stores outstanding while
-// other streams compute.
-constexpr uint32_t kCopyGroupsIn = 64;
+// Inputs: 32.  64 keep one direction of the host link at its full rate alone
+// (57 GB/s measured on MI355X, the same as a DMA copy), but in the serving mix,
+// where compute launches and the result copies run at the same time, 32 gave
+// 20.4k requests/s saturated against 18.4k for 64 (16-24: 19.8-20.2k, 48:
+// 18.7k, 128: 18.2k): fewer reads across the link outstanding, and less
+// interference with the kernels of the other streams.  Results: 16, enough for
+// the mix's result traffic (about 0.6 of its input bytes) with fewer stores
+// outstanding while other streams compute (8 and 32 measured lower).
+constexpr uint32_t kCopyGroupsIn = 32;
 constexpr uint32_t kCopyGroupsOut = 16;
 
 // spans: `count` spans in memory the device can read (page-locked host
