@@ -611,7 +611,7 @@ def main():
                     help="cap on offered time per rate: requests = min(--serving-requests, rate * this)")
     ap.add_argument("--serving-plain-requests", type=int, default=3000)
     ap.add_argument("--serving-pool", type=int, default=256)
-    ap.add_argument("--serving-max-batch", type=int, default=256)
+    ap.add_argument("--serving-max-batch", type=int, default=512)
     ap.add_argument("--serving-streams", type=int, default=6,
                     help="stream slots of the batched server (batches in flight)")
     ap.add_argument("--serving-pageable", action="store_true",
