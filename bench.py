@@ -611,9 +611,12 @@ def main():
                     help="cap on offered time per rate: requests = min(--serving-requests, rate * this)")
     ap.add_argument("--serving-plain-requests", type=int, default=3000)
     ap.add_argument("--serving-pool", type=int, default=256)
-    ap.add_argument("--serving-max-batch", type=int, default=512)
-    ap.add_argument("--serving-streams", type=int, default=6,
-                    help="stream slots of the batched server (batches in flight)")
+    ap.add_argument("--serving-max-batch", type=int, default=None,
+                    help="requests per batch of the batched server (default 512 with page-locked request "
+                         "buffers, 256 with --serving-pageable)")
+    ap.add_argument("--serving-streams", type=int, default=None,
+                    help="stream slots of the batched server, batches in flight (default 6 with page-locked "
+                         "request buffers, 4 with --serving-pageable, whose staging copies run on the host)")
     ap.add_argument("--serving-pageable", action="store_true",
                     help="requests in ordinary (pageable) host memory, staged by the server")
     ap.add_argument("--serving-saturation-requests", type=int, default=20000)
@@ -638,6 +641,10 @@ def main():
         args.serving_plain_requests = 20000
         args.serving_saturation_requests = 20000
         args.serving_pool = 96
+    if args.serving_max_batch is None:
+        args.serving_max_batch = 256 if args.serving_pageable else 512
+    if args.serving_streams is None:
+        args.serving_streams = 4 if args.serving_pageable else 6
     if args.serving:
         # one hardware queue per server stream (slots, input copies, result
         # copies): HIP's default of 4 makes streams share queues, and a copy
