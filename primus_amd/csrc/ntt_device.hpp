@@ -498,6 +498,9 @@ __device__ __forceinline__ void gs_round(typename A::T *x, uint32_t P, uint32_t 
     }
 }
 
+// Two words for 16-byte accesses (global_load/store_dwordx4, ds_write_b128).
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
 // Coefficient rows are read once and written once per launch: non-temporal
 // accesses stream them past the caches, which keep the twiddles, operands and
 // keys every workgroup re-reads (and leave less dirty data to write back when

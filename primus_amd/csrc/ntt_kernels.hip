@@ -353,26 +353,78 @@ __device__ __forceinline__ uint32_t auto_source(uint32_t d, uint32_t log_n, uint
 }
 
 // out[d] = in[src(d)] on every limb-polynomial (ntt_poly_auto_inplace,
-// dcrt.rs:97-112).  Rows of up to 2^12 words are staged in LDS so that HBM
-// sees only whole-row reads and writes; longer rows gather from global
-// memory, where the row's lines stay in L2 between their reads.
+// dcrt.rs:97-112), rows of up to 2^12 words.  A row is staged in LDS, so HBM
+// sees only whole-row reads and writes, 16 bytes per lane when both pointers
+// allow it (WIDE), else 8.  The grid is persistent: a workgroup walks rows
+// blockIdx.x, + gridDim.x, ..., and loads its next row into registers before
+// it stores the current one, so its reads and writes overlap.  On 1024 rows of
+// 2^12 words the wide form runs at 0.74 of the HBM peak with 512 workgroups
+// (11.3 us; a plain 16-byte copy of the same bytes takes 11.9 us), against
+// 0.58 for one row per workgroup with 8-byte accesses.  LOGN > 0 fixes the
+// degree at compile time (its index arithmetic and bounds fold away: 12.2 ->
+// 11.3 us at 2^12); LOGN = 0 takes it from log_n_arg.
+// Needs 1 <= log_n <= 12 and 8 << log_n bytes of dynamic LDS.
+template <bool WIDE, uint32_t LOGN>
+__global__ __launch_bounds__(256) void automorphism_rows_kernel(const uint64_t *in, uint64_t *out,
+                                                                uint32_t log_n_arg, uint32_t k, uint64_t rows) {
+    static_assert(LOGN <= 12, "rows of up to 2^12 words");
+    const uint32_t log_n = LOGN ? LOGN : log_n_arg;
+    using V = std::conditional_t<WIDE, u64x2, uint64_t>;
+    constexpr uint32_t kWords = WIDE ? 2 : 1;
+    constexpr uint32_t kPer = (1u << 12) / kWords / 256;  // elements per thread at N = 2^12
+    extern __shared__ uint64_t lds[];
+    const uint32_t elems = (1u << log_n) / kWords;
+    const uint32_t t = threadIdx.x;
+    uint64_t row = blockIdx.x;
+    if (row >= rows) return;
+    V r[kPer];
+    {
+        const V *src = reinterpret_cast<const V *>(in + (row << log_n));
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i)
+            if (i * 256 + t < elems) r[i] = __builtin_nontemporal_load(src + i * 256 + t);
+    }
+    for (;;) {
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i)
+            if (i * 256 + t < elems) *reinterpret_cast<V *>(lds + kWords * (i * 256 + t)) = r[i];
+        __syncthreads();
+        const uint64_t next = row + gridDim.x;
+        if (next < rows) {
+            const V *src = reinterpret_cast<const V *>(in + (next << log_n));
+#pragma unroll
+            for (uint32_t i = 0; i < kPer; ++i)
+                if (i * 256 + t < elems) r[i] = __builtin_nontemporal_load(src + i * 256 + t);
+        }
+        V *dst = reinterpret_cast<V *>(out + (row << log_n));
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i) {
+            const uint32_t p = i * 256 + t;
+            if (p < elems) {
+                V v;
+                if constexpr (WIDE) {
+                    v.x = lds[auto_source(2 * p, log_n, k)];
+                    v.y = lds[auto_source(2 * p + 1, log_n, k)];
+                } else {
+                    v = lds[auto_source(p, log_n, k)];
+                }
+                __builtin_nontemporal_store(v, dst + p);
+            }
+        }
+        // every lane has read the row from LDS before the next one lands
+        __syncthreads();
+        if (next >= rows) break;
+        row = next;
+    }
+}
+
+// Rows longer than 2^12 words: 256 consecutive output slots per workgroup,
+// gathered from global memory, where the row's lines stay in L2 between their
+// reads.
 __global__ __launch_bounds__(256) void automorphism_kernel(const uint64_t *in, uint64_t *out, uint32_t log_n,
                                                            uint32_t k, uint64_t rows) {
-    __shared__ uint64_t lds[1u << 12];
     const uint32_t n = 1u << log_n;
-    const uint32_t t = threadIdx.x;
-    if (log_n <= 12) {
-        // one workgroup per row
-        const uint64_t row = blockIdx.x;
-        const uint64_t *src = in + row * n;
-        uint64_t *dst = out + row * n;
-        for (uint32_t d = t; d < n; d += 256) lds[d] = ld_stream(src + d);
-        __syncthreads();
-        for (uint32_t d = t; d < n; d += 256) st_stream(dst + d, lds[auto_source(d, log_n, k)]);
-        return;
-    }
-    // 2^12 < N: 256 consecutive output slots per workgroup
-    const uint64_t g = (uint64_t)blockIdx.x * 256u + t;
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (g >= (rows << log_n)) return;
     const uint64_t row = g >> log_n;
     const uint32_t d = (uint32_t)(g & (n - 1));
@@ -685,7 +737,26 @@ hipError_t launch_monomial(const DevTable *tabs, uint32_t log_n, uint64_t *out, 
 hipError_t launch_automorphism(const uint64_t *in, uint64_t *out, uint32_t log_n, uint32_t k, uint64_t rows,
                                hipStream_t s) {
     if (rows == 0) return hipSuccess;
-    const uint64_t blocks = log_n <= 12 ? rows : ((rows << log_n) + 255) / 256;
+    if (log_n == 0 || log_n > 17) return hipErrorInvalidValue;
+    if (log_n <= 12) {
+        // about 2^21 words of rows in flight at a time: 512 workgroups at
+        // N = 2^12 (two per CU), more for shorter rows
+        const uint64_t groups = std::min<uint64_t>(rows, 512ull << (12 - log_n));
+        const dim3 grid((uint32_t)groups), block(256);
+        const uint32_t lds = 8u << log_n;
+        if ((((uintptr_t)in | (uintptr_t)out) & 15) == 0) {
+            switch (log_n) {
+            case 12: automorphism_rows_kernel<true, 12><<<grid, block, lds, s>>>(in, out, log_n, k, rows); break;
+            case 11: automorphism_rows_kernel<true, 11><<<grid, block, lds, s>>>(in, out, log_n, k, rows); break;
+            case 10: automorphism_rows_kernel<true, 10><<<grid, block, lds, s>>>(in, out, log_n, k, rows); break;
+            default: automorphism_rows_kernel<true, 0><<<grid, block, lds, s>>>(in, out, log_n, k, rows);
+            }
+        } else {
+            automorphism_rows_kernel<false, 0><<<grid, block, lds, s>>>(in, out, log_n, k, rows);
+        }
+        return hipGetLastError();
+    }
+    const uint64_t blocks = ((rows << log_n) + 255) / 256;
     if (!grid_fits(blocks, 256)) return hipErrorInvalidConfiguration;
     automorphism_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(in, out, log_n, k, rows);
     return hipGetLastError();

@@ -534,6 +534,32 @@ def test_automorphism_device(log_n):
             np.testing.assert_array_equal(got[i], O.ntt_automorphism(x[i], k), err_msg="k=%d row %d" % (k, i))
 
 
+@pytest.mark.parametrize("log_n", [1, 10, 11, 12])
+@pytest.mark.parametrize("aligned", [True, False])
+def test_automorphism_device_many_rows(log_n, aligned):
+    """Batches with more rows than the persistent grid of the LDS-staged form
+    (so workgroups walk several rows, the last pass partial), 16-byte aligned
+    rows and rows offset by one word (8-byte accesses): every row equals the
+    oracle's permutation of it."""
+    n = 1 << log_n
+    rows = (512 << (12 - log_n)) + 37  # the launcher's grid for this degree, plus a partial pass
+    q = pa.ntt_primes_below(50, 2 * n, 1)[0]
+    tab = pa.UintCrtNttTable.new(log_n, [q])
+    rng = np.random.default_rng(log_n * 7 + aligned)
+    x = rng.integers(0, q, size=(rows, n), dtype=np.uint64)
+    off = 0 if aligned else 1
+    din = pa.DeviceBuffer.from_array(np.concatenate([np.zeros(off, np.uint64), x.ravel()]))
+    dout = pa.DeviceBuffer(rows * n + off)
+    for k in (5, 2 * n - 1):
+        perm = O.ntt_automorphism(np.arange(n, dtype=np.uint64), k).astype(np.int64)
+        pa.automorphism_device(tab, ctypes.c_void_p(din.ptr.value + 8 * off), ctypes.c_void_p(dout.ptr.value + 8 * off),
+                               rows, k)
+        pa.synchronize()
+        got = dout.download()[off:].reshape(rows, n)
+        np.testing.assert_array_equal(got, x[:, perm], err_msg="k=%d" % k)
+        np.testing.assert_array_equal(got[rows - 1], O.ntt_automorphism(x[rows - 1], k), err_msg="k=%d last row" % k)
+
+
 def test_automorphism_rejects():
     n = 1 << 10
     tab = pa.UintCrtNttTable.new(10, pa.ntt_primes_below(59, 2 * n, 1))
