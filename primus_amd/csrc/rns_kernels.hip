@@ -13,28 +13,34 @@
 
 namespace ph {
 
-template <bool EXACT>
+// IN / OUT > 0 fix the input and output limb counts at compile time (the limb
+// loops then unroll exactly, and the matrix offsets fold); 0 takes them from c.
+template <bool EXACT, uint32_t IN, uint32_t OUT>
 __global__ __launch_bounds__(256) void base_convert_kernel(BaseConvDev c, const uint64_t *in, uint64_t *out,
                                                            uint32_t log_n, uint64_t polys) {
+    static_assert(IN <= kMaxRnsModuli && OUT <= kMaxRnsModuli, "at most kMaxRnsModuli limbs");
+    constexpr uint32_t kIn = IN ? IN : kMaxRnsModuli;
+    const uint32_t in_count = IN ? IN : c.in_count;
+    const uint32_t out_count = OUT ? OUT : c.out_count;
     const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (g >= (polys << log_n)) return;
     const uint64_t n = 1ull << log_n;
     const uint64_t poly = g >> log_n, k = g & (n - 1);
-    const uint64_t *src = in + poly * c.in_count * n + k;
+    const uint64_t *src = in + poly * in_count * n + k;
     const auto q = as_global(c.q), q_r0 = as_global(c.q_r0), q_r1 = as_global(c.q_r1);
     const auto inv = as_global(c.inv), inv_q = as_global(c.inv_q);
     const auto qd = as_global(c.qd);
     const auto m = as_global(c.m);
-    uint64_t y[kMaxRnsModuli];
+    uint64_t y[kIn];
     double v = -0.0;
 #pragma unroll
-    for (uint32_t i = 0; i < kMaxRnsModuli; ++i)
-        if (i < c.in_count) y[i] = ld_stream(src + (uint64_t)i * n);  // every row load issued first
+    for (uint32_t i = 0; i < kIn; ++i)
+        if (i < in_count) y[i] = ld_stream(src + (uint64_t)i * n);  // every row load issued first
     // y_i = [x_i (Q/q_i)^-1]_{q_i}: fill_fast_convert_array_buffer
     // (primus_rns/src/converter.rs:103-146), a factor of one reducing x instead
 #pragma unroll
-    for (uint32_t i = 0; i < kMaxRnsModuli; ++i) {
-        if (i < c.in_count) {
+    for (uint32_t i = 0; i < kIn; ++i) {
+        if (i < in_count) {
             const uint64_t x = y[i];
             const uint64_t w = inv[i];
             y[i] = w == 1 ? barrett_reduce128(x, 0, q[i], q_r0[i], q_r1[i]) : shoup_full(w, inv_q[i], x, q[i]);
@@ -43,8 +49,8 @@ __global__ __launch_bounds__(256) void base_convert_kernel(BaseConvDev c, const 
             if constexpr (EXACT) v += (double)y[i] / qd[i];
         }
     }
-    uint64_t *dst = out + poly * c.out_count * n + k;
-    for (uint32_t j = 0; j < c.out_count; ++j) {
+    uint64_t *dst = out + poly * out_count * n + k;
+    for (uint32_t j = 0; j < out_count; ++j) {
         const uint64_t p = c.p[j], r0 = c.p_r0[j], r1 = c.p_r1[j];
         // reduce_dot_product: canonical sum_i y_i * ((Q/q_i) mod p_j).  The
         // products are summed in 128 bits, lazy_terms at a time, and each
@@ -52,9 +58,9 @@ __global__ __launch_bounds__(256) void base_convert_kernel(BaseConvDev c, const 
         uint64_t acc = 0, lo = 0, hi = 0;
         uint32_t terms = 0;
 #pragma unroll
-        for (uint32_t i = 0; i < kMaxRnsModuli; ++i) {
-            if (i < c.in_count) {
-                const uint64_t mm = m[(uint64_t)j * c.in_count + i];
+        for (uint32_t i = 0; i < kIn; ++i) {
+            if (i < in_count) {
+                const uint64_t mm = m[(uint64_t)j * in_count + i];
                 const uint64_t plo = y[i] * mm, phi = mulhi64(y[i], mm);
                 lo += plo;
                 hi += phi + (lo < plo);
@@ -157,8 +163,16 @@ hipError_t launch_base_convert(const BaseConvDev &c, const uint64_t *in, uint64_
         if (exact) base_convert_f64_kernel<true><<<grid, block, 0, s>>>(c, in, out, log_n, polys);
         else base_convert_f64_kernel<false><<<grid, block, 0, s>>>(c, in, out, log_n, polys);
     } else {
-        if (exact) base_convert_kernel<true><<<grid, block, 0, s>>>(c, in, out, log_n, polys);
-        else base_convert_kernel<false><<<grid, block, 0, s>>>(c, in, out, log_n, polys);
+        // the shapes of a 4-limb ciphertext modulus: fast 4 -> 4 (modulus
+        // switching, key switching's extension) and exact 4 -> 1
+        if (exact && c.in_count == 4 && c.out_count == 1)
+            base_convert_kernel<true, 4, 1><<<grid, block, 0, s>>>(c, in, out, log_n, polys);
+        else if (exact)
+            base_convert_kernel<true, 0, 0><<<grid, block, 0, s>>>(c, in, out, log_n, polys);
+        else if (c.in_count == 4 && c.out_count == 4)
+            base_convert_kernel<false, 4, 4><<<grid, block, 0, s>>>(c, in, out, log_n, polys);
+        else
+            base_convert_kernel<false, 0, 0><<<grid, block, 0, s>>>(c, in, out, log_n, polys);
     }
     return hipGetLastError();
 }
